@@ -1,0 +1,343 @@
+"""Python view of libkacomp's C ABI (include/kacomp.h), used by the tests and
+bench.py.  The compute path is the HIP library; this module only moves arrays
+across the boundary with ctypes.  Loading fails loudly when the library has
+not been built: there is no fallback.
+
+The class ``ExponentialContraction`` mirrors the reference's component API
+(lib/components/exponential_contraction.h:42-60, 325-335):
+``ExponentialContraction(conf, rank, size).FindComponents(graph, labels)``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libkacomp.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+KC_OK = 0
+KC_ERR_INVALID = -1
+KC_ERR_HIP = -2
+KC_ERR_OOM = -3
+KC_ERR_IO = -4
+KC_ERR_UNSUPPORTED = -5
+
+VARIANT_EXP, VARIANT_LOCAL, VARIANT_SHORTCUT, VARIANT_LABELS = 0, 1, 2, 3
+VARIANTS = {"exp": 0, "local": 1, "shortcut": 2, "labels": 3}
+
+KERNEL_CLASSES = ["deviates", "sweep", "bin", "count", "scan", "scatter", "dedup", "csr", "undo", "canon",
+                  "jump", "other"]
+
+
+class CsrHost(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("offsets", ctypes.POINTER(ctypes.c_uint64)),
+                ("targets", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class Csr(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("m", ctypes.c_uint64), ("offsets", ctypes.POINTER(ctypes.c_uint64)),
+                ("targets", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("seed", ctypes.c_int32), ("beta", ctypes.c_double),
+                ("seq_limit", ctypes.c_uint64), ("use_bfs", ctypes.c_int32), ("deviate_fma", ctypes.c_int32),
+                ("learned", ctypes.c_int32), ("record_trace", ctypes.c_int32), ("time_kernels", ctypes.c_int32),
+                ("max_levels", ctypes.c_int32), ("sweep_group_small", ctypes.c_int32),
+                ("sweep_group_mid", ctypes.c_int32), ("sweep_group_large", ctypes.c_int32),
+                ("deviate_stream", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("time_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("comm_time_s", ctypes.c_double),
+                ("send_volume", ctypes.c_uint64), ("recv_volume", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("m", ctypes.c_uint64), ("levels", ctypes.c_uint32), ("components", ctypes.c_uint32),
+                ("sweeps", ctypes.c_uint64), ("radius_sum", ctypes.c_uint64), ("alg_bytes", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_double * 12), ("kernel_launches", ctypes.c_uint64 * 12),
+                ("kernel_bytes", ctypes.c_uint64 * 12)]
+
+
+class GenParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("m", ctypes.c_uint64), ("r", ctypes.c_double), ("p", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("d", ctypes.c_double), ("periodic", ctypes.c_int32),
+                ("seed", ctypes.c_int32), ("rmat_a", ctypes.c_double), ("rmat_b", ctypes.c_double),
+                ("rmat_c", ctypes.c_double), ("nx", ctypes.c_uint64), ("ny", ctypes.c_uint64),
+                ("nz", ctypes.c_uint64)]
+
+
+# Every symbol include/kacomp.h declares, with (restype, argtypes).
+_P = ctypes.POINTER
+_SIGNATURES = {
+    "kc_last_error": (ctypes.c_char_p, []),
+    "kc_options_init": (None, [_P(Options)]),
+    "kc_graph_create": (ctypes.c_int, [_P(CsrHost), _P(ctypes.c_void_p)]),
+    "kc_graph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "kc_graph_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
+    "kc_find_components": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(ctypes.c_uint32), _P(Stats)]),
+    "kc_copy_labels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
+    "kc_trace_levels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
+    "kc_trace_level_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint64)]),
+    "kc_trace_level_keys": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint64), _P(ctypes.c_uint32)]),
+    "kc_deviates": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64,
+                                   ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_uint32)]),
+    "kc_read_metis": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
+    "kc_write_metis": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
+    "kc_gen_params_init": (None, [_P(GenParams)]),
+    "kc_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(Csr)]),
+    "kc_csr_free": (None, [_P(Csr)]),
+}
+
+_lib = None
+
+
+class KcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libkacomp error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Loads libkacomp.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C components_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != KC_OK:
+        raise KcError(rc, lib().kc_last_error().decode())
+
+
+def _u64p(a):
+    return a.ctypes.data_as(_P(ctypes.c_uint64))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(_P(ctypes.c_uint32))
+
+
+@dataclass
+class Graph:
+    """Host CSR: offsets u64[n+1], targets u32[2m]."""
+    offsets: np.ndarray
+    targets: np.ndarray
+
+    @property
+    def n(self):
+        return len(self.offsets) - 1
+
+    @property
+    def m(self):
+        return int(self.offsets[-1]) // 2
+
+    @staticmethod
+    def from_edges(n, edges):
+        """Symmetric CSR (no self-loops, no duplicates) from an (k,2) edge array."""
+        e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+        e = e[e[:, 0] != e[:, 1]]
+        both = np.concatenate([e, e[:, ::-1]]) if len(e) else np.zeros((0, 2), np.int64)
+        key = np.unique(both[:, 0] * (n + 1) + both[:, 1]) if len(both) else np.zeros(0, np.int64)
+        src = key // (n + 1)
+        dst = key % (n + 1)
+        off = np.zeros(n + 1, np.uint64)
+        np.add.at(off, src + 1, 1)
+        off = np.cumsum(off, dtype=np.uint64)
+        return Graph(off, dst.astype(np.uint32))
+
+
+def _take_csr(c):
+    n = int(c.n)
+    off = np.ctypeslib.as_array(c.offsets, shape=(n + 1,)).copy()
+    m2 = int(off[-1])
+    tgt = np.ctypeslib.as_array(c.targets, shape=(max(m2, 1),))[:m2].copy() if m2 else np.zeros(0, np.uint32)
+    lib().kc_csr_free(ctypes.byref(c))
+    return Graph(off, tgt)
+
+
+def gen_params(**kw):
+    p = GenParams()
+    lib().kc_gen_params_init(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def generate(name, **kw):
+    """kc_generate with KaGen-style parameters (n, m, r, p, gamma, d, periodic, seed, rmat_*, nx, ny, nz)."""
+    c = Csr()
+    _check(lib().kc_generate(name.encode(), ctypes.byref(gen_params(**kw)), ctypes.byref(c)))
+    return _take_csr(c)
+
+
+def read_metis(path):
+    c = Csr()
+    _check(lib().kc_read_metis(str(path).encode(), ctypes.byref(c)))
+    return _take_csr(c)
+
+
+def write_metis(path, g):
+    off = np.ascontiguousarray(g.offsets, np.uint64)
+    tgt = np.ascontiguousarray(g.targets, np.uint32)
+    c = Csr(g.n, g.m, _u64p(off), _u32p(tgt) if len(tgt) else _P(ctypes.c_uint32)())
+    _check(lib().kc_write_metis(str(path).encode(), ctypes.byref(c)))
+
+
+STREAM_REFERENCE, STREAM_HASHED = 0, 1
+
+
+def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
+            tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE):
+    o = Options()
+    lib().kc_options_init(ctypes.byref(o))
+    o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    o.seed = seed
+    o.beta = beta
+    o.seq_limit = seq_limit
+    o.deviate_fma = 1 if fma else 0
+    o.record_trace = 1 if record_trace else 0
+    o.time_kernels = 1 if time_kernels else 0
+    o.sweep_group_small, o.sweep_group_mid, o.sweep_group_large = tmax, smax, wmax
+    o.use_bfs = 1 if use_bfs else 0
+    o.max_levels = max_levels
+    o.deviate_stream = stream
+    return o
+
+
+def stats_dict(st):
+    d = {k: getattr(st, k) for k, _ in Stats._fields_ if not k.startswith("kernel_")}
+    d["kernel_ms"] = dict(zip(KERNEL_CLASSES, list(st.kernel_ms)))
+    d["kernel_launches"] = dict(zip(KERNEL_CLASSES, list(st.kernel_launches)))
+    d["kernel_bytes"] = dict(zip(KERNEL_CLASSES, list(st.kernel_bytes)))
+    return d
+
+
+class DeviceGraph:
+    """kc_graph handle: the CSR copied once to HBM."""
+
+    def __init__(self, g: Graph):
+        self._off = np.ascontiguousarray(g.offsets, np.uint64)
+        self._tgt = np.ascontiguousarray(g.targets, np.uint32)
+        if len(self._tgt) == 0:
+            self._tgt = np.zeros(1, np.uint32)
+        h = CsrHost(g.n, _u64p(self._off), _u32p(self._tgt))
+        self.h = ctypes.c_void_p()
+        _check(lib().kc_graph_create(ctypes.byref(h), ctypes.byref(self.h)))
+        self.n = g.n
+
+    def close(self):
+        if self.h:
+            lib().kc_graph_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def find_components(self, copy_labels=True, **kw):
+        o = options(**kw)
+        st = Stats()
+        labels = np.empty(max(self.n, 1), np.uint32)
+        _check(lib().kc_find_components(self.h, ctypes.byref(o), _u32p(labels) if copy_labels else None,
+                                        ctypes.byref(st)))
+        return (labels[: self.n] if copy_labels else None), stats_dict(st)
+
+    def labels(self):
+        labels = np.empty(max(self.n, 1), np.uint32)
+        _check(lib().kc_copy_labels(self.h, _u32p(labels)))
+        return labels[: self.n]
+
+    def trace(self):
+        """Per-level (info, keys, gid) of the last run with record_trace=True."""
+        nl = ctypes.c_uint32()
+        _check(lib().kc_trace_levels(self.h, ctypes.byref(nl)))
+        out = []
+        for lvl in range(nl.value):
+            info = (ctypes.c_uint64 * 4)()
+            _check(lib().kc_trace_level_info(self.h, lvl, info))
+            n = int(info[0])
+            keys = np.empty(max(n, 1), np.uint64)
+            gid = np.empty(max(n, 1), np.uint32)
+            _check(lib().kc_trace_level_keys(self.h, lvl, _u64p(keys), _u32p(gid)))
+            out.append((tuple(int(x) for x in info), keys[:n], gid[:n]))
+        return out
+
+
+def deviates(gid0, count, seed, iteration, rng_offset, beta=0.1, fma=True, stream=STREAM_REFERENCE):
+    out = np.empty(max(count, 1), np.uint32)
+    _check(lib().kc_deviates(gid0, count, seed, iteration, rng_offset, beta, 1 if fma else 0, stream, _u32p(out)))
+    return out[:count]
+
+
+def find_components(g: Graph, **kw):
+    dg = DeviceGraph(g)
+    try:
+        return dg.find_components(**kw)
+    finally:
+        dg.close()
+
+
+class Config:
+    """Subset of the reference's Config (lib/config.h:29-87) the components read."""
+
+    def __init__(self, seed=1, beta=0.1, sequential_limit=1, fixed_limit=False, use_bfs=False):
+        self.seed = seed
+        self.beta = beta
+        self.sequential_limit = sequential_limit
+        self.fixed_limit = fixed_limit
+        self.use_bfs = use_bfs
+
+
+class _Component:
+    variant = "exp"
+
+    def __init__(self, conf: Config, rank: int = 0, size: int = 1):
+        if size != 1:
+            raise ValueError("one GPU holds the whole graph: size must be 1")
+        self.conf = conf
+        self.stats = None
+
+    def FindComponents(self, g: Graph, labels: np.ndarray):
+        c = self.conf
+        seq = c.sequential_limit if c.fixed_limit else max((g.m // 256) * c.sequential_limit, 4096)
+        out, self.stats = find_components(g, variant=self.variant, seed=c.seed, beta=c.beta, seq_limit=seq,
+                                          use_bfs=c.use_bfs)
+        labels[: g.n] = out
+
+    def GetCommTime(self):
+        return 0.0
+
+    def GetSendVolume(self):
+        return 0
+
+    def GetReceiveVolume(self):
+        return 0
+
+
+class ExponentialContraction(_Component):
+    variant = "exp"
+
+
+class LocalContraction(_Component):
+    variant = "local"
+
+
+class ShortcutPropagation(_Component):
+    variant = "shortcut"
+
+
+class Propagation(_Component):
+    variant = "labels"

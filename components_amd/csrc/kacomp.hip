@@ -1,0 +1,841 @@
+// libkacomp C ABI: graph handle, level loop of exponential contraction and its
+// variants, undo and canonical labels.  One HIP stream per graph handle; every
+// call is synchronous on return.
+//
+// Level loop (one GPU, the P -> n limit of the reference where every vertex is
+// an interface vertex, so the contraction runs on the whole graph):
+//   RunContractionLP           lib/components/exponential_contraction.h:439-628
+//   ExponentialContraction     lib/contraction/dynamic_contraction.h:62-136
+//   UndoContraction            lib/contraction/dynamic_contraction.h:568-658
+//   ApplyToLocalComponents     lib/components/exponential_contraction.h:876-889
+// Ghost exchange (lib/communication) has no counterpart: neighbours read the
+// key array directly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "deviates.h"
+#include "kc_common.h"
+#include "kernels.h"
+
+namespace kc {
+
+// ---------------------------------------------------------------------------
+// Errors.
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+void fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  throw Failure{code};
+}
+
+// ---------------------------------------------------------------------------
+// Arena.
+Arena::~Arena() {
+  for (auto &c : chunks_) (void)hipFree(c.first);
+}
+
+void *Arena::alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  while (cur_ < chunks_.size()) {
+    if (used_ + bytes <= chunks_[cur_].second) {
+      void *p = chunks_[cur_].first + used_;
+      used_ += bytes;
+      return p;
+    }
+    ++cur_;
+    used_ = 0;
+  }
+  size_t sz = std::max<size_t>(bytes, size_t(64) << 20);
+  char *p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) fail(KC_ERR_OOM, "device arena: cannot allocate %zu bytes", sz);
+  chunks_.emplace_back(p, sz);
+  cur_ = chunks_.size() - 1;
+  used_ = bytes;
+  return p;
+}
+
+size_t Arena::reserved() const {
+  size_t s = 0;
+  for (auto &c : chunks_) s += c.second;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timer.
+KernelTimer::~KernelTimer() {
+  for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+void KernelTimer::begin_run(bool enabled, hipStream_t s) {
+  enabled_ = enabled;
+  stream_ = s;
+  recs_.clear();
+  next_ev_ = 0;
+}
+
+int KernelTimer::start(int cls) {
+  int ev = -1;
+  if (enabled_) {
+    while (next_ev_ + 2 > pool_.size()) {
+      hipEvent_t e;
+      KC_HIP(hipEventCreate(&e));
+      pool_.push_back(e);
+    }
+    ev = static_cast<int>(next_ev_);
+    next_ev_ += 2;
+    KC_HIP(hipEventRecord(pool_[ev], stream_));
+  }
+  recs_.push_back({cls, ev, 0});
+  return static_cast<int>(recs_.size() - 1);
+}
+
+void KernelTimer::stop(int slot, uint64_t alg_bytes) {
+  Rec &r = recs_[slot];
+  r.bytes += alg_bytes;
+  if (r.ev >= 0) KC_HIP(hipEventRecord(pool_[r.ev + 1], stream_));
+}
+
+void KernelTimer::retag(int slot, int cls, uint64_t alg_bytes) {
+  recs_[slot].cls = cls;
+  recs_[slot].bytes = alg_bytes;
+}
+
+void KernelTimer::finish(kc_stats *st) {
+  if (!st) return;
+  for (int i = 0; i < KC_KERNEL_CLASSES; ++i) {
+    st->kernel_ms[i] = 0.0;
+    st->kernel_launches[i] = 0;
+    st->kernel_bytes[i] = 0;
+  }
+  if (enabled_) KC_HIP(hipStreamSynchronize(stream_));
+  for (auto &r : recs_) {
+    st->kernel_launches[r.cls]++;
+    st->kernel_bytes[r.cls] += r.bytes;
+    if (r.ev >= 0) {
+      float ms = 0.f;
+      KC_HIP(hipEventElapsedTime(&ms, pool_[r.ev], pool_[r.ev + 1]));
+      st->kernel_ms[r.cls] += ms;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deviate threshold table (host, glibc log as in the reference).
+static uint64_t truncated_deviate(double w, double beta) {
+  return static_cast<uint64_t>(-std::log(w) / beta);
+}
+
+static std::vector<double> build_deviate_table(double beta) {
+  const double wmin = 0x1p-53;  // smallest 1-u the canonical draw can produce
+  const uint64_t kmax = truncated_deviate(wmin, beta);
+  std::vector<double> t(kmax);
+  uint64_t lo_bits, hi_bits;
+  for (uint64_t k = 1; k <= kmax; ++k) {
+    double lo = wmin, hi = 1.0;  // F(lo) >= k, F(hi) = 0 < k
+    std::memcpy(&lo_bits, &lo, 8);
+    std::memcpy(&hi_bits, &hi, 8);
+    while (hi_bits - lo_bits > 1) {
+      uint64_t mid = lo_bits + (hi_bits - lo_bits) / 2;
+      double w;
+      std::memcpy(&w, &mid, 8);
+      if (truncated_deviate(w, beta) >= k)
+        lo_bits = mid;
+      else
+        hi_bits = mid;
+    }
+    std::memcpy(&t[k - 1], &lo_bits, 8);
+  }
+  return t;
+}
+
+}  // namespace kc
+
+using namespace kc;
+
+// ---------------------------------------------------------------------------
+// Graph handle.
+struct LevelRec {
+  bool stall = false;  // no key changed: the contraction is the identity and is skipped
+  uint64_t n = 0, m2 = 0;
+  uint32_t *lab = nullptr, *newidx = nullptr, *gid = nullptr;
+  uint64_t sweeps = 0;
+  uint32_t radius = 0;
+  uint64_t *keys_copy = nullptr;  // trace only
+};
+
+struct kc_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t n = 0, m2 = 0, non_isolated = 0;
+  // level-0 CSR
+  uint64_t *off0 = nullptr;
+  uint32_t *col0 = nullptr;
+  // work buffers sized for level 0
+  uint64_t *key = nullptr, *key2 = nullptr;
+  uint32_t *deg_inter = nullptr, *lists = nullptr, *hub_inter = nullptr;
+  HubChunk *hubs = nullptr;
+  uint64_t hubs_cap = 0;
+  uint32_t *cnt = nullptr, *fill = nullptr, *bucket = nullptr, *old_of = nullptr, *uniq = nullptr;
+  uint64_t *bucket_off = nullptr;
+  uint32_t *rep_a = nullptr, *rep_b = nullptr, *minid = nullptr, *labels = nullptr;
+  uint64_t *scan_scratch = nullptr;
+  uint64_t *off_pp[2] = {nullptr, nullptr};
+  uint32_t *col_pp[2] = {nullptr, nullptr};
+  uint32_t *dl_medium = nullptr, *dl_huge = nullptr, *hcount = nullptr;
+  uint64_t *dl_huge_tab = nullptr;
+  uint32_t *huge_tables = nullptr;
+  uint64_t huge_cap = 0;
+  int *flags = nullptr;
+  uint32_t *small = nullptr;  // [0..3] bin counters, [4] radius, [5] roots
+  uint64_t *sizes = nullptr;  // [0] n_next [1] P [2] m2_next [3..5] dedup counters
+  // pinned host mirrors
+  uint64_t *h_sizes = nullptr;
+  uint32_t *h_small = nullptr;
+  int *h_flags = nullptr;
+  // deviate table
+  double *d_table = nullptr;
+  uint32_t table_k = 0;
+  double table_beta = -1.0;
+  uint64_t table_cap = 0;
+  // run state
+  Arena arena;
+  KernelTimer timer;
+  std::vector<LevelRec> levels;
+  bool base_case_used = false;
+  uint64_t base_case_rounds = 0;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  uint32_t components = 0;
+  bool have_labels = false;
+  std::vector<void *> owned;
+
+  template <class T>
+  T *dalloc(uint64_t count) {
+    void *p = nullptr;
+    size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T) + 64;
+    if (hipMalloc(&p, bytes) != hipSuccess) fail(KC_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+    owned.push_back(p);
+    return static_cast<T *>(p);
+  }
+  ~kc_graph() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : owned) (void)hipFree(p);
+    if (h_sizes) (void)hipHostFree(h_sizes);
+    if (h_small) (void)hipHostFree(h_small);
+    if (h_flags) (void)hipHostFree(h_flags);
+    if (ev_begin) (void)hipEventDestroy(ev_begin);
+    if (ev_end) (void)hipEventDestroy(ev_end);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+constexpr uint32_t kMaxFlags = 1u << 16;
+constexpr uint32_t kMinWaveBin = 256;
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return KC_OK;
+  } catch (const Failure &e) {
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return KC_ERR_OOM;
+  } catch (...) {
+    set_error("unexpected exception");
+    return KC_ERR_INVALID;
+  }
+}
+
+void sync(kc_graph *g) { KC_HIP(hipStreamSynchronize(g->stream)); }
+
+void d2h(kc_graph *g, void *dst, const void *src, size_t bytes) {
+  KC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream));
+}
+
+void ensure_table(kc_graph *g, double beta) {
+  if (g->table_beta == beta && g->d_table) return;
+  if (!(beta > 1e-6) || !(beta < 1e6)) fail(KC_ERR_INVALID, "beta must be in (1e-6, 1e6), got %g", beta);
+  std::vector<double> t = build_deviate_table(beta);
+  if (t.size() + 1 > g->table_cap) {
+    g->table_cap = t.size() + 1;
+    g->d_table = g->dalloc<double>(g->table_cap);
+  }
+  if (!t.empty())
+    KC_HIP(hipMemcpy(g->d_table, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  g->table_k = static_cast<uint32_t>(t.size());
+  g->table_beta = beta;
+}
+
+BinCfg bin_cfg(const kc_options *o) {
+  BinCfg c;
+  c.tmax = o->sweep_group_small > 0 ? static_cast<uint32_t>(o->sweep_group_small) : 16u;
+  c.smax = o->sweep_group_mid > 0 ? static_cast<uint32_t>(o->sweep_group_mid) : 256u;
+  c.wmax = o->sweep_group_large > 0 ? static_cast<uint32_t>(o->sweep_group_large) : 4096u;
+  if (c.wmax < kMinWaveBin) c.wmax = kMinWaveBin;
+  if (c.smax > c.wmax) c.smax = c.wmax;
+  if (c.tmax > c.smax) c.tmax = c.smax;
+  return c;
+}
+
+// Bins of the level CSR (one sync to size the binned launches).
+Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg) {
+  Bins b;
+  b.listT = g->lists;
+  b.listS = g->lists + g->n;
+  b.listW = g->lists + 2 * g->n;
+  b.hubs = g->hubs;
+  fill_u32(g->small, 4, 0, g->stream);
+  {
+    TimedScope ts(g->timer, KC_K_BIN, 16 * n + 8);
+    bin_vertices(n, off, cfg, b.listT, b.listS, b.listW, b.hubs, g->small, g->deg_inter, g->stream);
+  }
+  d2h(g, g->h_small, g->small, 4 * sizeof(uint32_t));
+  sync(g);
+  b.nT = g->h_small[0];
+  b.nS = g->h_small[1];
+  b.nW = g->h_small[2];
+  b.nH = g->h_small[3];
+  if (b.nH > g->hubs_cap) fail(KC_ERR_INVALID, "hub chunk overflow (%u > %llu)", b.nH, (unsigned long long)g->hubs_cap);
+  finish_bins(b);
+  return b;
+}
+
+uint64_t sweep_bytes(uint64_t n, uint64_t m2, bool count) {
+  // offsets, neighbour ids, neighbour keys, own key read + write, inter count
+  return 8 * (n + 1) + 12 * m2 + 16 * n + (count ? 4 * n : 0);
+}
+
+// Runs in-place sweeps (and optionally pointer jumps) until no key changes.
+// Sweeps are launched in speculative batches; a sweep whose predecessor
+// changed nothing returns at once.  Returns the number of sweeps that did work
+// (the last of them found no change).
+uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
+                           uint64_t *key, uint64_t hop, const Bins &b, bool count, bool with_jump,
+                           uint32_t &flag_base) {
+  uint64_t done = 0;
+  const uint32_t batch = n < (1u << 16) ? 8u : 3u;
+  const uint64_t bytes = sweep_bytes(n, m2, count);
+  while (true) {
+    if (flag_base + batch + 2 >= kMaxFlags) {
+      fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
+      flag_base = 0;
+    }
+    int slots[8];
+    for (uint32_t i = 0; i < batch; ++i) {
+      const uint32_t f = flag_base + i;
+      // the first sweep of a batch always runs: the previous batch ended with a change
+      const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
+      slots[i] = g->timer.start(KC_K_SWEEP);
+      sweep(kSweepInPlace, count, off, col, key, key, hop, b, g->deg_inter, g->hub_inter, &g->flags[f], prev,
+            g->stream);
+      g->timer.stop(slots[i], bytes);
+      if (with_jump) {
+        TimedScope js(g->timer, KC_K_JUMP, 16 * n);
+        jump(n, key, &g->flags[f], g->stream);
+      }
+    }
+    d2h(g, g->h_flags, &g->flags[flag_base], batch * sizeof(int));
+    sync(g);
+    uint32_t i = 0;
+    while (i < batch && g->h_flags[i] != 0) ++i;
+    const uint32_t worked = i < batch ? i + 1 : batch;
+    for (uint32_t j = worked; j < batch; ++j) g->timer.retag(slots[j], KC_K_OTHER, 0);
+    done += worked;
+    flag_base += batch;
+    if (i < batch) break;
+  }
+  return done;
+}
+
+struct RunOut {
+  uint64_t sweeps = 0, radius_sum = 0;
+  uint32_t levels = 0;
+};
+
+// Exponential (or local) contraction down to an edgeless graph or the base case.
+void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
+  const bool local = o->variant == KC_VARIANT_LOCAL;
+  const BinCfg cfg = bin_cfg(o);
+  uint64_t n_l = g->n, m2_l = g->m2;
+  const uint64_t *off_l = g->off0;
+  const uint32_t *col_l = g->col0;
+  uint32_t *gid_l = nullptr;
+  uint32_t iteration = 1;
+  uint32_t flag_base = 0;
+  int pp = 0;
+  fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
+  g->levels.clear();
+  g->base_case_used = false;
+  g->base_case_rounds = 0;
+  const int max_levels = o->max_levels > 0 ? o->max_levels : 4096;
+  DeviateArgs da{o->seed, 0, g->non_isolated, o->deviate_fma, o->deviate_stream, g->d_table, g->table_k};
+  bool top_from_keys = false;  // top rep computed by the base case into rep of that level
+  uint32_t *top_rep = nullptr;
+  Bins b;
+  bool bins_valid = false;  // bins of the current level graph (kept across stall levels)
+
+  while (n_l > 0) {
+    if (m2_l == 0 || (o->seq_limit > 0 && m2_l / 2 <= o->seq_limit)) {
+      // Base case (RunSequentialCC, exponential_contraction.h:908-922): min-label
+      // propagation with pointer jumping on the remaining graph.
+      if (!bins_valid) b = make_bins(g, n_l, off_l, cfg);
+      {
+        TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l);
+        init_keys_index(n_l, g->key, g->stream);
+      }
+      uint64_t r = 0;
+      if (m2_l > 0) r = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, 0, b, false, true, flag_base);
+      g->base_case_used = m2_l > 0;
+      g->base_case_rounds = r;
+      out.sweeps += r;
+      top_rep = g->rep_a;
+      {
+        TimedScope ts(g->timer, KC_K_UNDO, 16 * n_l);
+        rep_from_keys(n_l, g->key, gid_l, top_rep, g->stream);
+      }
+      top_from_keys = true;
+      break;
+    }
+    if (static_cast<int>(g->levels.size()) >= max_levels)
+      fail(KC_ERR_INVALID, "contraction did not finish within %d levels", max_levels);
+
+    if (!bins_valid) {
+      b = make_bins(g, n_l, off_l, cfg);
+      bins_valid = true;
+    }
+    // deviates -> keys
+    {
+      TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l + (gid_l ? 4 * n_l : 0));
+      if (local) {
+        init_keys_uniform(n_l, gid_l, o->seed, iteration, g->key, g->stream);
+      } else {
+        da.iteration = iteration;
+        init_keys_exp(n_l, gid_l, da, LearnedArgs{nullptr}, g->key, g->stream);
+      }
+    }
+    uint64_t sweeps = 0;
+    if (local) {
+      // two synchronous min passes over the closed neighbourhood
+      // (local_contraction.h:195-274), then a counting pass on the final keys
+      for (int pass = 0; pass < 2; ++pass) {
+        uint64_t *kin = pass == 0 ? g->key : g->key2;
+        uint64_t *kout = pass == 0 ? g->key2 : g->key;
+        KC_HIP(hipMemcpyAsync(kout, kin, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
+        TimedScope ts(g->timer, KC_K_SWEEP, sweep_bytes(n_l, m2_l, false));
+        sweep(kSweepJacobi, false, off_l, col_l, kin, kout, 0, b, g->deg_inter, g->hub_inter,
+              &g->flags[kMaxFlags - 1], nullptr, g->stream);
+      }
+      {
+        TimedScope ts(g->timer, KC_K_COUNT, sweep_bytes(n_l, m2_l, true) - 8 * n_l);
+        sweep(kSweepCountOnly, true, off_l, col_l, g->key, g->key, 0, b, g->deg_inter, g->hub_inter,
+              &g->flags[kMaxFlags - 1], nullptr, g->stream);
+      }
+      sweeps = 2;
+    } else {
+      sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, true, false, flag_base);
+    }
+    if (!local && gid_l && sweeps == 1) {
+      // Stall: the first sweep changed no key, so every vertex is its own
+      // centre and keeps an edge; the contracted graph equals this one.  The
+      // level is recorded and the next iteration draws new deviates on the
+      // same graph (level >= 1 graphs have no isolated vertices).
+      LevelRec rec;
+      rec.stall = true;
+      rec.n = n_l;
+      rec.m2 = m2_l;
+      rec.gid = gid_l;
+      rec.sweeps = 1;
+      if (o->record_trace) {
+        rec.keys_copy = g->arena.alloc_n<uint64_t>(n_l);
+        KC_HIP(hipMemcpyAsync(rec.keys_copy, g->key, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
+      }
+      out.sweeps += 1;
+      g->levels.push_back(rec);
+      ++iteration;
+      continue;
+    }
+    if (b.nH) {
+      TimedScope ts(g->timer, KC_K_COUNT, 8ull * b.nH + 4ull * b.nH);
+      hub_fold(b, g->hub_inter, g->deg_inter, g->stream);
+    }
+    LevelRec rec;
+    rec.n = n_l;
+    rec.m2 = m2_l;
+    rec.gid = gid_l;
+    rec.sweeps = sweeps;
+    rec.lab = g->arena.alloc_n<uint32_t>(n_l);
+    rec.newidx = g->arena.alloc_n<uint32_t>(n_l);
+    uint32_t *gid_next = g->arena.alloc_n<uint32_t>(n_l);
+    if (o->record_trace) {
+      rec.keys_copy = g->arena.alloc_n<uint64_t>(n_l);
+      KC_HIP(hipMemcpyAsync(rec.keys_copy, g->key, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
+    }
+    fill_u32(g->cnt, n_l, 0, g->stream);
+    fill_u32(g->small + 4, 1, 0, g->stream);
+    {
+      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + 4 * n_l + 8 * n_l);
+      cluster_count(n_l, g->key, g->deg_inter, rec.lab, g->cnt, g->small + 4, !local, g->stream);
+    }
+    {
+      TimedScope ts(g->timer, KC_K_SCAN, 4 * n_l + 8 * n_l + 4 * n_l + 12 * n_l);
+      kept_scan(n_l, g->cnt, gid_l, rec.newidx, g->bucket_off, gid_next, g->old_of, g->sizes, g->scan_scratch,
+                g->stream);
+    }
+    d2h(g, g->h_sizes, g->sizes, 2 * sizeof(uint64_t));
+    d2h(g, g->h_small + 4, g->small + 4, sizeof(uint32_t));
+    sync(g);
+    const uint64_t n_next = g->h_sizes[0];
+    const uint64_t P = g->h_sizes[1];
+    rec.radius = local ? 0 : g->h_small[4];
+    out.sweeps += sweeps;
+    out.radius_sum += rec.radius;
+    g->levels.push_back(rec);
+    if (n_next == 0) break;  // every cluster is a whole component
+
+    // relabel: scatter cluster edges into the buckets of their centres
+    fill_u32(g->fill, n_l, 0, g->stream);
+    {
+      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 12 * m2_l + 4 * n_l + 8 * P + 8 * n_l);
+      scatter(off_l, col_l, g->key, b, g->deg_inter, g->hub_inter, rec.newidx, g->bucket_off, g->fill, g->bucket,
+              g->stream);
+    }
+    // dedup of every bucket
+    fill_u64(g->sizes + 3, 3, 0, g->stream);
+    DedupLists dl{g->dl_medium, g->dl_huge, g->dl_huge_tab, g->sizes + 3};
+    {
+      TimedScope ts(g->timer, KC_K_DEDUP, 8 * P);
+      dedup_small(n_next, g->old_of, g->cnt, g->bucket_off, g->bucket, g->uniq, dl, g->stream);
+    }
+    d2h(g, g->h_sizes + 3, g->sizes + 3, 3 * sizeof(uint64_t));
+    sync(g);
+    const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
+    if (n_medium) {
+      TimedScope ts(g->timer, KC_K_DEDUP, 0);
+      dedup_medium(static_cast<uint32_t>(n_medium), g->dl_medium, g->old_of, g->cnt, g->bucket_off, g->bucket, g->uniq,
+                   g->stream);
+    }
+    if (n_huge) {
+      if (huge_slots > g->huge_cap) {
+        g->huge_cap = huge_slots + (huge_slots >> 2);
+        g->huge_tables = g->dalloc<uint32_t>(g->huge_cap);
+      }
+      fill_u32(g->huge_tables, huge_slots, kNone, g->stream);
+      fill_u32(g->hcount, n_huge, 0, g->stream);
+      TimedScope ts(g->timer, KC_K_DEDUP, 0);
+      dedup_huge(static_cast<uint32_t>(n_huge), g->dl_huge, g->dl_huge_tab, g->huge_tables, g->old_of, g->cnt,
+                 g->bucket_off, g->bucket, g->uniq, g->hcount, g->stream);
+    }
+    // next-level CSR
+    uint64_t *off_next = g->off_pp[pp];
+    uint32_t *col_next = g->col_pp[pp];
+    {
+      TimedScope ts(g->timer, KC_K_SCAN, 4 * n_next + 8 * (n_next + 1));
+      csr_offsets(n_next, g->uniq, off_next, g->sizes, g->scan_scratch, g->stream);
+    }
+    d2h(g, g->h_sizes + 2, g->sizes + 2, sizeof(uint64_t));
+    sync(g);
+    const uint64_t m2_next = g->h_sizes[2];
+    {
+      TimedScope ts(g->timer, KC_K_CSR, 4 * P + 4 * m2_next);
+      compact(P, g->bucket, col_next, g->scan_scratch, g->stream);
+    }
+    n_l = n_next;
+    m2_l = m2_next;
+    off_l = off_next;
+    col_l = col_next;
+    gid_l = gid_next;
+    pp ^= 1;
+    bins_valid = false;
+    ++iteration;
+  }
+  out.levels = static_cast<uint32_t>(g->levels.size());
+
+  // Undo: top-down, rep_l[v] = rep_{l+1}[newidx[lab[v]]] or gid[lab[v]].
+  uint32_t *rep_up = top_from_keys ? top_rep : nullptr;
+  for (int l = static_cast<int>(g->levels.size()) - 1; l >= 0; --l) {
+    const LevelRec &r = g->levels[l];
+    if (r.stall) continue;  // identity contraction
+    uint32_t *rep_out = (rep_up == g->rep_a) ? g->rep_b : g->rep_a;
+    TimedScope ts(g->timer, KC_K_UNDO, 12 * r.n);
+    undo(r.n, r.lab, r.newidx, r.gid, rep_up ? rep_up : g->rep_b, rep_out, g->stream);
+    rep_up = rep_out;
+  }
+  if (!rep_up) {  // no level at all (n == 0)
+    return;
+  }
+  // canonical labels: minimum vertex id per component
+  fill_u32(g->minid, g->n, kNone, g->stream);
+  fill_u32(g->small + 5, 1, 0, g->stream);
+  {
+    TimedScope ts(g->timer, KC_K_CANON, 8 * g->n);
+    canon_min(g->n, rep_up, g->minid, g->stream);
+  }
+  {
+    TimedScope ts(g->timer, KC_K_CANON, 12 * g->n);
+    canon_label(g->n, rep_up, g->minid, g->labels, g->small + 5, g->stream);
+  }
+}
+
+// Min-label propagation (labels) or propagation + pointer jumping (shortcuts).
+void run_propagation(kc_graph *g, const kc_options *o, RunOut &out) {
+  const BinCfg cfg = bin_cfg(o);
+  uint32_t flag_base = 0;
+  fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
+  g->levels.clear();
+  Bins b = make_bins(g, g->n, g->off0, cfg);
+  {
+    TimedScope ts(g->timer, KC_K_DEVIATES, 8 * g->n);
+    init_keys_index(g->n, g->key, g->stream);
+  }
+  out.sweeps = sweep_to_fixpoint(g, g->n, g->m2, g->off0, g->col0, g->key, 0, b, false,
+                                 o->variant == KC_VARIANT_SHORTCUT, flag_base);
+  out.levels = 0;
+  fill_u32(g->small + 5, 1, 0, g->stream);
+  TimedScope ts(g->timer, KC_K_CANON, 12 * g->n);
+  labels_from_keys(g->n, g->key, g->labels, g->small + 5, g->stream);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI.
+extern "C" {
+
+const char *kc_last_error(void) { return g_last_error.c_str(); }
+
+void kc_options_init(kc_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->variant = KC_VARIANT_EXP;
+  o->seed = 1;
+  o->beta = 0.1;
+  o->seq_limit = 4096;
+  o->deviate_fma = 1;
+  o->deviate_stream = KC_STREAM_REFERENCE;
+  o->max_levels = 4096;
+}
+
+int kc_graph_create(const kc_csr_host *csr, kc_graph **out) {
+  return guarded([&] {
+    if (!csr || !out) fail(KC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const uint64_t n = csr->n;
+    if (n >= 0xFFFFFFFFull) fail(KC_ERR_INVALID, "n=%llu exceeds 32-bit vertex ids", (unsigned long long)n);
+    if (n > 0 && (!csr->offsets || (!csr->targets && csr->offsets[n] > 0))) fail(KC_ERR_INVALID, "null CSR arrays");
+    const uint64_t m2 = n ? csr->offsets[n] : 0;
+    if (n && csr->offsets[0] != 0) fail(KC_ERR_INVALID, "offsets[0] must be 0");
+    if (m2 >= (1ull << 32)) fail(KC_ERR_INVALID, "2m=%llu exceeds 2^32 directed edges", (unsigned long long)m2);
+    uint64_t non_isolated = 0;
+    for (uint64_t v = 0; v < n; ++v) {
+      if (csr->offsets[v + 1] < csr->offsets[v]) fail(KC_ERR_INVALID, "offsets not monotone at %llu", (unsigned long long)v);
+      non_isolated += csr->offsets[v + 1] > csr->offsets[v];
+    }
+    for (uint64_t e = 0; e < m2; ++e)
+      if (csr->targets[e] >= n) fail(KC_ERR_INVALID, "target %u out of range at edge %llu", csr->targets[e], (unsigned long long)e);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(KC_ERR_HIP, "no HIP device available");
+    std::unique_ptr<kc_graph> g(new kc_graph());
+    KC_HIP(hipGetDevice(&g->device));
+    KC_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    KC_HIP(hipEventCreate(&g->ev_begin));
+    KC_HIP(hipEventCreate(&g->ev_end));
+    g->n = n;
+    g->m2 = m2;
+    g->non_isolated = non_isolated;
+    g->off0 = g->dalloc<uint64_t>(n + 1);
+    g->col0 = g->dalloc<uint32_t>(m2);
+    KC_HIP(hipMemcpy(g->off0, csr->offsets, (n + 1) * 8, hipMemcpyHostToDevice));
+    if (m2) KC_HIP(hipMemcpy(g->col0, csr->targets, m2 * 4, hipMemcpyHostToDevice));
+    g->key = g->dalloc<uint64_t>(n);
+    g->key2 = g->dalloc<uint64_t>(n);
+    g->deg_inter = g->dalloc<uint32_t>(n);
+    g->lists = g->dalloc<uint32_t>(3 * n);
+    g->hubs_cap = m2 / (kMinWaveBin / 2) + 2;
+    g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
+    g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
+    g->cnt = g->dalloc<uint32_t>(n);
+    g->fill = g->dalloc<uint32_t>(n);
+    g->bucket = g->dalloc<uint32_t>(m2);
+    g->old_of = g->dalloc<uint32_t>(n);
+    g->uniq = g->dalloc<uint32_t>(n);
+    g->bucket_off = g->dalloc<uint64_t>(n + 1);
+    g->rep_a = g->dalloc<uint32_t>(n);
+    g->rep_b = g->dalloc<uint32_t>(n);
+    g->minid = g->dalloc<uint32_t>(n);
+    g->labels = g->dalloc<uint32_t>(n);
+    g->scan_scratch = g->dalloc<uint64_t>(scan_scratch_words(n + m2) + 1024);
+    for (int i = 0; i < 2; ++i) {
+      g->off_pp[i] = g->dalloc<uint64_t>(n + 1);
+      g->col_pp[i] = g->dalloc<uint32_t>(m2);
+    }
+    g->dl_medium = g->dalloc<uint32_t>(n);
+    g->dl_huge = g->dalloc<uint32_t>(n);
+    g->dl_huge_tab = g->dalloc<uint64_t>(n);
+    g->hcount = g->dalloc<uint32_t>(n);
+    g->flags = g->dalloc<int>(kMaxFlags);
+    g->small = g->dalloc<uint32_t>(16);
+    g->sizes = g->dalloc<uint64_t>(16);
+    KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_sizes), 16 * sizeof(uint64_t), hipHostMallocDefault));
+    KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_small), 16 * sizeof(uint32_t), hipHostMallocDefault));
+    KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_flags), 64 * sizeof(int), hipHostMallocDefault));
+    KC_HIP(hipDeviceSynchronize());
+    *out = g.release();
+  });
+}
+
+int kc_graph_destroy(kc_graph *g) {
+  return guarded([&] { delete g; });
+}
+
+int kc_graph_info(const kc_graph *g, uint64_t *n, uint64_t *m, uint64_t *non_isolated) {
+  return guarded([&] {
+    if (!g) fail(KC_ERR_INVALID, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m2 / 2;
+    if (non_isolated) *non_isolated = g->non_isolated;
+  });
+}
+
+int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out, kc_stats *stats) {
+  return guarded([&] {
+    if (!g) fail(KC_ERR_INVALID, "null graph");
+    kc_options o;
+    if (opt)
+      o = *opt;
+    else
+      kc_options_init(&o);
+    if (o.variant < KC_VARIANT_EXP || o.variant > KC_VARIANT_LABELS) fail(KC_ERR_INVALID, "unknown variant %d", o.variant);
+    if (o.learned) fail(KC_ERR_UNSUPPORTED, "learned centre proposals need kc_mlp_set first");
+    const auto t0 = std::chrono::steady_clock::now();
+    g->have_labels = false;
+    if (o.variant == KC_VARIANT_EXP) ensure_table(g, o.beta);
+    g->arena.reset();
+    g->timer.begin_run(o.time_kernels != 0, g->stream);
+    KC_HIP(hipEventRecord(g->ev_begin, g->stream));
+    RunOut out;
+    if (o.variant == KC_VARIANT_EXP || o.variant == KC_VARIANT_LOCAL)
+      run_contraction(g, &o, out);
+    else
+      run_propagation(g, &o, out);
+    KC_HIP(hipEventRecord(g->ev_end, g->stream));
+    d2h(g, g->h_small + 5, g->small + 5, sizeof(uint32_t));
+    sync(g);
+    g->components = g->n ? g->h_small[5] : 0;
+    g->have_labels = true;
+    float ms = 0.f;
+    KC_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+    if (labels_out && g->n) KC_HIP(hipMemcpy(labels_out, g->labels, g->n * 4, hipMemcpyDeviceToHost));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->time_ms = ms;
+      stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+      stats->n = g->n;
+      stats->m = g->m2 / 2;
+      stats->levels = out.levels;
+      stats->components = g->components;
+      stats->sweeps = out.sweeps;
+      stats->radius_sum = out.radius_sum;
+      g->timer.finish(stats);
+      uint64_t tot = 0;
+      for (int i = 0; i < KC_KERNEL_CLASSES; ++i) tot += stats->kernel_bytes[i];
+      stats->alg_bytes = tot;
+    }
+  });
+}
+
+int kc_copy_labels(kc_graph *g, uint32_t *labels_out) {
+  return guarded([&] {
+    if (!g || !labels_out) fail(KC_ERR_INVALID, "null argument");
+    if (!g->have_labels) fail(KC_ERR_INVALID, "no labels: kc_find_components has not completed");
+    if (g->n) KC_HIP(hipMemcpy(labels_out, g->labels, g->n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int kc_trace_levels(const kc_graph *g, uint32_t *levels) {
+  return guarded([&] {
+    if (!g || !levels) fail(KC_ERR_INVALID, "null argument");
+    *levels = static_cast<uint32_t>(g->levels.size());
+  });
+}
+
+int kc_trace_level_info(const kc_graph *g, uint32_t level, uint64_t info[4]) {
+  return guarded([&] {
+    if (!g || !info) fail(KC_ERR_INVALID, "null argument");
+    if (level >= g->levels.size()) fail(KC_ERR_INVALID, "level %u out of range", level);
+    const LevelRec &r = g->levels[level];
+    info[0] = r.n;
+    info[1] = r.m2 / 2;
+    info[2] = r.sweeps;
+    info[3] = r.radius;
+  });
+}
+
+int kc_trace_level_keys(const kc_graph *g, uint32_t level, uint64_t *keys_out, uint32_t *gid_out) {
+  return guarded([&] {
+    if (!g) fail(KC_ERR_INVALID, "null graph");
+    if (level >= g->levels.size()) fail(KC_ERR_INVALID, "level %u out of range", level);
+    const LevelRec &r = g->levels[level];
+    if (keys_out) {
+      if (!r.keys_copy) fail(KC_ERR_INVALID, "run without record_trace");
+      KC_HIP(hipMemcpy(keys_out, r.keys_copy, r.n * 8, hipMemcpyDeviceToHost));
+    }
+    if (gid_out) {
+      if (r.gid) {
+        KC_HIP(hipMemcpy(gid_out, r.gid, r.n * 4, hipMemcpyDeviceToHost));
+      } else {
+        for (uint64_t v = 0; v < r.n; ++v) gid_out[v] = static_cast<uint32_t>(v);
+      }
+    }
+  });
+}
+
+int kc_deviates(uint32_t gid0, uint32_t count, int32_t seed, uint32_t iteration, uint64_t rng_offset, double beta,
+                int32_t fma, int32_t stream, uint32_t *out) {
+  return guarded([&] {
+    if (!out && count) fail(KC_ERR_INVALID, "null output");
+    if (!(beta > 1e-6) || !(beta < 1e6)) fail(KC_ERR_INVALID, "beta must be in (1e-6, 1e6), got %g", beta);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(KC_ERR_HIP, "no HIP device available");
+    std::vector<double> t = build_deviate_table(beta);
+    double *dt = nullptr;
+    uint32_t *dout = nullptr;
+    KC_HIP(hipMalloc(&dt, (t.size() + 1) * 8));
+    KC_HIP(hipMalloc(&dout, (static_cast<size_t>(count) + 1) * 4));
+    KC_HIP(hipMemcpy(dt, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+    DeviateArgs d{seed, iteration, rng_offset, fma, stream, dt, static_cast<uint32_t>(t.size())};
+    deviates_only(gid0, count, d, dout, nullptr);
+    KC_HIP(hipDeviceSynchronize());
+    KC_HIP(hipMemcpy(out, dout, static_cast<size_t>(count) * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(dt);
+    (void)hipFree(dout);
+  });
+}
+
+}  // extern "C"

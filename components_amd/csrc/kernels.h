@@ -1,0 +1,129 @@
+// Launch wrappers of the device kernels (kernels.hip).  Every wrapper enqueues
+// on the given stream and never synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kc {
+
+// A slice of a hub row (degree above the wave bin) handled by one wave.
+struct HubChunk {
+  uint32_t v;
+  uint32_t first;  // 1 for the first chunk of v
+  uint64_t e0, e1;
+};
+
+struct BinCfg {
+  uint32_t tmax;  // degree <= tmax: one lane per vertex
+  uint32_t smax;  // degree <= smax: 16 lanes per vertex
+  uint32_t wmax;  // degree <= wmax: one wave per vertex; above: wave chunks of wmax edges
+};
+
+// Degree bins of one level; lists and chunks live on the device.
+struct Bins {
+  uint32_t *listT = nullptr, *listS = nullptr, *listW = nullptr;
+  HubChunk *hubs = nullptr;
+  uint32_t nT = 0, nS = 0, nW = 0, nH = 0;
+  // first block of each section in a binned launch (thread section starts at 0)
+  uint32_t bS = 0, bW = 0, bH = 0, blocks = 0;
+};
+
+enum SweepMode : int {
+  kSweepInPlace = 0,  // key[v] <- min(key[v], key[w] + hop), updates visible at once
+  kSweepJacobi = 1,   // kout[v] <- min over closed neighbourhood of kin (kout pre-copied)
+  kSweepCountOnly = 2 // no update: count neighbours with a different label
+};
+
+struct DeviateArgs {
+  int32_t seed;
+  uint32_t iteration;
+  uint64_t rng_offset;
+  int32_t fused;
+  int32_t stream;  // 0: reference draw, 1: hashed draw
+  const double *table;
+  uint32_t k_max;
+};
+
+// Learned centre proposals: per-vertex deviate divisor in 1/256 units (256 = 1.0).
+struct LearnedArgs {
+  const uint16_t *scale_q8;  // null: plain deviates
+};
+
+void fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+void fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+
+void init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, LearnedArgs la, uint64_t *key,
+                   hipStream_t s);
+void init_keys_index(uint64_t n, uint64_t *key, hipStream_t s);
+void init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t iteration, uint64_t *key,
+                       hipStream_t s);
+void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, hipStream_t s);
+
+// Bins vertices of a CSR by degree; counters[0..3] (zeroed by the caller)
+// receive nT, nS, nW, nH.  deg_inter of degree-0 vertices is cleared.
+void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS,
+                  uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter,
+                  hipStream_t s);
+void finish_bins(Bins &b);  // computes block offsets from the counts
+
+// One binned sweep.  flag: set to 1 when a key decreased; prev_flag (may be
+// null): when it reads 0 the launch returns at once (converged earlier).
+void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin,
+           uint64_t *kout, uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter,
+           int *flag, const int *prev_flag, hipStream_t s);
+void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, hipStream_t s);
+
+// Pointer jumping on keys: key[v] <- min(key[v], key[label(v)]).
+void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
+
+// lab[v] = low half of key[v]; cnt[lab] += deg_inter[v]; max cluster radius.
+void cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab,
+                   uint32_t *cnt, uint32_t *radius, bool with_radius, hipStream_t s);
+
+// Kept clusters: newidx, bucket offsets, next-level gid, old index of each
+// kept cluster.  sizes[0] = n_next, sizes[1] = total bucket entries.
+void kept_scan(uint64_t n, const uint32_t *cnt, const uint32_t *gid, uint32_t *newidx,
+               uint64_t *bucket_off, uint32_t *gid_next, uint32_t *old_of, uint64_t *sizes,
+               uint64_t *scratch, hipStream_t s);
+
+void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b,
+             const uint32_t *deg_inter, const uint32_t *hub_inter, const uint32_t *newidx,
+             const uint64_t *bucket_off, uint32_t *fill, uint32_t *bucket, hipStream_t s);
+
+// Dedup of every kept bucket; lists[0..2] count medium/huge buckets and huge
+// table slots (zeroed by the caller).
+struct DedupLists {
+  uint32_t *medium;    // a' of buckets with wave_cap < size <= block_cap
+  uint32_t *huge;      // a' of buckets above block_cap
+  uint64_t *huge_tab;  // table offset per huge bucket
+  uint64_t *counters;  // [0] medium count, [1] huge count, [2] huge table slots
+};
+constexpr uint32_t kDedupWaveCap = 512;
+constexpr uint32_t kDedupBlockCap = 8192;
+void dedup_small(uint64_t n_next, const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off,
+                 uint32_t *bucket, uint32_t *uniq, DedupLists l, hipStream_t s);
+void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint32_t *old_of, const uint32_t *cnt,
+                  const uint64_t *bucket_off, uint32_t *bucket, uint32_t *uniq, hipStream_t s);
+void dedup_huge(uint32_t n_huge, const uint32_t *huge, const uint64_t *huge_tab, uint32_t *tables,
+                const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off, uint32_t *bucket,
+                uint32_t *uniq, uint32_t *hcount, hipStream_t s);
+
+// off_next = exclusive scan of uniq (n_next+1 entries); sizes[2] = total.
+void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint64_t *sizes,
+                 uint64_t *scratch, hipStream_t s);
+// Stream compaction of bucket[0..total) dropping kNone into col_next.
+void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, uint64_t *scratch, hipStream_t s);
+uint64_t scan_scratch_words(uint64_t n);
+
+void undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_t *gid,
+          const uint32_t *rep_up, uint32_t *rep_out, hipStream_t s);
+// rep of a base-case level: rep[v] = gid[low(key[v])].
+void rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_t *rep, hipStream_t s);
+void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s);
+void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,
+                 uint32_t *roots, hipStream_t s);
+void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, hipStream_t s);
+
+}  // namespace kc

@@ -1,0 +1,976 @@
+// Device kernels of the contraction path for gfx950 (wave64, 160 KB LDS/CU).
+//
+// Data layout per level l (all in HBM):
+//   off   u64[n_l+1]  CSR row offsets        col  u32[2 m_l]  neighbour indices
+//   key   u64[n_l]    (deviate + hops) << 32 | centre index  (lexicographic key
+//                     of lib/components/exponential_contraction.h:531-537 as one integer)
+//   lab   u32[n_l]    centre index after the decomposition
+//   cnt   u32[n_l]    cluster-edge count of each centre's bucket
+//   bucket u32[P]     next-level neighbour indices grouped by cluster (P = sum cnt)
+// Integer work only; no MFMA.  Every kernel is bandwidth-bound.
+#include <hip/hip_runtime.h>
+
+#include "deviates.h"
+#include "kc_common.h"
+#include "kernels.h"
+
+namespace kc {
+
+namespace {
+
+inline unsigned grid_for(uint64_t n, unsigned per_block = kBlock) {
+  uint64_t g = ceil_div(n, per_block);
+  if (g == 0) g = 1;
+  if (g > 0x7fffffffull) g = 0x7fffffffull;
+  return static_cast<unsigned>(g);
+}
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Wave-aggregated append: every lane with pred gets a distinct slot of *counter.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
+  uint64_t m = __ballot(pred);
+  if (m == 0) return 0;
+  int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+  uint32_t base = 0;
+  if (static_cast<int>(lane_id()) == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(m)));
+  base = __shfl(base, leader);
+  return base + static_cast<uint32_t>(__popcll(m & lanes_below()));
+}
+
+// ---------------------------------------------------------------------------
+// Fills.
+template <class T>
+__global__ void k_fill(T *p, uint64_t n, T v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Keys.
+__global__ void k_init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, LearnedArgs la,
+                                uint64_t *key) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  uint32_t g = gid ? gid[v] : static_cast<uint32_t>(v);
+  uint32_t dev = exp_deviate(g, d.seed, d.iteration, d.rng_offset, d.fused != 0, d.stream, d.table, d.k_max);
+  if (la.scale_q8) dev = static_cast<uint32_t>((static_cast<uint64_t>(dev) * 256u) / la.scale_q8[v]);
+  key[v] = (static_cast<uint64_t>(dev) << 32) | static_cast<uint32_t>(v);
+}
+
+__global__ void k_init_keys_index(uint64_t n, uint64_t *key) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) key[v] = static_cast<uint32_t>(v);
+}
+
+__global__ void k_init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t it, uint64_t *key) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  uint32_t g = gid ? gid[v] : static_cast<uint32_t>(v);
+  key[v] = (static_cast<uint64_t>(uniform_deviate_100(g, seed, it)) << 32) | static_cast<uint32_t>(v);
+}
+
+__global__ void k_deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  out[i] = exp_deviate(gid0 + i, d.seed, d.iteration, d.rng_offset, d.fused != 0, d.stream, d.table, d.k_max);
+}
+
+// ---------------------------------------------------------------------------
+// Degree binning.
+__global__ void k_bin(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS,
+                      uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t deg = 0, e0 = 0;
+  if (v < n) {
+    e0 = off[v];
+    deg = off[v + 1] - e0;
+    if (deg == 0 && deg_inter) deg_inter[v] = 0;
+  }
+  const bool live = v < n && deg > 0;
+  const bool inT = live && deg <= cfg.tmax;
+  const bool inS = live && deg > cfg.tmax && deg <= cfg.smax;
+  const bool inW = live && deg > cfg.smax && deg <= cfg.wmax;
+  const bool inH = live && deg > cfg.wmax;
+  uint32_t p;
+  p = wave_append(inT, &counters[0]);
+  if (inT) listT[p] = static_cast<uint32_t>(v);
+  p = wave_append(inS, &counters[1]);
+  if (inS) listS[p] = static_cast<uint32_t>(v);
+  p = wave_append(inW, &counters[2]);
+  if (inW) listW[p] = static_cast<uint32_t>(v);
+  if (inH) {
+    uint32_t chunks = static_cast<uint32_t>(ceil_div(deg, cfg.wmax));
+    uint32_t base = atomicAdd(&counters[3], chunks);
+    for (uint32_t c = 0; c < chunks; ++c) {
+      HubChunk h;
+      h.v = static_cast<uint32_t>(v);
+      h.first = c == 0;
+      h.e0 = e0 + static_cast<uint64_t>(c) * cfg.wmax;
+      h.e1 = min(e0 + deg, h.e0 + cfg.wmax);
+      hubs[base + c] = h;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binned sweep.  The block index selects the bin section; inside a section a
+// group of W lanes owns one vertex (or one hub chunk) and strides over its
+// edges, then reduces with cross-lane shuffles.
+struct SweepArgs {
+  const uint64_t *off;
+  const uint32_t *col;
+  const uint64_t *kin;
+  uint64_t *kout;
+  uint64_t hop;
+  Bins b;
+  uint32_t *deg_inter;
+  uint32_t *hub_inter;
+  int *flag;
+  const int *prev_flag;
+};
+
+template <int MODE, bool COUNT, int W>
+__device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uint32_t v, uint64_t e0,
+                                            uint64_t e1, int gl, bool hub, uint32_t chunk) {
+  uint64_t kv = active ? a.kin[v] : ~0ull;
+  uint64_t best = kv;
+  uint32_t cnt = 0;
+  if (active) {
+    const uint32_t lv = static_cast<uint32_t>(kv);
+    for (uint64_t e = e0 + gl; e < e1; e += W) {
+      const uint64_t kw = a.kin[a.col[e]];
+      best = min_u64(best, kw + a.hop);
+      if (COUNT) cnt += static_cast<uint32_t>(kw) != lv;
+    }
+  }
+  if (W > 1) {
+    best = group_min_u64(best, W);
+    if (COUNT) cnt = group_sum_u32(cnt, W);
+  }
+  bool changed = false;
+  if (active && gl == 0) {
+    if (MODE != kSweepCountOnly && best < kv) {
+      if (hub) {
+        uint64_t old = atomicMin(reinterpret_cast<unsigned long long *>(&a.kout[v]),
+                                 static_cast<unsigned long long>(best));
+        changed = old > best;
+      } else {
+        a.kout[v] = best;
+        changed = true;
+      }
+    }
+    if (COUNT) {
+      if (hub)
+        a.hub_inter[chunk] = cnt;
+      else
+        a.deg_inter[v] = cnt;
+    }
+  }
+  return changed;
+}
+
+template <int MODE, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
+  if (a.prev_flag && *a.prev_flag == 0) return;
+  const uint32_t blk = blockIdx.x;
+  const int tid = threadIdx.x;
+  bool changed = false;
+  if (blk < a.b.bS) {
+    const uint32_t i = blk * kBlock + tid;
+    const bool act = i < a.b.nT;
+    uint32_t v = act ? a.b.listT[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    changed = sweep_group<MODE, COUNT, 1>(a, act, v, e0, e1, 0, false, 0);
+  } else if (blk < a.b.bW) {
+    const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
+    const bool act = i < a.b.nS;
+    uint32_t v = act ? a.b.listS[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    changed = sweep_group<MODE, COUNT, 16>(a, act, v, e0, e1, tid & 15, false, 0);
+  } else if (blk < a.b.bH) {
+    const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nW;
+    uint32_t v = act ? a.b.listW[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    changed = sweep_group<MODE, COUNT, kWave>(a, act, v, e0, e1, tid & 63, false, 0);
+  } else {
+    const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nH;
+    HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
+    changed = sweep_group<MODE, COUNT, kWave>(a, act, h.v, h.e0, h.e1, tid & 63, true, i);
+  }
+  if (MODE != kSweepCountOnly) {
+    if (__ballot(changed) != 0 && lane_id() == 0) *a.flag = 1;
+  }
+}
+
+__global__ void k_hub_fold(HubChunk *hubs, uint32_t nH, const uint32_t *hub_inter, uint32_t *deg_inter) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nH || !hubs[i].first) return;
+  const uint32_t v = hubs[i].v;
+  uint32_t sum = 0;
+  for (uint32_t c = i; c < nH && hubs[c].v == v; ++c) sum += hub_inter[c];
+  deg_inter[v] = sum;
+}
+
+__global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  bool changed = false;
+  if (v < n) {
+    const uint64_t k = key[v];
+    const uint64_t k2 = key[static_cast<uint32_t>(k)];
+    if (k2 < k) {
+      key[v] = k2;
+      changed = true;
+    }
+  }
+  if (__ballot(changed) != 0 && lane_id() == 0) *flag = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Cluster counting.
+__global__ void k_cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab,
+                                uint32_t *cnt, uint32_t *radius, int with_radius) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint32_t r = 0;
+  if (v < n) {
+    const uint64_t k = key[v];
+    const uint32_t l = static_cast<uint32_t>(k);
+    lab[v] = l;
+    const uint32_t d = deg_inter[v];
+    if (d) atomicAdd(&cnt[l], d);
+    if (with_radius) r = static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[l] >> 32);
+  }
+  if (with_radius) {
+    for (int o = 32; o > 0; o >>= 1) r = max(r, static_cast<uint32_t>(__shfl_xor(r, o)));
+    if (lane_id() == 0 && r) atomicMax(radius, r);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic three-phase exclusive scan over functor-defined values.
+constexpr int kScanPerThread = 8;
+constexpr int kScanTile = kBlock * kScanPerThread;
+
+struct Pair64 {
+  uint64_t a, b;
+  __host__ __device__ Pair64 operator+(const Pair64 &o) const { return {a + o.a, b + o.b}; }
+};
+
+// Block-wide exclusive scan (Hillis-Steele in LDS); *total receives the block sum.
+template <class T>
+__device__ T block_exclusive_scan_shift(T v, T *smem, T *total) {
+  const int tid = threadIdx.x;
+  smem[tid] = v;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {
+    T t = tid >= o ? smem[tid - o] : T{};
+    __syncthreads();
+    if (tid >= o) smem[tid] = smem[tid] + t;
+    __syncthreads();
+  }
+  T excl = tid > 0 ? smem[tid - 1] : T{};
+  T tot = smem[kBlock - 1];
+  __syncthreads();
+  if (total) *total = tot;
+  return excl;
+}
+
+template <class T, class Load>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(uint64_t n, Load load, T *block_sums) {
+  __shared__ T smem[kBlock];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  T acc{};
+  for (int j = 0; j < kScanPerThread; ++j) {
+    uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
+    if (i < n) acc = acc + load(i);
+  }
+  T tot;
+  block_exclusive_scan_shift<T>(acc, smem, &tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+template <class T, class Load, class Store>
+__global__ __launch_bounds__(kBlock) void k_scan_down(uint64_t n, Load load, Store store, const T *block_prefix,
+                                                     T *total_out) {
+  __shared__ T smem[kBlock];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + static_cast<uint64_t>(threadIdx.x) * kScanPerThread;
+  T vals[kScanPerThread];
+  T acc{};
+#pragma unroll
+  for (int j = 0; j < kScanPerThread; ++j) {
+    uint64_t i = base + j;
+    vals[j] = i < n ? load(i) : T{};
+    acc = acc + vals[j];
+  }
+  T tot;
+  T run = block_exclusive_scan_shift<T>(acc, smem, &tot);
+  const T bp = block_prefix ? block_prefix[blockIdx.x] : T{};
+  run = bp + run;
+#pragma unroll
+  for (int j = 0; j < kScanPerThread; ++j) {
+    uint64_t i = base + j;
+    if (i < n) store(i, run, vals[j]);
+    run = run + vals[j];
+  }
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total_out = run;
+}
+
+template <class T>
+struct ArrayLoad {
+  const T *p;
+  __device__ T operator()(uint64_t i) const { return p[i]; }
+};
+template <class T>
+struct ArrayStore {
+  T *p;
+  __device__ void operator()(uint64_t i, T prefix, T) const { p[i] = prefix; }
+};
+
+// Scratch words (u64) needed by scan_exclusive for n elements.
+uint64_t scan_words_for(uint64_t n, size_t elem_words) {
+  uint64_t words = 0;
+  uint64_t blocks = ceil_div(n, kScanTile);
+  while (blocks > 1) {
+    words += 2 * blocks * elem_words + 8;
+    blocks = ceil_div(blocks, kScanTile);
+  }
+  return words + 16 * elem_words;
+}
+
+// Exclusive scan; the total lands in *total (device).  scratch must hold
+// scan_words_for(n) u64 words.
+template <class T, class Load, class Store>
+void scan_exclusive(uint64_t n, Load load, Store store, T *total, uint64_t *scratch, hipStream_t s) {
+  if (n == 0) {
+    hipLaunchKernelGGL((k_fill<T>), dim3(1), dim3(1), 0, s, total, 1, T{});
+    KC_LAUNCH_CHECK();
+    return;
+  }
+  const uint64_t blocks = ceil_div(n, kScanTile);
+  if (blocks == 1) {
+    hipLaunchKernelGGL((k_scan_down<T, Load, Store>), dim3(1), dim3(kBlock), 0, s, n, load, store,
+                       static_cast<const T *>(nullptr), total);
+    KC_LAUNCH_CHECK();
+    return;
+  }
+  T *sums = reinterpret_cast<T *>(scratch);
+  T *prefix = sums + blocks;
+  uint64_t *rest = reinterpret_cast<uint64_t *>(prefix + blocks) + 4;
+  hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n, load,
+                     sums);
+  KC_LAUNCH_CHECK();
+  scan_exclusive<T>(blocks, ArrayLoad<T>{sums}, ArrayStore<T>{prefix}, total, rest, s);
+  hipLaunchKernelGGL((k_scan_down<T, Load, Store>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n,
+                     load, store, static_cast<const T *>(prefix), static_cast<T *>(nullptr));
+  KC_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Kept clusters.
+struct KeptLoad {
+  const uint32_t *cnt;
+  __device__ Pair64 operator()(uint64_t i) const {
+    const uint32_t c = cnt[i];
+    return {c != 0 ? 1ull : 0ull, c};
+  }
+};
+struct KeptStore {
+  const uint32_t *gid;
+  uint32_t *newidx;
+  uint64_t *bucket_off;
+  uint32_t *gid_next;
+  uint32_t *old_of;
+  __device__ void operator()(uint64_t i, Pair64 prefix, Pair64 v) const {
+    bucket_off[i] = prefix.b;
+    if (v.a) {
+      const uint32_t a2 = static_cast<uint32_t>(prefix.a);
+      newidx[i] = a2;
+      gid_next[a2] = gid ? gid[i] : static_cast<uint32_t>(i);
+      old_of[a2] = static_cast<uint32_t>(i);
+    } else {
+      newidx[i] = kNone;
+    }
+  }
+};
+
+__global__ void k_pair_to_sizes(const Pair64 *p, uint64_t *sizes) {
+  sizes[0] = p->a;
+  sizes[1] = p->b;
+}
+
+// ---------------------------------------------------------------------------
+// Scatter of cluster edges into buckets (binned like the sweep).
+struct ScatterArgs {
+  const uint64_t *off;
+  const uint32_t *col;
+  const uint64_t *key;
+  Bins b;
+  const uint32_t *deg_inter;
+  const uint32_t *hub_inter;
+  const uint32_t *newidx;
+  const uint64_t *bucket_off;
+  uint32_t *fill;
+  uint32_t *bucket;
+};
+
+template <int W>
+__device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active, uint32_t v, uint64_t e0,
+                                              uint64_t e1, int gl, uint32_t d) {
+  // group base lane within the wave
+  const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
+  uint32_t lv = 0;
+  uint64_t pos = 0;
+  active = active && d > 0;
+  if (active) {
+    lv = static_cast<uint32_t>(a.key[v]);
+    if (gl == 0) pos = a.bucket_off[lv] + atomicAdd(&a.fill[lv], d);
+  }
+  if (W > 1) pos = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), gbase, 64));
+  if (W == 1) {
+    if (!active) return;
+    for (uint64_t e = e0; e < e1; ++e) {
+      const uint32_t lw = static_cast<uint32_t>(a.key[a.col[e]]);
+      if (lw != lv) a.bucket[pos++] = a.newidx[lw];
+    }
+    return;
+  }
+  // all lanes of the wave iterate the same number of rounds (max over groups)
+  uint64_t len = active ? e1 - e0 : 0;
+  uint64_t rounds = ceil_div(len, W);
+  for (int o = 32; o > 0; o >>= 1) {
+    uint64_t t = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(rounds), o, 64));
+    rounds = t > rounds ? t : rounds;
+  }
+  const uint64_t gmask = (W >= 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << gbase);
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t e = e0 + r * W + gl;
+    bool pred = false;
+    uint32_t lw = 0;
+    if (active && e < e1) {
+      lw = static_cast<uint32_t>(a.key[a.col[e]]);
+      pred = lw != lv;
+    }
+    const uint64_t m = __ballot(pred) & gmask;
+    if (pred) a.bucket[pos + __popcll(m & lanes_below())] = a.newidx[lw];
+    pos += __popcll(m);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(ScatterArgs a) {
+  const uint32_t blk = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (blk < a.b.bS) {
+    const uint32_t i = blk * kBlock + tid;
+    const bool act = i < a.b.nT;
+    uint32_t v = act ? a.b.listT[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    scatter_group<1>(a, act, v, e0, e1, 0, act ? a.deg_inter[v] : 0);
+  } else if (blk < a.b.bW) {
+    const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
+    const bool act = i < a.b.nS;
+    uint32_t v = act ? a.b.listS[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    scatter_group<16>(a, act, v, e0, e1, tid & 15, act ? a.deg_inter[v] : 0);
+  } else if (blk < a.b.bH) {
+    const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nW;
+    uint32_t v = act ? a.b.listW[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    scatter_group<kWave>(a, act, v, e0, e1, tid & 63, act ? a.deg_inter[v] : 0);
+  } else {
+    const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nH;
+    HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
+    scatter_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, act ? a.hub_inter[i] : 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bucket dedup.  Small buckets: one wave, open-addressing table in LDS.
+constexpr int kWaveTab = 2 * kDedupWaveCap;   // 1024 slots per wave
+constexpr int kBlockTab = 2 * kDedupBlockCap;  // 16384 slots per block (64 KB)
+
+__device__ __forceinline__ bool lds_insert(uint32_t *tab, uint32_t mask, uint32_t b) {
+  uint32_t h = hash32(b) & mask;
+  while (true) {
+    uint32_t prev = atomicCAS(&tab[h], kNone, b);
+    if (prev == kNone) return true;
+    if (prev == b) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint32_t *old_of, const uint32_t *cnt,
+                                                       const uint64_t *bucket_off, uint32_t *bucket, uint32_t *uniq,
+                                                       DedupLists l) {
+  __shared__ uint32_t tabs[kBlock / kWave][kWaveTab];
+  const int w = threadIdx.x / kWave;
+  const int ln = threadIdx.x & 63;
+  uint32_t *tab = tabs[w];
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t a2 = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + w; a2 < n_next; a2 += waves) {
+    const uint32_t a = old_of[a2];
+    const uint32_t s = cnt[a];
+    const uint64_t base = bucket_off[a];
+    if (s > kDedupWaveCap) {
+      if (ln == 0) {
+        if (s <= kDedupBlockCap) {
+          uint64_t p = atomicAdd(reinterpret_cast<unsigned long long *>(&l.counters[0]), 1ull);
+          l.medium[p] = static_cast<uint32_t>(a2);
+        } else {
+          uint64_t p = atomicAdd(reinterpret_cast<unsigned long long *>(&l.counters[1]), 1ull);
+          uint64_t tsz = 1;
+          while (tsz < 2ull * s) tsz <<= 1;
+          uint64_t t = atomicAdd(reinterpret_cast<unsigned long long *>(&l.counters[2]),
+                                 static_cast<unsigned long long>(tsz));
+          l.huge[p] = static_cast<uint32_t>(a2);
+          l.huge_tab[p] = t;
+        }
+      }
+      continue;
+    }
+    if (s == 1) {
+      if (ln == 0) uniq[a2] = 1;
+      continue;
+    }
+    uint32_t tsz = 2;
+    while (tsz < 2u * s) tsz <<= 1;
+    const uint32_t mask = tsz - 1;
+    for (uint32_t j = ln; j < tsz; j += kWave) tab[j] = kNone;
+    wave_fence();
+    for (uint32_t j = ln; j < s; j += kWave) lds_insert(tab, mask, bucket[base + j]);
+    wave_fence();
+    // emit table contents in slot order
+    uint32_t written = 0;
+    for (uint32_t j0 = 0; j0 < tsz; j0 += kWave) {
+      const uint32_t j = j0 + ln;
+      const uint32_t val = j < tsz ? tab[j] : kNone;
+      const bool keep = val != kNone;
+      const uint64_t m = __ballot(keep);
+      if (keep) bucket[base + written + __popcll(m & lanes_below())] = val;
+      written += __popcll(m);
+    }
+    for (uint32_t j = written + ln; j < s; j += kWave) bucket[base + j] = kNone;
+    if (ln == 0) uniq[a2] = written;
+    wave_fence();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, const uint32_t *medium,
+                                                        const uint32_t *old_of, const uint32_t *cnt,
+                                                        const uint64_t *bucket_off, uint32_t *bucket,
+                                                        uint32_t *uniq) {
+  __shared__ uint32_t tab[kBlockTab];
+  __shared__ uint32_t psum[kBlock];
+  const int tid = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < n_medium; i += gridDim.x) {
+    const uint32_t a2 = medium[i];
+    const uint32_t a = old_of[a2];
+    const uint32_t s = cnt[a];
+    const uint64_t base = bucket_off[a];
+    uint32_t tsz = 2;
+    while (tsz < 2u * s) tsz <<= 1;
+    const uint32_t mask = tsz - 1;
+    for (uint32_t j = tid; j < tsz; j += kBlock) tab[j] = kNone;
+    __syncthreads();
+    for (uint32_t j = tid; j < s; j += kBlock) lds_insert(tab, mask, bucket[base + j]);
+    __syncthreads();
+    // each thread owns a contiguous range of slots
+    const uint32_t per = tsz / kBlock > 0 ? tsz / kBlock : 1;
+    const uint32_t j0 = tid * per;
+    uint32_t c = 0;
+    for (uint32_t j = j0; j < j0 + per && j < tsz; ++j) c += tab[j] != kNone;
+    uint32_t tot;
+    uint32_t p = block_exclusive_scan_shift<uint32_t>(c, psum, &tot);
+    for (uint32_t j = j0; j < j0 + per && j < tsz; ++j)
+      if (tab[j] != kNone) bucket[base + p++] = tab[j];
+    __syncthreads();
+    for (uint32_t j = tot + tid; j < s; j += kBlock) bucket[base + j] = kNone;
+    if (tid == 0) uniq[a2] = tot;
+    __syncthreads();
+  }
+}
+
+// Huge buckets: one global open-addressing table per bucket (pre-filled with kNone).
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_insert(uint32_t n_huge, const uint32_t *huge,
+                                                             const uint64_t *huge_tab, uint32_t *tables,
+                                                             const uint32_t *old_of, const uint32_t *cnt,
+                                                             const uint64_t *bucket_off, const uint32_t *bucket) {
+  for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
+  const uint32_t a = old_of[huge[h]];
+  const uint32_t s = cnt[a];
+  const uint64_t base = bucket_off[a];
+  uint64_t tsz = 1;
+  while (tsz < 2ull * s) tsz <<= 1;
+  const uint32_t mask = static_cast<uint32_t>(tsz - 1);
+  uint32_t *tab = tables + huge_tab[h];
+  for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < s;
+       j += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint32_t b = bucket[base + j];
+    uint32_t k = hash32(b) & mask;
+    while (true) {
+      uint32_t prev = atomicCAS(&tab[k], kNone, b);
+      if (prev == kNone || prev == b) break;
+      k = (k + 1) & mask;
+    }
+  }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(uint32_t n_huge, const uint32_t *huge,
+                                                           const uint64_t *huge_tab, const uint32_t *tables,
+                                                           const uint32_t *old_of, const uint32_t *cnt,
+                                                           const uint64_t *bucket_off, uint32_t *bucket,
+                                                           uint32_t *hcount) {
+  for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
+  const uint32_t a = old_of[huge[h]];
+  const uint32_t s = cnt[a];
+  const uint64_t base = bucket_off[a];
+  uint64_t tsz = 1;
+  while (tsz < 2ull * s) tsz <<= 1;
+  const uint32_t *tab = tables + huge_tab[h];
+  for (uint64_t j0 = static_cast<uint64_t>(blockIdx.x) * kBlock; j0 < tsz; j0 += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint64_t j = j0 + threadIdx.x;
+    const uint32_t val = j < tsz ? tab[j] : kNone;
+    const bool keep = val != kNone;
+    const uint32_t p = wave_append(keep, &hcount[h]);
+    if (keep) bucket[base + p] = val;
+  }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_tail(uint32_t n_huge, const uint32_t *huge,
+                                                           const uint32_t *old_of, const uint32_t *cnt,
+                                                           const uint64_t *bucket_off, uint32_t *bucket,
+                                                           uint32_t *uniq, const uint32_t *hcount) {
+  for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
+  const uint32_t a2 = huge[h];
+  const uint32_t a = old_of[a2];
+  const uint32_t s = cnt[a];
+  const uint64_t base = bucket_off[a];
+  const uint32_t u = hcount[h];
+  if (blockIdx.x == 0 && threadIdx.x == 0) uniq[a2] = u;
+  for (uint64_t j = u + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < s;
+       j += static_cast<uint64_t>(gridDim.x) * kBlock)
+    bucket[base + j] = kNone;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Next-level CSR.
+struct UniqLoad {
+  const uint32_t *u;
+  __device__ uint64_t operator()(uint64_t i) const { return u[i]; }
+};
+struct OffStore {
+  uint64_t *off;
+  uint64_t n;
+  __device__ void operator()(uint64_t i, uint64_t prefix, uint64_t v) const {
+    off[i] = prefix;
+    if (i + 1 == n) off[n] = prefix + v;
+  }
+};
+__global__ void k_copy_total(const uint64_t *t, uint64_t *dst) { *dst = *t; }
+
+// Compaction: count non-kNone per tile, scan, write.
+__global__ __launch_bounds__(kBlock) void k_compact_count(uint64_t total, const uint32_t *src, uint64_t *counts) {
+  __shared__ uint64_t smem[kBlock];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  uint64_t c = 0;
+  for (int j = 0; j < kScanPerThread; ++j) {
+    const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
+    if (i < total) c += src[i] != kNone;
+  }
+  uint64_t tot;
+  block_exclusive_scan_shift<uint64_t>(c, smem, &tot);
+  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_write(uint64_t total, const uint32_t *src, uint32_t *dst,
+                                                         const uint64_t *prefix) {
+  __shared__ uint64_t smem[kBlock];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  uint64_t out = prefix[blockIdx.x];
+  for (int j = 0; j < kScanPerThread; ++j) {
+    const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
+    const uint32_t val = i < total ? src[i] : kNone;
+    const uint64_t keep = val != kNone;
+    uint64_t tot;
+    uint64_t p = block_exclusive_scan_shift<uint64_t>(keep, smem, &tot);
+    if (keep) dst[out + p] = val;
+    out += tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Undo and canonical labels.
+__global__ void k_undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_t *gid,
+                       const uint32_t *rep_up, uint32_t *rep_out) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t a = lab[v];
+  const uint32_t a2 = newidx[a];
+  rep_out[v] = a2 != kNone ? rep_up[a2] : (gid ? gid[a] : a);
+}
+
+__global__ void k_rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_t *rep) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t a = static_cast<uint32_t>(key[v]);
+  rep[v] = gid ? gid[a] : a;
+}
+
+__global__ void k_canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  atomicMin(&minid[rep[v]], static_cast<uint32_t>(v));
+}
+
+__global__ void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,
+                              uint32_t *roots) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  bool root = false;
+  if (v < n) {
+    const uint32_t l = minid[rep[v]];
+    labels[v] = l;
+    root = l == static_cast<uint32_t>(v);
+  }
+  const uint64_t m = __ballot(root);
+  if (m && lane_id() == 0) atomicAdd(roots, static_cast<uint32_t>(__popcll(m)));
+}
+
+__global__ void k_labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots) {
+  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  bool root = false;
+  if (v < n) {
+    const uint32_t l = static_cast<uint32_t>(key[v]);
+    labels[v] = l;
+    root = l == static_cast<uint32_t>(v);
+  }
+  const uint64_t m = __ballot(root);
+  if (m && lane_id() == 0) atomicAdd(roots, static_cast<uint32_t>(__popcll(m)));
+}
+
+}  // namespace
+
+// ===========================================================================
+// Launch wrappers.
+
+void fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_fill<uint32_t>), dim3(grid_for(n) > 8192 ? 8192 : grid_for(n)), dim3(kBlock), 0, s, p, n, v);
+  KC_LAUNCH_CHECK();
+}
+void fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_fill<uint64_t>), dim3(grid_for(n) > 8192 ? 8192 : grid_for(n)), dim3(kBlock), 0, s, p, n, v);
+  KC_LAUNCH_CHECK();
+}
+
+void init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, LearnedArgs la, uint64_t *key, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_init_keys_exp, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gid, d, la, key);
+  KC_LAUNCH_CHECK();
+}
+void init_keys_index(uint64_t n, uint64_t *key, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_init_keys_index, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key);
+  KC_LAUNCH_CHECK();
+}
+void init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t it, uint64_t *key, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_init_keys_uniform, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gid, seed, it, key);
+  KC_LAUNCH_CHECK();
+}
+void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_deviates_only, dim3(grid_for(count)), dim3(kBlock), 0, s, gid0, count, d, out);
+  KC_LAUNCH_CHECK();
+}
+
+void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS, uint32_t *listW,
+                  HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(n)), dim3(kBlock), 0, s, n, off, cfg, listT, listS, listW, hubs, counters,
+                     deg_inter);
+  KC_LAUNCH_CHECK();
+}
+
+void finish_bins(Bins &b) {
+  const uint32_t blocksT = static_cast<uint32_t>(ceil_div(b.nT, kBlock));
+  const uint32_t blocksS = static_cast<uint32_t>(ceil_div(b.nS, kBlock / 16));
+  const uint32_t blocksW = static_cast<uint32_t>(ceil_div(b.nW, kBlock / kWave));
+  const uint32_t blocksH = static_cast<uint32_t>(ceil_div(b.nH, kBlock / kWave));
+  b.bS = blocksT;
+  b.bW = b.bS + blocksS;
+  b.bH = b.bW + blocksW;
+  b.blocks = b.bH + blocksH;
+}
+
+void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin, uint64_t *kout,
+           uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter, int *flag, const int *prev_flag,
+           hipStream_t s) {
+  if (b.blocks == 0) return;
+  SweepArgs a{off, col, kin, kout, hop, b, deg_inter, hub_inter, flag, prev_flag};
+  dim3 g(b.blocks), t(kBlock);
+  if (mode == kSweepInPlace && count)
+    hipLaunchKernelGGL((k_sweep<kSweepInPlace, true>), g, t, 0, s, a);
+  else if (mode == kSweepInPlace)
+    hipLaunchKernelGGL((k_sweep<kSweepInPlace, false>), g, t, 0, s, a);
+  else if (mode == kSweepJacobi)
+    hipLaunchKernelGGL((k_sweep<kSweepJacobi, false>), g, t, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep<kSweepCountOnly, true>), g, t, 0, s, a);
+  KC_LAUNCH_CHECK();
+}
+
+void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, hipStream_t s) {
+  if (!b.nH) return;
+  hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, hub_inter, deg_inter);
+  KC_LAUNCH_CHECK();
+}
+
+void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_jump, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, flag);
+  KC_LAUNCH_CHECK();
+}
+
+void cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab, uint32_t *cnt,
+                   uint32_t *radius, bool with_radius, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cluster_count, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, deg_inter, lab, cnt, radius,
+                     with_radius ? 1 : 0);
+  KC_LAUNCH_CHECK();
+}
+
+uint64_t scan_scratch_words(uint64_t n) { return scan_words_for(n, 2) + 64; }
+
+void kept_scan(uint64_t n, const uint32_t *cnt, const uint32_t *gid, uint32_t *newidx, uint64_t *bucket_off,
+               uint32_t *gid_next, uint32_t *old_of, uint64_t *sizes, uint64_t *scratch, hipStream_t s) {
+  Pair64 *total = reinterpret_cast<Pair64 *>(scratch);
+  scan_exclusive<Pair64>(n, KeptLoad{cnt}, KeptStore{gid, newidx, bucket_off, gid_next, old_of}, total, scratch + 4, s);
+  hipLaunchKernelGGL(k_pair_to_sizes, dim3(1), dim3(1), 0, s, total, sizes);
+  KC_LAUNCH_CHECK();
+}
+
+void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, const uint32_t *deg_inter,
+             const uint32_t *hub_inter, const uint32_t *newidx, const uint64_t *bucket_off, uint32_t *fill,
+             uint32_t *bucket, hipStream_t s) {
+  if (b.blocks == 0) return;
+  ScatterArgs a{off, col, key, b, deg_inter, hub_inter, newidx, bucket_off, fill, bucket};
+  hipLaunchKernelGGL(k_scatter, dim3(b.blocks), dim3(kBlock), 0, s, a);
+  KC_LAUNCH_CHECK();
+}
+
+void dedup_small(uint64_t n_next, const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off,
+                 uint32_t *bucket, uint32_t *uniq, DedupLists l, hipStream_t s) {
+  if (!n_next) return;
+  uint64_t blocks = ceil_div(n_next, kBlock / kWave);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_dedup_small, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n_next, old_of, cnt,
+                     bucket_off, bucket, uniq, l);
+  KC_LAUNCH_CHECK();
+}
+
+void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint32_t *old_of, const uint32_t *cnt,
+                  const uint64_t *bucket_off, uint32_t *bucket, uint32_t *uniq, hipStream_t s) {
+  if (!n_medium) return;
+  unsigned blocks = n_medium > 1024 ? 1024 : n_medium;
+  hipLaunchKernelGGL(k_dedup_medium, dim3(blocks), dim3(kBlock), 0, s, n_medium, medium, old_of, cnt, bucket_off,
+                     bucket, uniq);
+  KC_LAUNCH_CHECK();
+}
+
+void dedup_huge(uint32_t n_huge, const uint32_t *huge, const uint64_t *huge_tab, uint32_t *tables,
+                const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off, uint32_t *bucket,
+                uint32_t *uniq, uint32_t *hcount, hipStream_t s) {
+  if (!n_huge) return;
+  dim3 g(64, n_huge < 4096 ? n_huge : 4096);
+  hipLaunchKernelGGL(k_dedup_huge_insert, g, dim3(kBlock), 0, s, n_huge, huge, huge_tab, tables, old_of, cnt,
+                     bucket_off, bucket);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dedup_huge_emit, g, dim3(kBlock), 0, s, n_huge, huge, huge_tab, tables, old_of, cnt, bucket_off,
+                     bucket, hcount);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dedup_huge_tail, g, dim3(kBlock), 0, s, n_huge, huge, old_of, cnt, bucket_off, bucket, uniq,
+                     hcount);
+  KC_LAUNCH_CHECK();
+}
+
+void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint64_t *sizes, uint64_t *scratch,
+                 hipStream_t s) {
+  if (n_next == 0) {
+    fill_u64(off_next, 1, 0, s);
+    fill_u64(sizes + 2, 1, 0, s);
+    return;
+  }
+  uint64_t *total = scratch;
+  scan_exclusive<uint64_t>(n_next, UniqLoad{uniq}, OffStore{off_next, n_next}, total, scratch + 4, s);
+  hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, total, sizes + 2);
+  KC_LAUNCH_CHECK();
+}
+
+void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, uint64_t *scratch, hipStream_t s) {
+  if (!total) return;
+  const uint64_t blocks = ceil_div(total, kScanTile);
+  uint64_t *counts = scratch;
+  uint64_t *prefix = scratch + blocks + 1;
+  uint64_t *tot = prefix + blocks + 1;
+  uint64_t *rest = tot + 4;
+  hipLaunchKernelGGL(k_compact_count, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, total, bucket, counts);
+  KC_LAUNCH_CHECK();
+  scan_exclusive<uint64_t>(blocks, ArrayLoad<uint64_t>{counts}, ArrayStore<uint64_t>{prefix}, tot, rest, s);
+  hipLaunchKernelGGL(k_compact_write, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, total, bucket, col_next,
+                     prefix);
+  KC_LAUNCH_CHECK();
+}
+
+void undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_t *gid, const uint32_t *rep_up,
+          uint32_t *rep_out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_undo, dim3(grid_for(n)), dim3(kBlock), 0, s, n, lab, newidx, gid, rep_up, rep_out);
+  KC_LAUNCH_CHECK();
+}
+
+void rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_t *rep, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_rep_from_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, gid, rep);
+  KC_LAUNCH_CHECK();
+}
+
+void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_canon_min, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, minid);
+  KC_LAUNCH_CHECK();
+}
+
+void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels, uint32_t *roots,
+                 hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_canon_label, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, minid, labels, roots);
+  KC_LAUNCH_CHECK();
+}
+
+void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_labels_from_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, labels, roots);
+  KC_LAUNCH_CHECK();
+}
+
+}  // namespace kc

@@ -1,0 +1,200 @@
+/*
+ * kacomp.h -- C ABI of libkacomp: connected components by exponential
+ * contraction on one MI355X (gfx950).
+ *
+ * The whole graph lives on one GPU as CSR (u64 row offsets, u32 neighbour
+ * ids, both directions of every undirected edge present).  Every entry point
+ * returns KC_OK (0) or a negative KC_ERR_* code; the message of the last
+ * failure on the calling thread is returned by kc_last_error().  No
+ * exceptions cross this boundary and nothing calls exit().
+ *
+ * Reference interfaces replaced (sebalamm/components):
+ *   kc_graph_create / kc_graph_destroy
+ *       StaticGraph construction by IOUtility::LoadGraph
+ *       (lib/io/io_utils.h:33-57, lib/data_structures/static_graph.h:80-134);
+ *       the graph is copied to the device once.
+ *   kc_find_components (variant KC_VARIANT_EXP)
+ *       ExponentialContraction::FindComponents(G, labels)
+ *       (lib/components/exponential_contraction.h:60-319)
+ *   kc_find_components (variant KC_VARIANT_LOCAL)
+ *       LocalContraction::FindComponents (lib/components/local_contraction.h:82-90)
+ *   kc_find_components (variant KC_VARIANT_SHORTCUT)
+ *       ShortcutPropagation::FindComponents (lib/components/shortcut_propagation.h:89-94)
+ *   kc_find_components (variant KC_VARIANT_LABELS)
+ *       Propagation::FindComponents (lib/components/propagation.h:48-53)
+ *   kc_stats.comm_time_s / send_volume / recv_volume
+ *       GetCommTime / GetSendVolume / GetReceiveVolume
+ *       (lib/components/exponential_contraction.h:325-335); always 0 on one GPU.
+ *   kc_read_metis / kc_write_metis
+ *       GraphIO::ReadMETISFile (lib/io/graph_io.h:281-382, :1207-1248)
+ *   kc_generate
+ *       IOUtility::GenerateSyntheticGraph (lib/io/io_utils.h:59-83); KaGen's
+ *       generators are not available, so the generators are the project's own
+ *       (counter-based SplitMix64 streams) with KaGen-style parameters.
+ *
+ * Labels written by kc_find_components are canonical: every vertex receives
+ * the minimum vertex id of its connected component (isolated vertices receive
+ * their own id).
+ */
+#ifndef KACOMP_H_
+#define KACOMP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KC_OK 0
+#define KC_ERR_INVALID (-1)     /* bad argument or malformed input */
+#define KC_ERR_HIP (-2)         /* HIP runtime failure (no device, launch error) */
+#define KC_ERR_OOM (-3)         /* device or host allocation failed */
+#define KC_ERR_IO (-4)          /* file could not be opened or parsed */
+#define KC_ERR_UNSUPPORTED (-5) /* option or generator not available */
+
+typedef enum kc_variant {
+  KC_VARIANT_EXP = 0,      /* exponential contraction (exp_contraction) */
+  KC_VARIANT_LOCAL = 1,    /* 2-hop local contraction (local_contraction) */
+  KC_VARIANT_SHORTCUT = 2, /* min-label propagation + pointer jumping (shortcuts) */
+  KC_VARIANT_LABELS = 3    /* plain min-label propagation (labels) */
+} kc_variant;
+
+/* Host CSR handed over by the caller.  offsets has n+1 entries; targets has
+ * offsets[n] entries, all < n.  Undirected edges appear in both rows. */
+typedef struct kc_csr_host {
+  uint64_t n;
+  const uint64_t *offsets;
+  const uint32_t *targets;
+} kc_csr_host;
+
+/* Host CSR owned by the library (results of kc_read_metis / kc_generate);
+ * free with kc_csr_free. */
+typedef struct kc_csr {
+  uint64_t n;
+  uint64_t m;        /* undirected edges = offsets[n] / 2 */
+  uint64_t *offsets; /* n+1 */
+  uint32_t *targets; /* offsets[n] */
+} kc_csr;
+
+/* Options of one kc_find_components call.  kc_options_init fills the
+ * reference's defaults (app/parse_parameters.h:30-100). */
+typedef struct kc_options {
+  int32_t variant;      /* kc_variant */
+  int32_t seed;         /* -seed (default 1); the benchmark harness passes seed+i+1000 */
+  double beta;          /* -beta (default 0.1), rate of the exponential deviates */
+  uint64_t seq_limit;   /* base case once remaining undirected edges <= seq_limit
+                           (default 4096; 0 = contract until no edge is left) */
+  int32_t use_bfs;      /* -bfs: time-stepped frontier schedule of the decomposition */
+  int32_t deviate_fma;  /* 1: canonical draw summed with a fused multiply-add, as
+                           libstdc++ compiled with -O3 -march=native on FMA hardware;
+                           0: separate multiply and add */
+  int32_t learned;      /* 1: centre proposals scored by the MLP loaded with kc_mlp_set */
+  int32_t record_trace; /* keep per-level sizes/keys for kc_trace_* */
+  int32_t time_kernels; /* bracket every launch with HIP events (per-kernel stats) */
+  int32_t max_levels;   /* safety cap on contraction levels (default 4096) */
+  int32_t sweep_group_small; /* tuning: degree bin thresholds (0 = built-in defaults) */
+  int32_t sweep_group_mid;
+  int32_t sweep_group_large;
+  int32_t deviate_stream; /* KC_STREAM_REFERENCE (default) or KC_STREAM_HASHED */
+  int32_t reserved[6];
+} kc_options;
+
+/* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit
+ * (per-vertex minstd_rand0 seeds; its outputs are linear in the seed, so the
+ * deviates of the surviving centres correlate and some levels contract
+ * nothing).  HASHED: independent SplitMix64 draws of the same exponential law. */
+#define KC_STREAM_REFERENCE 0
+#define KC_STREAM_HASHED 1
+
+#define KC_KERNEL_CLASSES 12
+/* Kernel classes accounted in kc_stats.kernel_* (index = class). */
+#define KC_K_DEVIATES 0  /* deviate sampling + key init */
+#define KC_K_SWEEP 1     /* hooking sweep (decomposition) */
+#define KC_K_BIN 2       /* degree binning */
+#define KC_K_COUNT 3     /* cluster edge counting */
+#define KC_K_SCAN 4      /* prefix sums */
+#define KC_K_SCATTER 5   /* relabel: scatter cluster edges into buckets */
+#define KC_K_DEDUP 6     /* LDS-staged bucket dedup */
+#define KC_K_CSR 7       /* next-level CSR compaction */
+#define KC_K_UNDO 8      /* undo gather */
+#define KC_K_CANON 9     /* canonical min-id labels */
+#define KC_K_JUMP 10     /* pointer jumping */
+#define KC_K_OTHER 11
+
+typedef struct kc_stats {
+  double time_ms;       /* device time of the whole call (HIP events), labels resident on device */
+  double total_ms;      /* host wall time of the call including the label copy-out */
+  double comm_time_s;   /* always 0: no inter-process communication on one GPU */
+  uint64_t send_volume; /* always 0 */
+  uint64_t recv_volume; /* always 0 */
+  uint64_t n, m;        /* vertices, undirected edges of the input */
+  uint32_t levels;      /* contraction levels (decompositions) executed */
+  uint32_t components;  /* number of connected components (isolated vertices count) */
+  uint64_t sweeps;      /* sum over levels of sweeps executed */
+  uint64_t radius_sum;  /* sum over levels of the maximum cluster radius */
+  uint64_t alg_bytes;   /* algorithmic bytes of all launched kernels */
+  double kernel_ms[KC_KERNEL_CLASSES];        /* valid with time_kernels */
+  uint64_t kernel_launches[KC_KERNEL_CLASSES];
+  uint64_t kernel_bytes[KC_KERNEL_CLASSES];   /* algorithmic bytes per class */
+} kc_stats;
+
+typedef struct kc_graph kc_graph;
+
+const char *kc_last_error(void);
+void kc_options_init(kc_options *opt);
+
+/* Device graph. */
+int kc_graph_create(const kc_csr_host *csr, kc_graph **out);
+int kc_graph_destroy(kc_graph *g);
+int kc_graph_info(const kc_graph *g, uint64_t *n, uint64_t *m, uint64_t *non_isolated);
+
+/* Runs the selected variant.  labels_out (host, n entries) may be NULL: the
+ * labels then stay on the device (kc_copy_labels fetches them). */
+int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out, kc_stats *stats);
+int kc_copy_labels(kc_graph *g, uint32_t *labels_out);
+
+/* Per-level trace of the last kc_find_components call with record_trace=1.
+ * level_sizes: n_l, m_l (undirected), sweeps_l, radius_l for level l. */
+int kc_trace_levels(const kc_graph *g, uint32_t *levels);
+int kc_trace_level_info(const kc_graph *g, uint32_t level, uint64_t info[4]);
+/* keys_out: n_l u64 keys (deviate<<32 | centre index) at the fixed point;
+ * gid_out: n_l original ids of the level's vertices. */
+int kc_trace_level_keys(const kc_graph *g, uint32_t level, uint64_t *keys_out, uint32_t *gid_out);
+
+/* Deviate of vertex gid at contraction iteration it (1-based) exactly as
+ * RunContractionLP draws it (lib/components/exponential_contraction.h:467-471):
+ * minstd_rand0 seeded with (unsigned)(gid + seed + it*(rng_offset+seed)),
+ * exponential_distribution<double>(beta), truncated.  Computed on the device
+ * for count consecutive gids starting at gid0 (test and trace helper);
+ * stream KC_STREAM_HASHED gives the hashed draw instead. */
+int kc_deviates(uint32_t gid0, uint32_t count, int32_t seed, uint32_t iteration, uint64_t rng_offset,
+                double beta, int32_t fma, int32_t stream, uint32_t *out);
+
+/* Host I/O and generators. */
+int kc_read_metis(const char *path, kc_csr *out);
+int kc_write_metis(const char *path, const kc_csr *g);
+/* name: gnm_undirected, rgg_2d, rgg_3d, grid_2d, grid_3d, rmat, rhg, ba.
+ * params (KaGen-style, app/parse_parameters.h:83-99): n, m (exact counts),
+ * r, p, gamma, d, periodic, plus rmat_a/b/c. */
+typedef struct kc_gen_params {
+  uint64_t n;
+  uint64_t m;
+  double r;
+  double p;
+  double gamma;
+  double d;
+  int32_t periodic;
+  int32_t seed;
+  double rmat_a, rmat_b, rmat_c;
+  uint64_t nx, ny, nz; /* grid dimensions (grid_2d: nx=n, ny=m) */
+} kc_gen_params;
+void kc_gen_params_init(kc_gen_params *p);
+int kc_generate(const char *name, const kc_gen_params *p, kc_csr *out);
+void kc_csr_free(kc_csr *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KACOMP_H_ */

@@ -1,0 +1,143 @@
+"""ctypes view of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker and the CPU baseline.  The product
+path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_P = ctypes.POINTER
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        u64, u32, i32, dbl = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double
+        sig = {
+            "or_canonical_labels": (None, [u64, _P(u64), _P(u32), _P(u32)]),
+            "or_exp_deviate": (u32, [u32, i32, u32, u64, dbl, i32]),
+            "or_std_deviate_nofma": (u64, [u32, i32, u32, u64, dbl]),
+            "or_std_deviate_fma": (u64, [u32, i32, u32, u64, dbl]),
+            "or_std_fma_compiled": (i32, []),
+            "or_lp_fixpoint": (u64, [u64, _P(u64), _P(u32), _P(u64), u64, _P(u64)]),
+            "or_lp_closed_form": (None, [u64, _P(u64), _P(u32), _P(u64), u64, _P(u64)]),
+            "or_run": (ctypes.c_void_p, [u64, _P(u64), _P(u32), i32, i32, dbl, i32, i32, u64, i32]),
+            "or_hashed_deviate": (u32, [u32, i32, u32, dbl]),
+            "or_free": (None, [ctypes.c_void_p]),
+            "or_labels": (None, [ctypes.c_void_p, _P(u32)]),
+            "or_levels": (u32, [ctypes.c_void_p]),
+            "or_level_info": (None, [ctypes.c_void_p, u32, _P(u64)]),
+            "or_level_keys": (None, [ctypes.c_void_p, u32, _P(u64), _P(u32)]),
+            "or_total_sweeps": (u64, [ctypes.c_void_p]),
+            "or_seconds": (dbl, [ctypes.c_void_p]),
+            "or_uniform_deviate_100": (u32, [u32, i32, u32]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _arrs(offsets, targets):
+    off = np.ascontiguousarray(offsets, np.uint64)
+    tgt = np.ascontiguousarray(targets, np.uint32)
+    if len(tgt) == 0:
+        tgt = np.zeros(1, np.uint32)
+    return off, tgt
+
+
+def canonical_labels(offsets, targets):
+    off, tgt = _arrs(offsets, targets)
+    n = len(off) - 1
+    out = np.empty(max(n, 1), np.uint32)
+    lib().or_canonical_labels(n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
+                              out.ctypes.data_as(_P(ctypes.c_uint32)))
+    return out[:n]
+
+
+def lp_fixpoint(offsets, targets, key0, hop=1 << 32):
+    off, tgt = _arrs(offsets, targets)
+    k0 = np.ascontiguousarray(key0, np.uint64)
+    out = np.empty(max(len(k0), 1), np.uint64)
+    r = lib().or_lp_fixpoint(len(off) - 1, off.ctypes.data_as(_P(ctypes.c_uint64)),
+                             tgt.ctypes.data_as(_P(ctypes.c_uint32)), k0.ctypes.data_as(_P(ctypes.c_uint64)), hop,
+                             out.ctypes.data_as(_P(ctypes.c_uint64)))
+    return out[: len(k0)], int(r)
+
+
+def lp_closed_form(offsets, targets, key0, hop=1 << 32):
+    off, tgt = _arrs(offsets, targets)
+    k0 = np.ascontiguousarray(key0, np.uint64)
+    out = np.empty(max(len(k0), 1), np.uint64)
+    lib().or_lp_closed_form(len(off) - 1, off.ctypes.data_as(_P(ctypes.c_uint64)),
+                            tgt.ctypes.data_as(_P(ctypes.c_uint32)), k0.ctypes.data_as(_P(ctypes.c_uint64)), hop,
+                            out.ctypes.data_as(_P(ctypes.c_uint64)))
+    return out[: len(k0)]
+
+
+def exp_deviate(gid, seed, iteration, rng_offset, beta=0.1, fma=True):
+    return int(lib().or_exp_deviate(gid, seed, iteration, rng_offset, beta, 1 if fma else 0))
+
+
+def hashed_deviate(gid, seed, iteration, beta=0.1):
+    return int(lib().or_hashed_deviate(gid, seed, iteration, beta))
+
+
+class Run:
+    """Result of or_run: labels and per-level trace."""
+
+    VARIANTS = {"exp": 0, "local": 1, "shortcut": 2, "labels": 3}
+
+    def __init__(self, offsets, targets, variant="exp", seed=1, beta=0.1, fma=True, seq_limit=4096, keep_keys=False,
+                 stream=0):
+        off, tgt = _arrs(offsets, targets)
+        self.n = len(off) - 1
+        v = self.VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        self.h = lib().or_run(self.n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
+                              v, seed, beta, 1 if fma else 0, stream, seq_limit, 1 if keep_keys else 0)
+
+    def labels(self):
+        out = np.empty(max(self.n, 1), np.uint32)
+        lib().or_labels(self.h, out.ctypes.data_as(_P(ctypes.c_uint32)))
+        return out[: self.n]
+
+    def levels(self):
+        out = []
+        for lvl in range(lib().or_levels(self.h)):
+            info = (ctypes.c_uint64 * 4)()
+            lib().or_level_info(self.h, lvl, info)
+            out.append(tuple(int(x) for x in info))
+        return out
+
+    def level_keys(self, lvl):
+        n = self.levels()[lvl][0]
+        keys = np.empty(max(n, 1), np.uint64)
+        gid = np.empty(max(n, 1), np.uint32)
+        lib().or_level_keys(self.h, lvl, keys.ctypes.data_as(_P(ctypes.c_uint64)),
+                            gid.ctypes.data_as(_P(ctypes.c_uint32)))
+        return keys[:n], gid[:n]
+
+    def sweeps(self):
+        return int(lib().or_total_sweeps(self.h))
+
+    def seconds(self):
+        return float(lib().or_seconds(self.h))
+
+    def __del__(self):
+        try:
+            lib().or_free(self.h)
+        except Exception:
+            pass

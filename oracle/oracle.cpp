@@ -1,0 +1,387 @@
+// CPU restatement of the connected-components contraction path of
+// sebalamm/components, in the limit of one vertex per PE (every vertex is an
+// interface vertex, so the local-component pre-contraction of
+// CAGBuilder::BuildLocalComponentGraph keeps the whole graph and exponential
+// contraction runs on it).  TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// Compiled with -ffp-contract=off: every fused multiply-add is explicit.
+#include "oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <queue>
+#include <utility>
+#include <vector>
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint64_t kHop = 1ull << 32;
+
+struct Csr {
+  uint64_t n = 0;
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> tgt;
+};
+
+// Union-find with union by smaller root id and path halving; the root of each
+// set is its minimum member, which is the canonical label.
+struct MinUnionFind {
+  std::vector<uint32_t> p;
+  explicit MinUnionFind(uint64_t n) : p(n) {
+    for (uint64_t i = 0; i < n; ++i) p[i] = static_cast<uint32_t>(i);
+  }
+  uint32_t find(uint32_t x) {
+    while (p[x] != x) {
+      p[x] = p[p[x]];
+      x = p[x];
+    }
+    return x;
+  }
+  void unite(uint32_t a, uint32_t b) {
+    a = find(a);
+    b = find(b);
+    if (a == b) return;
+    if (a < b)
+      p[b] = a;
+    else
+      p[a] = b;
+  }
+};
+
+void canonical(uint64_t n, const uint64_t *off, const uint32_t *tgt, uint32_t *labels) {
+  MinUnionFind uf(n);
+  for (uint64_t v = 0; v < n; ++v)
+    for (uint64_t e = off[v]; e < off[v + 1]; ++e) uf.unite(static_cast<uint32_t>(v), tgt[e]);
+  for (uint64_t v = 0; v < n; ++v) labels[v] = uf.find(static_cast<uint32_t>(v));
+}
+
+// exponential_contraction.h:467-471, restated: minstd_rand0 seeded with
+// (unsigned)(gid + seed + iteration*(rng_offset + seed)), two outputs combined
+// as in generate_canonical<double,53> (C++11 [rand.util.canonical]: k = 2 calls
+// of 30 bits for R = 2^31-2), then -log(1-u)/beta truncated.
+uint32_t exp_deviate(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rng_offset, double beta, bool fma) {
+  const uint64_t g = gid;
+  const uint64_t s64 = g + static_cast<uint64_t>(static_cast<int64_t>(seed)) +
+                       static_cast<uint64_t>(iteration) * (rng_offset + static_cast<uint64_t>(static_cast<int64_t>(seed)));
+  const uint32_t s = static_cast<uint32_t>(s64);
+  const uint64_t m = 2147483647ull;
+  uint64_t x = s % m;
+  if (x == 0) x = 1;
+  x = (x * 16807ull) % m;
+  const double g1 = static_cast<double>(x - 1);
+  x = (x * 16807ull) % m;
+  const double g2 = static_cast<double>(x - 1);
+  const double r = static_cast<double>(m - 1);  // max - min + 1
+  double sum = 0.0;
+  double tmp = 1.0;
+  sum = sum + g1 * tmp;
+  tmp = tmp * r;
+  if (fma)
+    sum = std::fma(g2, tmp, sum);
+  else
+    sum = sum + g2 * tmp;
+  tmp = tmp * r;
+  double u = sum / tmp;
+  if (u >= 1.0) u = std::nextafter(1.0, 0.0);
+  const double dev = -std::log(1.0 - u) / beta;
+  return static_cast<uint32_t>(static_cast<uint64_t>(1.0 * dev));
+}
+
+// Hashed stream of the library (SplitMix64 over seed, iteration, gid), then
+// the exponential law evaluated directly with std::log.
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+uint32_t hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta) {
+  const uint64_t key = ((static_cast<uint64_t>(static_cast<uint32_t>(seed)) << 32) | iteration) * 0x9e3779b97f4a7c15ull;
+  const uint64_t z = mix64(mix64(key) + static_cast<uint64_t>(gid) * 0xd1b54a32d192ed03ull);
+  const double u = static_cast<double>(z >> 11) * 0x1p-53;
+  return static_cast<uint32_t>(static_cast<uint64_t>(-std::log(1.0 - u) / beta));
+}
+
+uint32_t uniform100(uint32_t gid, int32_t seed, uint32_t iteration) {
+  auto mix = [](uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  const uint64_t z = mix((0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(static_cast<uint32_t>(seed)) + 1)) ^
+                         (static_cast<uint64_t>(iteration) << 32) ^ gid);
+  return static_cast<uint32_t>(((z >> 32) * 100ull) >> 32);
+}
+
+// The LP sweep loop, in place, vertex order (one PE holds every vertex).
+uint64_t lp_fixpoint(uint64_t n, const uint64_t *off, const uint32_t *tgt, std::vector<uint64_t> &key, uint64_t hop) {
+  uint64_t rounds = 0;
+  bool converged = false;
+  while (!converged) {
+    converged = true;
+    for (uint64_t v = 0; v < n; ++v) {
+      uint64_t best = key[v];
+      for (uint64_t e = off[v]; e < off[v + 1]; ++e) {
+        const uint64_t cand = key[tgt[e]] + hop;
+        if (cand < best) {
+          best = cand;
+          converged = false;
+        }
+      }
+      key[v] = best;
+    }
+    ++rounds;
+  }
+  return rounds;
+}
+
+void lp_closed_form(uint64_t n, const uint64_t *off, const uint32_t *tgt, const uint64_t *key0, uint64_t hop,
+                    uint64_t *out) {
+  std::vector<uint64_t> dist(key0, key0 + n);
+  using Item = std::pair<uint64_t, uint32_t>;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+  for (uint64_t v = 0; v < n; ++v) pq.push({dist[v], static_cast<uint32_t>(v)});
+  while (!pq.empty()) {
+    auto [d, v] = pq.top();
+    pq.pop();
+    if (d != dist[v]) continue;
+    for (uint64_t e = off[v]; e < off[v + 1]; ++e) {
+      const uint32_t w = tgt[e];
+      if (d + hop < dist[w]) {
+        dist[w] = d + hop;
+        pq.push({dist[w], w});
+      }
+    }
+  }
+  std::copy(dist.begin(), dist.end(), out);
+}
+
+}  // namespace
+
+struct LevelTrace {
+  uint64_t n = 0, m = 0, sweeps = 0;
+  uint32_t radius = 0;
+  std::vector<uint64_t> keys;
+  std::vector<uint32_t> gid, lab, newidx;
+};
+
+struct or_result {
+  std::vector<uint32_t> labels;
+  std::vector<LevelTrace> levels;
+  uint64_t sweeps = 0;
+  double seconds = 0.0;
+};
+
+namespace {
+
+// Contracted graph of one level (DynamicContraction::ExponentialContraction,
+// dynamic_contraction.h:62-136 with FindExponentialConflictingEdges :251-290
+// and InsertAndClearEdges :513-559): vertices are the centres that keep an
+// inter-cluster edge, edges the distinct centre pairs.
+void contract(const Csr &g, const std::vector<uint32_t> &lab, std::vector<uint32_t> &newidx, Csr &next) {
+  std::vector<uint64_t> pairs;
+  for (uint64_t v = 0; v < g.n; ++v)
+    for (uint64_t e = g.off[v]; e < g.off[v + 1]; ++e) {
+      const uint32_t a = lab[v], b = lab[g.tgt[e]];
+      if (a != b) pairs.push_back((static_cast<uint64_t>(a) << 32) | b);
+    }
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+  newidx.assign(g.n, kNone);
+  uint32_t k = 0;
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    const uint32_t a = static_cast<uint32_t>(pairs[i] >> 32);
+    if (newidx[a] == kNone) newidx[a] = k++;
+  }
+  next.n = k;
+  next.off.assign(k + 1, 0);
+  next.tgt.resize(pairs.size());
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    const uint32_t a = newidx[pairs[i] >> 32];
+    next.off[a + 1]++;
+    next.tgt[i] = newidx[static_cast<uint32_t>(pairs[i])];
+  }
+  for (uint32_t a = 0; a < k; ++a) next.off[a + 1] += next.off[a];
+}
+
+void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool fma, int stream,
+                     uint64_t seq_limit, bool keep_keys, or_result &res) {
+  uint64_t rng_offset = 0;
+  for (uint64_t v = 0; v < g0.n; ++v) rng_offset += g0.off[v + 1] > g0.off[v];
+  Csr cur = g0;
+  std::vector<uint32_t> gid(g0.n);
+  for (uint64_t v = 0; v < g0.n; ++v) gid[v] = static_cast<uint32_t>(v);
+  uint32_t iteration = 1;
+  std::vector<uint32_t> top_rep;
+  bool have_top = false;
+  while (cur.n > 0) {
+    const uint64_t m2 = cur.off[cur.n];
+    if (m2 == 0 || (seq_limit > 0 && m2 / 2 <= seq_limit)) {
+      // RunSequentialCC (exponential_contraction.h:908-922): components of the
+      // remaining graph, represented by their smallest member.
+      std::vector<uint32_t> l(cur.n);
+      canonical(cur.n, cur.off.data(), cur.tgt.data(), l.data());
+      top_rep.resize(cur.n);
+      for (uint64_t v = 0; v < cur.n; ++v) top_rep[v] = gid[l[v]];
+      have_top = true;
+      break;
+    }
+    LevelTrace t;
+    t.n = cur.n;
+    t.m = m2 / 2;
+    std::vector<uint64_t> key(cur.n);
+    if (variant == 1) {
+      // LocalContraction::RunContraction (local_contraction.h:157-300): uniform
+      // deviates, min over N[v], then min over N[N[v]], both synchronous.
+      for (uint64_t v = 0; v < cur.n; ++v)
+        key[v] = (static_cast<uint64_t>(uniform100(gid[v], seed, iteration)) << 32) | v;
+      for (int pass = 0; pass < 2; ++pass) {
+        std::vector<uint64_t> nk(key);
+        for (uint64_t v = 0; v < cur.n; ++v)
+          for (uint64_t e = cur.off[v]; e < cur.off[v + 1]; ++e) nk[v] = std::min(nk[v], key[cur.tgt[e]]);
+        key.swap(nk);
+      }
+      t.sweeps = 2;
+    } else {
+      for (uint64_t v = 0; v < cur.n; ++v)
+        key[v] = (static_cast<uint64_t>(stream == 0 ? exp_deviate(gid[v], seed, iteration, rng_offset, beta, fma)
+                                                    : hashed_deviate(gid[v], seed, iteration, beta))
+                  << 32) |
+                 v;
+      t.sweeps = lp_fixpoint(cur.n, cur.off.data(), cur.tgt.data(), key, kHop);
+    }
+    t.lab.resize(cur.n);
+    for (uint64_t v = 0; v < cur.n; ++v) {
+      t.lab[v] = static_cast<uint32_t>(key[v]);
+      if (variant != 1) {
+        const uint32_t r = static_cast<uint32_t>(key[v] >> 32) - static_cast<uint32_t>(key[t.lab[v]] >> 32);
+        t.radius = std::max(t.radius, r);
+      }
+    }
+    Csr next;
+    contract(cur, t.lab, t.newidx, next);
+    std::vector<uint32_t> gid_next(next.n);
+    for (uint64_t a = 0; a < cur.n; ++a)
+      if (t.newidx[a] != kNone) gid_next[t.newidx[a]] = gid[a];
+    t.gid = gid;
+    if (keep_keys) t.keys = key;
+    res.sweeps += t.sweeps;
+    res.levels.push_back(std::move(t));
+    cur = std::move(next);
+    gid.swap(gid_next);
+    ++iteration;
+  }
+  // UndoContraction (dynamic_contraction.h:568-658): every vertex takes the
+  // representative of its centre one level up, or the centre itself when the
+  // cluster was a whole component.
+  std::vector<uint32_t> rep = top_rep;
+  for (int l = static_cast<int>(res.levels.size()) - 1; l >= 0; --l) {
+    const LevelTrace &t = res.levels[l];
+    std::vector<uint32_t> down(t.n);
+    for (uint64_t v = 0; v < t.n; ++v) {
+      const uint32_t a = t.lab[v];
+      down[v] = t.newidx[a] != kNone ? rep[t.newidx[a]] : t.gid[a];
+    }
+    rep.swap(down);
+  }
+  (void)have_top;
+  // ApplyToLocalComponents with canonical (minimum id) labels.
+  res.labels.assign(g0.n, 0);
+  std::vector<uint32_t> minid(g0.n, kNone);
+  for (uint64_t v = 0; v < g0.n; ++v) minid[rep[v]] = std::min<uint32_t>(minid[rep[v]], static_cast<uint32_t>(v));
+  for (uint64_t v = 0; v < g0.n; ++v) res.labels[v] = minid[rep[v]];
+}
+
+// Propagation (propagation.h:72-149) and ShortcutPropagation
+// (shortcut_propagation.h:124-158): synchronous min-label rounds, the latter
+// followed by one pointer jump l(v) <- min(l(v), l(l(v))) per round.
+void run_propagation(const Csr &g, bool shortcut, or_result &res) {
+  std::vector<uint32_t> label(g.n), next(g.n);
+  for (uint64_t v = 0; v < g.n; ++v) label[v] = static_cast<uint32_t>(v);
+  uint64_t rounds = 0;
+  while (true) {
+    for (uint64_t v = 0; v < g.n; ++v) {
+      uint32_t best = label[v];
+      for (uint64_t e = g.off[v]; e < g.off[v + 1]; ++e) best = std::min(best, label[g.tgt[e]]);
+      next[v] = best;
+    }
+    if (shortcut)
+      for (uint64_t v = 0; v < g.n; ++v) next[v] = std::min(next[v], label[next[v]]);
+    ++rounds;
+    if (next == label) break;
+    label.swap(next);
+  }
+  res.sweeps = rounds;
+  res.labels = label;
+}
+
+}  // namespace
+
+extern "C" {
+
+void or_canonical_labels(uint64_t n, const uint64_t *off, const uint32_t *tgt, uint32_t *labels) {
+  canonical(n, off, tgt, labels);
+}
+
+uint32_t or_exp_deviate(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rng_offset, double beta, int32_t fma) {
+  return exp_deviate(gid, seed, iteration, rng_offset, beta, fma != 0);
+}
+
+uint32_t or_hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta) {
+  return hashed_deviate(gid, seed, iteration, beta);
+}
+
+uint32_t or_uniform_deviate_100(uint32_t gid, int32_t seed, uint32_t iteration) {
+  return uniform100(gid, seed, iteration);
+}
+
+uint64_t or_lp_fixpoint(uint64_t n, const uint64_t *off, const uint32_t *tgt, const uint64_t *key0, uint64_t hop,
+                        uint64_t *key_out) {
+  std::vector<uint64_t> key(key0, key0 + n);
+  uint64_t r = lp_fixpoint(n, off, tgt, key, hop);
+  std::copy(key.begin(), key.end(), key_out);
+  return r;
+}
+
+void or_lp_closed_form(uint64_t n, const uint64_t *off, const uint32_t *tgt, const uint64_t *key0, uint64_t hop,
+                       uint64_t *key_out) {
+  lp_closed_form(n, off, tgt, key0, hop, key_out);
+}
+
+or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t variant, int32_t seed, double beta,
+                  int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys) {
+  Csr g;
+  g.n = n;
+  g.off.assign(off, off + n + 1);
+  g.tgt.assign(tgt, tgt + off[n]);
+  or_result *r = new or_result();
+  const auto t0 = std::chrono::steady_clock::now();
+  if (variant == 0 || variant == 1)
+    run_contraction(g, variant, seed, beta, fma != 0, stream, seq_limit, keep_keys != 0, *r);
+  else
+    run_propagation(g, variant == 2, *r);
+  r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return r;
+}
+
+void or_free(or_result *r) { delete r; }
+void or_labels(const or_result *r, uint32_t *labels) { std::copy(r->labels.begin(), r->labels.end(), labels); }
+uint32_t or_levels(const or_result *r) { return static_cast<uint32_t>(r->levels.size()); }
+void or_level_info(const or_result *r, uint32_t level, uint64_t info[4]) {
+  const LevelTrace &t = r->levels[level];
+  info[0] = t.n;
+  info[1] = t.m;
+  info[2] = t.sweeps;
+  info[3] = t.radius;
+}
+void or_level_keys(const or_result *r, uint32_t level, uint64_t *keys, uint32_t *gid) {
+  const LevelTrace &t = r->levels[level];
+  if (keys) std::copy(t.keys.begin(), t.keys.end(), keys);
+  if (gid) std::copy(t.gid.begin(), t.gid.end(), gid);
+}
+uint64_t or_total_sweeps(const or_result *r) { return r->sweeps; }
+double or_seconds(const or_result *r) { return r->seconds; }
+
+}  // extern "C"

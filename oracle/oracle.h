@@ -1,0 +1,73 @@
+/*
+ * oracle.h -- CPU restatement of the reference's connected-components
+ * contraction path.  TEST INFRASTRUCTURE ONLY: it is linked or called by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, as
+ * the checker and as the CPU baseline; the product (libkacomp) never calls it.
+ *
+ * Pinned by (tests/test_oracle.py):
+ *   or_canonical_labels   closed form (min id per component) against brute force
+ *                         BFS on every graph with <= 6 vertices, invariants, and the
+ *                         reference fixture example/simple.graph -> [0,0,0,3,3,3]
+ *   or_exp_deviate        against std::minstd_rand0 + std::exponential_distribution
+ *                         (the calls of exponential_contraction.h:467-471) built with
+ *                         and without fused multiply-add contraction
+ *   or_lp_fixpoint        against or_lp_closed_form (multi-source Dijkstra over the
+ *                         same deviates), which is the fixed point's closed form
+ *   or_run (EXP/LOCAL)    labels against or_canonical_labels; per-level sizes and
+ *                         keys against independent recomputation in the tests
+ *   or_run (LABELS/SHORTCUT) labels against or_canonical_labels
+ */
+#ifndef KC_ORACLE_H_
+#define KC_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Canonical labels: minimum vertex id of each component (union-find). */
+void or_canonical_labels(uint64_t n, const uint64_t *off, const uint32_t *tgt, uint32_t *labels);
+
+/* Deviate of RunContractionLP restated step by step (fma selects the fused
+ * canonical sum). */
+uint32_t or_exp_deviate(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rng_offset, double beta, int32_t fma);
+/* The same draw through the standard library objects the reference uses,
+ * compiled with contraction off / with FMA contraction. */
+uint64_t or_std_deviate_nofma(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rng_offset, double beta);
+uint64_t or_std_deviate_fma(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rng_offset, double beta);
+int32_t or_std_fma_compiled(void);
+
+/* LP sweep loop (exponential_contraction.h:518-586) from key0 to its fixed
+ * point, vertex order, in place.  Returns the number of sweeps (exchange
+ * rounds, the last one without change). */
+uint64_t or_lp_fixpoint(uint64_t n, const uint64_t *off, const uint32_t *tgt, const uint64_t *key0, uint64_t hop,
+                        uint64_t *key_out);
+/* Closed form of the same fixed point: key(v) = min_u (key0(u) + d(u,v)*hop). */
+void or_lp_closed_form(uint64_t n, const uint64_t *off, const uint32_t *tgt, const uint64_t *key0, uint64_t hop,
+                       uint64_t *key_out);
+
+/* Full runs.  variant: 0 exp, 1 local, 2 shortcut, 3 labels (kc_variant). */
+typedef struct or_result or_result;
+/* stream: 0 reference draw, 1 hashed draw (the library's KC_STREAM_*). */
+or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t variant, int32_t seed, double beta,
+                  int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys);
+void or_free(or_result *r);
+void or_labels(const or_result *r, uint32_t *labels);
+uint32_t or_levels(const or_result *r);
+/* info: n_l, m_l (undirected), sweeps_l, max radius_l */
+void or_level_info(const or_result *r, uint32_t level, uint64_t info[4]);
+void or_level_keys(const or_result *r, uint32_t level, uint64_t *keys, uint32_t *gid);
+uint64_t or_total_sweeps(const or_result *r);
+double or_seconds(const or_result *r);
+
+/* Hashed-stream deviate: SplitMix64 uniform, then -log(1-u)/beta with std::log. */
+uint32_t or_hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta);
+/* Uniform deviate in [0,99] used by local contraction (SplitMix64). */
+uint32_t or_uniform_deviate_100(uint32_t gid, int32_t seed, uint32_t iteration);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,216 @@
+"""GPU path through the C ABI against the oracle (bit-exact for all integer work)."""
+import itertools
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import components_amd as ca
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+VARIANTS = ["exp", "local", "shortcut", "labels"]
+
+
+def union(graphs):
+    """Disjoint union: canonical labels of the union are the per-graph labels plus offsets."""
+    offs, tgts, base_v, base_e = [np.zeros(1, np.uint64)], [], 0, 0
+    for g in graphs:
+        offs.append(g.offsets[1:].astype(np.uint64) + np.uint64(base_e))
+        tgts.append(g.targets.astype(np.uint32) + np.uint32(base_v))
+        base_v += g.n
+        base_e += int(g.offsets[-1])
+    return ca.Graph(np.concatenate(offs), np.concatenate(tgts) if tgts else np.zeros(0, np.uint32))
+
+
+def rand_graph(rng, n, m):
+    return ca.Graph.from_edges(n, rng.integers(0, n, size=(m, 2)))
+
+
+def check_labels(g, variants=VARIANTS, **kw):
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    dg = ca.DeviceGraph(g)
+    try:
+        for v in variants:
+            lab, st = dg.find_components(variant=v, **kw)
+            assert np.array_equal(lab, ref), f"variant {v}"
+            assert st["components"] == len(np.unique(ref))
+    finally:
+        dg.close()
+
+
+def test_deviates_golden_vectors():
+    with open(os.path.join(GOLDEN, "deviates.json")) as f:
+        cases = json.load(f)["cases"]
+    for c in cases:
+        for fma, key in ((True, "fma"), (False, "nofma")):
+            got = ca.deviates(c["gid0"], c["count"], c["seed"], c["iteration"], c["rng_offset"], c["beta"], fma)
+            assert list(got) == c[key]
+
+
+def test_deviates_bulk_against_oracle():
+    rng = np.random.default_rng(1)
+    for trial in range(4):
+        gid0 = int(rng.integers(0, 1 << 31))
+        seed, it, off = int(rng.integers(0, 1 << 20)), int(rng.integers(1, 50)), int(rng.integers(0, 1 << 32))
+        beta = [0.1, 0.05, 0.7, 0.013][trial]
+        got = ca.deviates(gid0, 50000, seed, it, off, beta, True)
+        want = [oracle.exp_deviate(gid0 + i, seed, it, off, beta, True) for i in range(50000)]
+        assert np.array_equal(got, np.array(want, np.uint32))
+        got = ca.deviates(gid0, 20000, seed, it, off, beta, True, ca.STREAM_HASHED)
+        want = [oracle.hashed_deviate(gid0 + i, seed, it, beta) for i in range(20000)]
+        assert np.array_equal(got, np.array(want, np.uint32))
+
+
+def test_simple_graph_fixture():
+    g = ca.read_metis(os.path.join(GOLDEN, "simple.graph"))
+    for v in VARIANTS:
+        lab, _ = ca.find_components(g, variant=v, seed=1001)
+        assert list(lab) == [0, 0, 0, 3, 3, 3]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+def test_all_graphs_up_to_5_vertices(n):
+    pairs = list(itertools.combinations(range(n), 2))
+    graphs = []
+    for mask in range(1 << len(pairs)):
+        graphs.append(ca.Graph.from_edges(n, [p for i, p in enumerate(pairs) if mask >> i & 1]))
+    g = union(graphs)
+    for seq in (0, 4096):
+        check_labels(g, seed=1001, seq_limit=seq, stream=ca.STREAM_HASHED)
+    check_labels(g, variants=["exp"], seed=1001, seq_limit=4096)
+
+
+def test_sampled_graphs_6_7_vertices():
+    rng = np.random.default_rng(7)
+    graphs = []
+    for n in (6, 7):
+        pairs = list(itertools.combinations(range(n), 2))
+        for _ in range(3000):
+            graphs.append(ca.Graph.from_edges(n, [p for p in pairs if rng.random() < 0.3]))
+    check_labels(union(graphs), seed=1002, seq_limit=0, stream=ca.STREAM_HASHED)
+
+
+@pytest.mark.parametrize("stream", [ca.STREAM_REFERENCE, ca.STREAM_HASHED])
+@pytest.mark.parametrize("case", ["gnm", "grid2", "grid3", "rmat", "rgg", "path", "star"])
+def test_level_trace_matches_oracle(case, stream):
+    rng = np.random.default_rng(abs(hash(case)) % 1000)
+    if case == "gnm":
+        g = ca.generate("gnm_undirected", n=1 << 14, m=1 << 15, seed=3)
+    elif case == "grid2":
+        g = ca.generate("grid_2d", nx=128, ny=100, p=0.55, seed=2)
+    elif case == "grid3":
+        g = ca.generate("grid_3d", nx=24, ny=24, nz=24, p=0.3, seed=2)
+    elif case == "rmat":
+        g = ca.generate("rmat", n=1 << 13, m=1 << 16, seed=3)
+    elif case == "rgg":
+        g = ca.generate("rgg_2d", n=1 << 13, r=0.02, seed=2)
+    elif case == "path":
+        g = ca.Graph.from_edges(5000, [(i, i + 1) for i in range(4999)])
+    else:
+        g = ca.Graph.from_edges(20001, [(0, i) for i in range(1, 20001)] + [(i, i + 1) for i in range(1, 1000)])
+    seq = 64
+    run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1005, seq_limit=seq, keep_keys=True, stream=stream)
+    dg = ca.DeviceGraph(g)
+    lab, st = dg.find_components(variant="exp", seed=1005, seq_limit=seq, record_trace=True, stream=stream)
+    tr = dg.trace()
+    dg.close()
+    want_levels = run.levels()
+    assert len(tr) == len(want_levels)
+    for lvl, ((info, keys, gid), winfo) in enumerate(zip(tr, want_levels)):
+        assert info[0] == winfo[0] and info[1] == winfo[1], f"level {lvl} sizes"
+        assert info[3] == winfo[3], f"level {lvl} max radius"
+        wkeys, wgid = run.level_keys(lvl)
+        assert np.array_equal(gid, wgid), f"level {lvl} gid"
+        assert np.array_equal(keys, wkeys), f"level {lvl} keys"
+    assert np.array_equal(lab, run.labels())
+    assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets))
+    assert st["radius_sum"] == sum(w[3] for w in want_levels)
+
+
+def test_local_contraction_trace_matches_oracle():
+    g = ca.generate("gnm_undirected", n=1 << 13, m=1 << 14, seed=9)
+    run = oracle.Run(g.offsets, g.targets, variant="local", seed=1001, seq_limit=64, keep_keys=True)
+    dg = ca.DeviceGraph(g)
+    lab, _ = dg.find_components(variant="local", seed=1001, seq_limit=64, record_trace=True)
+    tr = dg.trace()
+    dg.close()
+    assert len(tr) == len(run.levels())
+    for lvl, (info, keys, gid) in enumerate(tr):
+        wkeys, wgid = run.level_keys(lvl)
+        assert np.array_equal(keys, wkeys) and np.array_equal(gid, wgid)
+    assert np.array_equal(lab, run.labels())
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "isolated", "pairs", "clique", "star", "path", "twohubs"])
+def test_edge_cases(case):
+    if case == "empty":
+        g = ca.Graph(np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+    elif case == "one":
+        g = ca.Graph.from_edges(1, [])
+    elif case == "isolated":
+        g = ca.Graph.from_edges(1000, [])
+    elif case == "pairs":
+        g = ca.Graph.from_edges(2000, [(2 * i, 2 * i + 1) for i in range(1000)])
+    elif case == "clique":
+        g = ca.Graph.from_edges(300, list(itertools.combinations(range(300), 2)))
+    elif case == "star":
+        g = ca.Graph.from_edges(200001, [(200000, i) for i in range(200000)])
+    elif case == "path":
+        g = ca.Graph.from_edges(100000, [(i, i + 1) for i in range(99999)])
+    else:
+        e = [(0, i) for i in range(2, 60000)] + [(1, i) for i in range(30000, 90000)]
+        g = ca.Graph.from_edges(90000, e)
+    for stream in (ca.STREAM_REFERENCE, ca.STREAM_HASHED):
+        check_labels(g, seed=1001, stream=stream)
+    check_labels(g, variants=["exp"], seed=1001, seq_limit=0, stream=ca.STREAM_HASHED)
+
+
+@pytest.mark.parametrize("wmax,smax,tmax", [(256, 64, 4), (4096, 256, 16), (1 << 16, 1024, 32)])
+def test_bin_thresholds_do_not_change_results(wmax, smax, tmax):
+    g = ca.generate("rmat", n=1 << 15, m=1 << 19, seed=4)
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    lab, _ = ca.find_components(g, variant="exp", seed=1001, tmax=tmax, smax=smax, wmax=wmax)
+    assert np.array_equal(lab, ref)
+
+
+def test_random_graphs_many_seeds():
+    rng = np.random.default_rng(17)
+    for trial in range(8):
+        n = int(rng.integers(10, 50000))
+        g = rand_graph(rng, n, int(rng.integers(0, 3 * n)))
+        check_labels(g, seed=1000 + trial, stream=trial % 2)
+
+
+def test_config0_full_size():
+    g = ca.generate("gnm_undirected", n=1 << 20, m=1 << 22, seed=1)
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    dg = ca.DeviceGraph(g)
+    for stream in (ca.STREAM_REFERENCE, ca.STREAM_HASHED):
+        lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=16384, stream=stream)
+        assert np.array_equal(lab, ref)
+    dg.close()
+
+
+def test_rmat_scale20_invariants():
+    g = ca.generate("rmat", n=1 << 20, m=1 << 24, seed=3)
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    lab, st = ca.find_components(g, variant="exp", seed=1001, seq_limit=(g.m // 256), stream=ca.STREAM_HASHED)
+    assert np.array_equal(lab, ref)
+
+
+def test_app_check_flag():
+    exe = os.path.join(ca.BIN_DIR, "exp_contraction")
+    r = subprocess.run([exe, "-gen", "gnm_undirected", "-n", "14", "-m", "16", "-i", "2", "-check"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "CHECK labels_bit_exact=yes" in r.stdout
+    assert "GLOBAL RESULT runner=exp" in r.stdout
+    for app in ("local_contraction", "shortcuts", "labels"):
+        r = subprocess.run([os.path.join(ca.BIN_DIR, app), "-input", "file", "-in",
+                            os.path.join(GOLDEN, "simple.graph"), "-i", "1", "-check"],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and "CHECK labels_bit_exact=yes" in r.stdout, app

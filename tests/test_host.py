@@ -1,0 +1,166 @@
+"""libkacomp host side without a GPU: exported symbols, METIS I/O, generators,
+error paths."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import components_amd as ca
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "kacomp.h")).read()
+    return sorted(set(re.findall(r"\b(kc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ca.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+    nm = subprocess.run(["nm", "-D", "--defined-only", ca.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    for s in syms:
+        assert re.search(rf"\bT {s}$", nm, re.M), s
+    assert set(syms) == set(ca._SIGNATURES), "ctypes signatures and header differ"
+
+
+def test_read_metis_simple():
+    g = ca.read_metis(os.path.join(GOLDEN, "simple.graph"))
+    assert g.n == 6
+    assert list(g.offsets) == [0, 1, 2, 4, 6, 7, 8]
+    assert list(g.targets) == [2, 2, 0, 1, 4, 5, 3, 3]
+
+
+def test_metis_roundtrip(tmp_path):
+    g = ca.generate("gnm_undirected", n=500, m=900, seed=4)
+    p = tmp_path / "g.graph"
+    ca.write_metis(p, g)
+    h = ca.read_metis(p)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.targets, h.targets)
+
+
+def test_metis_semantics(tmp_path):
+    p = tmp_path / "c.graph"
+    # comment lines, a trailing 0 stops a line, an empty line is an isolated vertex,
+    # missing lines are isolated, a one-sided listing is symmetrised
+    p.write_text("% comment\n5 3 0\n2 0 4\n% inner comment\n1\n\n")
+    g = ca.read_metis(p)
+    assert g.n == 5
+    assert list(g.offsets) == [0, 1, 2, 2, 2, 2]
+    assert list(g.targets) == [1, 0]
+
+
+def test_metis_errors(tmp_path):
+    with pytest.raises(ca.KcError) as e:
+        ca.read_metis(tmp_path / "missing.graph")
+    assert e.value.code == ca.KC_ERR_IO
+    p = tmp_path / "bad.graph"
+    p.write_text("3 1\n4\n\n\n")
+    with pytest.raises(ca.KcError) as e:
+        ca.read_metis(p)
+    assert e.value.code == ca.KC_ERR_INVALID
+
+
+def check_csr(g):
+    n = g.n
+    off = g.offsets.astype(np.int64)
+    assert off[0] == 0 and np.all(np.diff(off) >= 0)
+    src = np.repeat(np.arange(n), np.diff(off))
+    tgt = g.targets.astype(np.int64)
+    assert np.all(tgt < n)
+    assert not np.any(src == tgt)
+    fwd = np.sort(src * n + tgt)
+    rev = np.sort(tgt * n + src)
+    assert np.array_equal(fwd, rev)            # symmetric
+    assert len(np.unique(fwd)) == len(fwd)     # no duplicates
+    for v in range(0, n, max(1, n // 50)):
+        row = g.targets[off[v]:off[v + 1]]
+        assert np.all(np.diff(row.astype(np.int64)) > 0)
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("gnm_undirected", dict(n=4096, m=16384, seed=1)),
+    ("rmat", dict(n=1 << 12, m=1 << 16, seed=3)),
+    ("grid_2d", dict(nx=64, ny=50, p=0.6, seed=2)),
+    ("grid_3d", dict(nx=16, ny=16, nz=16, p=0.3, seed=2)),
+    ("rgg_2d", dict(n=4096, r=0.03, seed=2)),
+    ("rgg_3d", dict(n=4096, r=0.1, seed=2)),
+    ("rhg", dict(n=4096, gamma=2.8, d=16.0, seed=5)),
+    ("ba", dict(n=3000, d=4.0, seed=6)),
+])
+def test_generators_valid_and_deterministic(name, kw):
+    g = ca.generate(name, **kw)
+    check_csr(g)
+    h = ca.generate(name, **kw)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.targets, h.targets)
+    kw2 = dict(kw, seed=kw["seed"] + 1)
+    k = ca.generate(name, **kw2)
+    assert not (np.array_equal(g.offsets, k.offsets) and np.array_equal(g.targets, k.targets))
+
+
+def test_generator_sizes():
+    g = ca.generate("grid_2d", nx=30, ny=20, p=1.0, seed=1)
+    assert g.n == 600 and g.m == 29 * 20 + 30 * 19
+    g = ca.generate("grid_3d", nx=8, ny=8, nz=8, p=1.0, seed=1)
+    assert g.n == 512 and g.m == 3 * 8 * 8 * 7
+    g = ca.generate("grid_2d", nx=10, ny=10, p=1.0, periodic=1, seed=1)
+    assert g.m == 200
+    g = ca.generate("gnm_undirected", n=1 << 16, m=1 << 18, seed=1)
+    assert (1 << 18) - 200 <= g.m <= (1 << 18)
+    g = ca.generate("rgg_2d", n=1 << 14, r=np.sqrt(8 / (np.pi * (1 << 14))), seed=2)
+    assert 6.5 < 2 * g.m / g.n < 8.5  # average degree about 8 (boundary loss)
+    g = ca.generate("grid_3d", nx=32, ny=32, nz=32, p=0.3, seed=2)
+    assert abs(g.m - 0.3 * 3 * 32 * 32 * 31) < 600
+
+
+def test_rmat_skew():
+    g = ca.generate("rmat", n=1 << 14, m=1 << 18, seed=3)
+    deg = np.diff(g.offsets.astype(np.int64))
+    assert deg.max() > 50 * max(1.0, deg.mean())
+    assert deg[0] == deg.max() or deg[:64].max() == deg.max()  # hubs at low ids (quadrant a)
+
+
+def test_generator_errors():
+    with pytest.raises(ca.KcError) as e:
+        ca.generate("rdg_2d", n=100)
+    assert e.value.code == ca.KC_ERR_UNSUPPORTED
+    with pytest.raises(ca.KcError) as e:
+        ca.generate("nonsense", n=100)
+    assert e.value.code == ca.KC_ERR_UNSUPPORTED
+    with pytest.raises(ca.KcError):
+        ca.generate("rmat", n=1000, m=100)
+
+
+def test_options_defaults_match_reference():
+    o = ca.options()
+    assert o.seed == 1 and abs(o.beta - 0.1) < 1e-15 and o.variant == 0 and o.deviate_fma == 1
+
+
+def test_no_device_is_a_clean_error():
+    # without a GPU the device entry points fail with KC_ERR_HIP rather than crashing
+    if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
+        g = ca.Graph.from_edges(3, [(0, 1)])
+        with pytest.raises(ca.KcError) as e:
+            ca.DeviceGraph(g)
+        assert e.value.code == ca.KC_ERR_HIP
+    else:
+        pytest.skip("a GPU is present")
+
+
+def test_cli_flag_parsing_without_device():
+    # the program prints the reference's CONFIG block before touching the device
+    exe = os.path.join(ca.BIN_DIR, "exp_contraction")
+    r = subprocess.run([exe, "-gen", "gnm_undirected", "-n", "8", "-m", "9", "-i", "1", "-beta", "0.2", "-seed", "5"],
+                       capture_output=True, text=True, timeout=120)
+    assert "CONFIG" in r.stdout
+    assert re.search(r"beta\s+0.2", r.stdout) and re.search(r"seed\s+5", r.stdout)
+    assert re.search(r"\bn\s+256\b", r.stdout) and re.search(r"\bm\s+512\b", r.stdout)
+    if not os.path.exists("/dev/kfd"):
+        assert r.returncode != 0
