@@ -1,0 +1,161 @@
+"""Benchmark: exponential contraction connected components on one MI355X.
+
+Workload: R-MAT scale 24, edge factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05),
+generator seed 3, symmetrised, self-loops and duplicates removed, u32 ids,
+u64 offsets; generated and built in HBM before the timed region.  A step is
+one kc_find_components call (exponential contraction down to the base case,
+undo, canonical min-id labels) on the resident graph; step i draws its
+deviates with seed 1 + i + 1000 as the reference harness does
+(app/exp_contraction.cpp:75-77).  value = GTEPS = m / step time.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS + contraction rounds, learned/batched vs plain, 1 MI355X, labels bit-exact"
+MI355X_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--stream", choices=["hashed", "reference"], default="hashed")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the label check")
+    ap.add_argument("--cpu-scale", type=int, default=21, help="R-MAT scale of the bounded CPU sample")
+    ap.add_argument("--tmax", type=int, default=0)
+    ap.add_argument("--smax", type=int, default=0)
+    ap.add_argument("--wmax", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding is a design "
+              "in DESIGN.md, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import __graft_entry__ as ge
+
+    if not ge.is_built():
+        ge.build()
+    import components_amd as ca
+
+    stream = ca.STREAM_HASHED if a.stream == "hashed" else ca.STREAM_REFERENCE
+    gp = dict(n=1 << a.scale, m=a.edge_factor << a.scale, seed=3, rmat_a=0.57, rmat_b=0.19, rmat_c=0.19)
+    t0 = time.time()
+    dg = ca.DeviceGraph.generate("rmat", **gp)
+    gen_s = time.time() - t0
+    n, m = dg.n, dg.m
+    seq_limit = max(m // 256, 4096)  # ComputeSequentialLimit with one PE per compute unit
+    run_kw = dict(variant="exp", beta=0.1, seq_limit=seq_limit, stream=stream, tmax=a.tmax, smax=a.smax, wmax=a.wmax)
+    print(f"[bench] rmat scale {a.scale}: n={n} m={m} non_isolated={dg.non_isolated} built in {gen_s:.1f}s",
+          file=sys.stderr, flush=True)
+
+    for i in range(a.warmup):
+        dg.find_components(copy_labels=False, seed=1 + i + 1000, **run_kw)
+    # timed region: every launch bracketed by HIP events on the library's stream
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        _, st = dg.find_components(copy_labels=False, seed=1 + i + 1000, time_kernels=True, **run_kw)
+        stats.append(st)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / a.steps
+    dev_ms = float(np.mean([s["time_ms"] for s in stats]))
+    value = m / (ms_per_step / 1e3) / 1e9
+
+    # per-kernel-class totals over the timed steps
+    cls_ms = {c: sum(s["kernel_ms"][c] for s in stats) for c in ca.KERNEL_CLASSES}
+    cls_bytes = {c: sum(s["kernel_bytes"][c] for s in stats) for c in ca.KERNEL_CLASSES}
+    cls_launch = {c: sum(s["kernel_launches"][c] for s in stats) for c in ca.KERNEL_CLASSES}
+    dom = max((c for c in ca.KERNEL_CLASSES if c != "other"), key=lambda c: cls_ms[c])
+    achieved = cls_bytes[dom] / (cls_ms[dom] / 1e3) / 1e9 if cls_ms[dom] > 0 else 0.0
+    total_kernel_ms = sum(cls_ms.values())
+    all_bytes = sum(cls_bytes.values())
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_rmat{a.scale}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("kernel_class") == dom:
+            traffic = tr.get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64 (integer)",
+        "data": "synthetic (device-generated R-MAT, counter-based SplitMix64 streams)",
+        "config": {"workload": f"rmat-scale{a.scale}-ef{a.edge_factor}-abcd(0.57,0.19,0.19,0.05)-seed3",
+                   "n": n, "m": m, "non_isolated": dg.non_isolated, "form": "plain",
+                   "deviate_stream": a.stream, "beta": 0.1, "seq_limit": seq_limit, "parallelism": "1 GPU"},
+        "levels": stats[-1]["levels"],
+        "sweeps": stats[-1]["sweeps"],
+        "radius_sum": stats[-1]["radius_sum"],
+        "device_ms_per_step": round(dev_ms, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": MI355X_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / MI355X_HBM_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": int(cls_bytes[dom] / max(cls_launch[dom], 1)),
+                     "avg_launch_ms": round(cls_ms[dom] / max(cls_launch[dom], 1), 5)},
+        "whole_step": {"alg_bytes": int(all_bytes / a.steps), "kernel_ms": round(total_kernel_ms / a.steps, 4),
+                       "alg_gbs": round(all_bytes / (total_kernel_ms / 1e3) / 1e9, 2) if total_kernel_ms else 0.0},
+        "kernel_ms_per_step": {c: round(cls_ms[c] / a.steps, 4) for c in ca.KERNEL_CLASSES if cls_ms[c] > 0},
+        "cpu_baseline": None,
+    }
+
+    if not a.no_cpu:
+        out["cpu_baseline"], out["labels_bit_exact"] = cpu_leg(a, dg, run_kw, stream)
+    dg.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_leg(a, dg, run_kw, stream):
+    """The oracle as checker (union-find labels of the full workload against the
+    GPU labels) and as the CPU baseline (its exponential contraction timed on a
+    bounded sample of the same workload, R-MAT at --cpu-scale)."""
+    import numpy as np
+
+    import components_amd as ca
+    import oracle
+
+    labels, _ = dg.find_components(seed=1001, **run_kw)
+    g = dg.download()
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    exact = bool(np.array_equal(labels, ref))
+    del g, ref
+    sample = ca.generate("rmat", n=1 << a.cpu_scale, m=a.edge_factor << a.cpu_scale, seed=3, rmat_a=0.57,
+                         rmat_b=0.19, rmat_c=0.19)
+    seq = max(sample.m // 256, 4096)
+    t0 = time.perf_counter()
+    run = oracle.Run(sample.offsets, sample.targets, variant="exp", seed=1001, seq_limit=seq, stream=stream)
+    secs = time.perf_counter() - t0
+    ok = bool(np.array_equal(run.labels(), oracle.canonical_labels(sample.offsets, sample.targets)))
+    cpu = {"value": round(sample.m / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+           "sample": f"R-MAT scale {a.cpu_scale}, edge factor {a.edge_factor}, same (a,b,c), seed 3 "
+                     f"(n={sample.n}, m={sample.m}); oracle exponential contraction, 1 thread, "
+                     f"{secs:.2f} s, labels canonical={ok}"}
+    return cpu, exact
+
+
+if __name__ == "__main__":
+    main()

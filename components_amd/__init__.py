@@ -89,6 +89,8 @@ _SIGNATURES = {
     "kc_gen_params_init": (None, [_P(GenParams)]),
     "kc_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(Csr)]),
     "kc_csr_free": (None, [_P(Csr)]),
+    "kc_graph_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(ctypes.c_void_p)]),
+    "kc_graph_download": (ctypes.c_int, [ctypes.c_void_p, _P(Csr)]),
 }
 
 _lib = None
@@ -226,15 +228,33 @@ def stats_dict(st):
 class DeviceGraph:
     """kc_graph handle: the CSR copied once to HBM."""
 
-    def __init__(self, g: Graph):
-        self._off = np.ascontiguousarray(g.offsets, np.uint64)
-        self._tgt = np.ascontiguousarray(g.targets, np.uint32)
-        if len(self._tgt) == 0:
-            self._tgt = np.zeros(1, np.uint32)
-        h = CsrHost(g.n, _u64p(self._off), _u32p(self._tgt))
+    def __init__(self, g: Graph = None, _handle=None):
         self.h = ctypes.c_void_p()
-        _check(lib().kc_graph_create(ctypes.byref(h), ctypes.byref(self.h)))
-        self.n = g.n
+        if _handle is not None:
+            self.h = _handle
+        else:
+            off = np.ascontiguousarray(g.offsets, np.uint64)
+            tgt = np.ascontiguousarray(g.targets, np.uint32)
+            if len(tgt) == 0:
+                tgt = np.zeros(1, np.uint32)
+            h = CsrHost(g.n, _u64p(off), _u32p(tgt))
+            _check(lib().kc_graph_create(ctypes.byref(h), ctypes.byref(self.h)))
+        n, m, ni = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().kc_graph_info(self.h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(ni)))
+        self.n, self.m, self.non_isolated = n.value, m.value, ni.value
+
+    @staticmethod
+    def generate(name, **kw):
+        """kc_graph_generate: the graph is generated and built in HBM."""
+        h = ctypes.c_void_p()
+        _check(lib().kc_graph_generate(name.encode(), ctypes.byref(gen_params(**kw)), ctypes.byref(h)))
+        return DeviceGraph(_handle=h)
+
+    def download(self):
+        """Level-0 CSR copied back to the host (rows unsorted for device-built graphs)."""
+        c = Csr()
+        _check(lib().kc_graph_download(self.h, ctypes.byref(c)))
+        return _take_csr(c)
 
     def close(self):
         if self.h:

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -641,6 +642,129 @@ void kc_options_init(kc_options *o) {
   o->max_levels = 4096;
 }
 
+}  // extern "C"
+
+namespace {
+
+// A handle with its stream and events; the CSR and work buffers come later.
+std::unique_ptr<kc_graph> new_handle() {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(KC_ERR_HIP, "no HIP device available");
+  std::unique_ptr<kc_graph> g(new kc_graph());
+  KC_HIP(hipGetDevice(&g->device));
+  KC_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  KC_HIP(hipEventCreate(&g->ev_begin));
+  KC_HIP(hipEventCreate(&g->ev_end));
+  KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_sizes), 16 * sizeof(uint64_t), hipHostMallocDefault));
+  KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_small), 16 * sizeof(uint32_t), hipHostMallocDefault));
+  KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_flags), 64 * sizeof(int), hipHostMallocDefault));
+  g->small = g->dalloc<uint32_t>(16);
+  g->sizes = g->dalloc<uint64_t>(16);
+  return g;
+}
+
+// Work buffers sized for level 0 (g->n, g->m2 set, off0/col0 in place).
+void init_work(kc_graph *g) {
+  const uint64_t n = g->n, m2 = g->m2;
+  g->key = g->dalloc<uint64_t>(n);
+  g->key2 = g->dalloc<uint64_t>(n);
+  g->deg_inter = g->dalloc<uint32_t>(n);
+  g->lists = g->dalloc<uint32_t>(3 * n);
+  g->hubs_cap = m2 / (kMinWaveBin / 2) + 2;
+  g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
+  g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
+  g->cnt = g->dalloc<uint32_t>(n);
+  g->fill = g->dalloc<uint32_t>(n);
+  g->bucket = g->dalloc<uint32_t>(m2);
+  g->old_of = g->dalloc<uint32_t>(n);
+  g->uniq = g->dalloc<uint32_t>(n);
+  g->bucket_off = g->dalloc<uint64_t>(n + 1);
+  g->rep_a = g->dalloc<uint32_t>(n);
+  g->rep_b = g->dalloc<uint32_t>(n);
+  g->minid = g->dalloc<uint32_t>(n);
+  g->labels = g->dalloc<uint32_t>(n);
+  g->scan_scratch = g->dalloc<uint64_t>(scan_scratch_words(n + m2) + 1024);
+  for (int i = 0; i < 2; ++i) {
+    g->off_pp[i] = g->dalloc<uint64_t>(n + 1);
+    g->col_pp[i] = g->dalloc<uint32_t>(m2);
+  }
+  g->dl_medium = g->dalloc<uint32_t>(n);
+  g->dl_huge = g->dalloc<uint32_t>(n);
+  g->dl_huge_tab = g->dalloc<uint64_t>(n);
+  g->hcount = g->dalloc<uint32_t>(n);
+  g->flags = g->dalloc<int>(kMaxFlags);
+  KC_HIP(hipDeviceSynchronize());
+}
+
+// Temporary device buffer freed at scope exit.
+struct DevTemp {
+  void *p = nullptr;
+  explicit DevTemp(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 64) != hipSuccess) fail(KC_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  ~DevTemp() { (void)hipFree(p); }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+// Symmetric, self-loop-free, duplicate-free CSR on the device from an edge
+// list (rows deduplicated in LDS hash tables, so the order inside a row is
+// not fixed).  Sets g->off0/col0/m2.
+void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  hipStream_t s = g->stream;
+  DevTemp deg((n + 1) * 4), off_tmp((n + 2) * 8), fill((n + 1) * 4), uniq((n + 1) * 4);
+  DevTemp scratch((scan_scratch_words(n + 2 * count) + 1024) * 8);
+  fill_u32(deg.as<uint32_t>(), n, 0, s);
+  deg_count(count, src, dst, deg.as<uint32_t>(), s);
+  csr_offsets(n, deg.as<uint32_t>(), off_tmp.as<uint64_t>(), g->sizes, scratch.as<uint64_t>(), s);
+  KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  const uint64_t P = g->h_sizes[2];
+  if (P >= (1ull << 32)) fail(KC_ERR_INVALID, "2m=%llu exceeds 2^32 directed edges", (unsigned long long)P);
+  DevTemp bucket(P * 4 + 64);
+  fill_u32(fill.as<uint32_t>(), n, 0, s);
+  edge_scatter(count, src, dst, off_tmp.as<uint64_t>(), fill.as<uint32_t>(), bucket.as<uint32_t>(), s);
+  DevTemp lists((3 * n + 8) * 8);
+  uint32_t *medium = lists.as<uint32_t>();
+  uint32_t *huge = medium + n + 2;
+  uint64_t *huge_tab = reinterpret_cast<uint64_t *>(huge + n + 2) + 1;
+  fill_u64(g->sizes + 3, 3, 0, s);
+  DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
+  dedup_small(n, nullptr, deg.as<uint32_t>(), off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), dl, s);
+  KC_HIP(hipMemcpyAsync(g->h_sizes + 3, g->sizes + 3, 3 * 8, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
+  dedup_medium(static_cast<uint32_t>(n_medium), medium, nullptr, deg.as<uint32_t>(), off_tmp.as<uint64_t>(),
+               bucket.as<uint32_t>(), uniq.as<uint32_t>(), s);
+  if (n_huge) {
+    DevTemp tables(huge_slots * 4 + 64), hcount(n_huge * 4 + 64);
+    fill_u32(tables.as<uint32_t>(), huge_slots, kNone, s);
+    fill_u32(hcount.as<uint32_t>(), n_huge, 0, s);
+    dedup_huge(static_cast<uint32_t>(n_huge), huge, huge_tab, tables.as<uint32_t>(), nullptr, deg.as<uint32_t>(),
+               off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), hcount.as<uint32_t>(), s);
+    KC_HIP(hipStreamSynchronize(s));
+  }
+  g->off0 = g->dalloc<uint64_t>(n + 1);
+  csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
+  KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  g->m2 = g->h_sizes[2];
+  g->col0 = g->dalloc<uint32_t>(g->m2);
+  compact(P, bucket.as<uint32_t>(), g->col0, scratch.as<uint64_t>(), s);
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(g->sizes + 6);
+  fill_u64(g->sizes + 6, 1, 0, s);
+  nonzero_rows(n, g->off0, cnt, s);
+  KC_HIP(hipMemcpyAsync(g->h_sizes + 6, g->sizes + 6, 8, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  g->non_isolated = g->h_sizes[6];
+}
+
+}  // namespace
+
+extern "C" {
+
 int kc_graph_create(const kc_csr_host *csr, kc_graph **out) {
   return guarded([&] {
     if (!csr || !out) fail(KC_ERR_INVALID, "null argument");
@@ -658,13 +782,7 @@ int kc_graph_create(const kc_csr_host *csr, kc_graph **out) {
     }
     for (uint64_t e = 0; e < m2; ++e)
       if (csr->targets[e] >= n) fail(KC_ERR_INVALID, "target %u out of range at edge %llu", csr->targets[e], (unsigned long long)e);
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(KC_ERR_HIP, "no HIP device available");
-    std::unique_ptr<kc_graph> g(new kc_graph());
-    KC_HIP(hipGetDevice(&g->device));
-    KC_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-    KC_HIP(hipEventCreate(&g->ev_begin));
-    KC_HIP(hipEventCreate(&g->ev_end));
+    std::unique_ptr<kc_graph> g = new_handle();
     g->n = n;
     g->m2 = m2;
     g->non_isolated = non_isolated;
@@ -672,40 +790,78 @@ int kc_graph_create(const kc_csr_host *csr, kc_graph **out) {
     g->col0 = g->dalloc<uint32_t>(m2);
     KC_HIP(hipMemcpy(g->off0, csr->offsets, (n + 1) * 8, hipMemcpyHostToDevice));
     if (m2) KC_HIP(hipMemcpy(g->col0, csr->targets, m2 * 4, hipMemcpyHostToDevice));
-    g->key = g->dalloc<uint64_t>(n);
-    g->key2 = g->dalloc<uint64_t>(n);
-    g->deg_inter = g->dalloc<uint32_t>(n);
-    g->lists = g->dalloc<uint32_t>(3 * n);
-    g->hubs_cap = m2 / (kMinWaveBin / 2) + 2;
-    g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
-    g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
-    g->cnt = g->dalloc<uint32_t>(n);
-    g->fill = g->dalloc<uint32_t>(n);
-    g->bucket = g->dalloc<uint32_t>(m2);
-    g->old_of = g->dalloc<uint32_t>(n);
-    g->uniq = g->dalloc<uint32_t>(n);
-    g->bucket_off = g->dalloc<uint64_t>(n + 1);
-    g->rep_a = g->dalloc<uint32_t>(n);
-    g->rep_b = g->dalloc<uint32_t>(n);
-    g->minid = g->dalloc<uint32_t>(n);
-    g->labels = g->dalloc<uint32_t>(n);
-    g->scan_scratch = g->dalloc<uint64_t>(scan_scratch_words(n + m2) + 1024);
-    for (int i = 0; i < 2; ++i) {
-      g->off_pp[i] = g->dalloc<uint64_t>(n + 1);
-      g->col_pp[i] = g->dalloc<uint32_t>(m2);
-    }
-    g->dl_medium = g->dalloc<uint32_t>(n);
-    g->dl_huge = g->dalloc<uint32_t>(n);
-    g->dl_huge_tab = g->dalloc<uint64_t>(n);
-    g->hcount = g->dalloc<uint32_t>(n);
-    g->flags = g->dalloc<int>(kMaxFlags);
-    g->small = g->dalloc<uint32_t>(16);
-    g->sizes = g->dalloc<uint64_t>(16);
-    KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_sizes), 16 * sizeof(uint64_t), hipHostMallocDefault));
-    KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_small), 16 * sizeof(uint32_t), hipHostMallocDefault));
-    KC_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->h_flags), 64 * sizeof(int), hipHostMallocDefault));
-    KC_HIP(hipDeviceSynchronize());
+    init_work(g.get());
     *out = g.release();
+  });
+}
+
+int kc_graph_generate(const char *name, const kc_gen_params *pp, kc_graph **out) {
+  return guarded([&] {
+    if (!name || !pp || !out) fail(KC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    kc_gen_params p = *pp;
+    const std::string gname(name);
+    uint64_t n = 0, slots = 0;
+    int dims = 0;
+    uint32_t scale = 0;
+    if (gname == "gnm_undirected") {
+      n = p.n;
+      slots = p.m;
+    } else if (gname == "rmat") {
+      while ((1ull << scale) < p.n) ++scale;
+      if ((1ull << scale) != p.n || scale > 31) fail(KC_ERR_INVALID, "rmat: n=%llu must be a power of two below 2^32", (unsigned long long)p.n);
+      n = p.n;
+      slots = p.m;
+    } else if (gname == "grid_2d" || gname == "grid_3d") {
+      dims = gname == "grid_2d" ? 2 : 3;
+      if (!p.nx) p.nx = p.n;
+      if (!p.ny) p.ny = dims == 2 ? p.m : p.nx;
+      if (dims == 3 && !p.nz) p.nz = p.nx;
+      if (dims == 2) p.nz = 1;
+      n = p.nx * p.ny * p.nz;
+      slots = n * dims;
+    } else if (gname == "ba") {
+      n = p.n;
+      slots = p.n * (p.d >= 1 ? static_cast<uint64_t>(p.d) : 1);
+    } else {
+      fail(KC_ERR_UNSUPPORTED, "device generator not available for %s (use kc_generate)", name);
+    }
+    if (n >= 0xFFFFFFFFull) fail(KC_ERR_INVALID, "%s: n=%llu exceeds 32-bit vertex ids", name, (unsigned long long)n);
+    std::unique_ptr<kc_graph> g = new_handle();
+    hipStream_t s = g->stream;
+    {
+      DevTemp src(slots * 4 + 64), dst(slots * 4 + 64);
+      uint64_t count = slots;
+      if (gname == "gnm_undirected") {
+        gen_gnm(slots, n, p.seed, src.as<uint32_t>(), dst.as<uint32_t>(), s);
+      } else if (gname == "rmat") {
+        gen_rmat(slots, scale, p.rmat_a, p.rmat_b, p.rmat_c, p.seed, src.as<uint32_t>(), dst.as<uint32_t>(), s);
+      } else if (gname == "ba") {
+        gen_ba(slots, p.d >= 1 ? static_cast<uint64_t>(p.d) : 1, p.seed, src.as<uint32_t>(), dst.as<uint32_t>(), s);
+      } else {
+        if (p.nx >= (1ull << 32) || p.ny >= (1ull << 32) || p.nz >= (1ull << 32))
+          fail(KC_ERR_INVALID, "%s: extents must be below 2^32", name);
+        gen_grid(p.nx, p.ny, p.nz, dims, p.periodic, p.p, p.seed, src.as<uint32_t>(), dst.as<uint32_t>(), s);
+      }
+      g->n = n;
+      build_device_csr(g.get(), n, src.as<uint32_t>(), dst.as<uint32_t>(), count);
+    }
+    init_work(g.get());
+    *out = g.release();
+  });
+}
+
+int kc_graph_download(const kc_graph *g, kc_csr *out) {
+  return guarded([&] {
+    if (!g || !out) fail(KC_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->n = g->n;
+    out->m = g->m2 / 2;
+    out->offsets = static_cast<uint64_t *>(std::malloc((g->n + 1) * 8));
+    out->targets = static_cast<uint32_t *>(std::malloc(g->m2 ? g->m2 * 4 : 4));
+    if (!out->offsets || !out->targets) fail(KC_ERR_OOM, "host allocation failed");
+    KC_HIP(hipMemcpy(out->offsets, g->off0, (g->n + 1) * 8, hipMemcpyDeviceToHost));
+    if (g->m2) KC_HIP(hipMemcpy(out->targets, g->col0, g->m2 * 4, hipMemcpyDeviceToHost));
   });
 }
 

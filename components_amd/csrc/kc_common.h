@@ -123,4 +123,46 @@ __device__ __forceinline__ uint32_t group_sum_u32(uint32_t v, int width) {
   return v;
 }
 
+// Atomic fetch-add on arr[key] for every lane with part set.  The lanes that
+// share the key of the first participating lane are merged into one atomic
+// (hub clusters and giant components make many lanes of a wave hit the same
+// counter); the remaining lanes issue their own.  Returns the old value each
+// lane would have seen with one atomic per lane in lane order.
+__device__ __forceinline__ uint32_t peeled_fetch_add(uint32_t *arr, uint32_t key, uint32_t val, bool part) {
+  const uint64_t pm = __ballot(part);
+  if (pm == 0) return 0;
+  const int leader = __ffsll(static_cast<unsigned long long>(pm)) - 1;
+  const uint32_t lk = __shfl(key, leader);
+  const bool same = part && key == lk;
+  const uint32_t x = same ? val : 0u;
+  uint32_t incl = x;
+  const int ln = static_cast<int>(lane_id());
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (ln >= o) incl += t;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  uint32_t base = 0;
+  if (ln == leader) base = atomicAdd(&arr[lk], total);
+  base = __shfl(base, leader);
+  if (same) return base + incl - x;
+  if (part) return atomicAdd(&arr[key], val);
+  return 0;
+}
+
+// atomicMin(&arr[key], val) for every lane with part set, merged for the
+// lanes sharing the first lane's key and skipped when arr[key] is already
+// smaller (values only decrease, so a stale read only costs an atomic).
+__device__ __forceinline__ void peeled_min(uint32_t *arr, uint32_t key, uint32_t val, bool part) {
+  const uint64_t pm = __ballot(part);
+  if (pm == 0) return;
+  const int leader = __ffsll(static_cast<unsigned long long>(pm)) - 1;
+  const uint32_t lk = __shfl(key, leader);
+  const bool same = part && key == lk;
+  uint32_t mn = same ? val : 0xFFFFFFFFu;
+  for (int o = 32; o > 0; o >>= 1) mn = min(mn, static_cast<uint32_t>(__shfl_xor(mn, o)));
+  if (static_cast<int>(lane_id()) == leader && mn < arr[lk]) atomicMin(&arr[lk], mn);
+  if (part && !same && val < arr[key]) atomicMin(&arr[key], val);
+}
+
 }  // namespace kc

@@ -93,7 +93,7 @@ void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, cons
              const uint64_t *bucket_off, uint32_t *fill, uint32_t *bucket, hipStream_t s);
 
 // Dedup of every kept bucket; lists[0..2] count medium/huge buckets and huge
-// table slots (zeroed by the caller).
+// table slots (zeroed by the caller).  old_of == nullptr: bucket a2 is row a2.
 struct DedupLists {
   uint32_t *medium;    // a' of buckets with wave_cap < size <= block_cap
   uint32_t *huge;      // a' of buckets above block_cap
@@ -125,5 +125,20 @@ void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s);
 void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,
                  uint32_t *roots, hipStream_t s);
 void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, hipStream_t s);
+
+// Device generators (gen_kernels.hip): the edge streams of the host generators.
+void gen_gnm(uint64_t count, uint64_t n, int32_t seed, uint32_t *src, uint32_t *dst, hipStream_t s);
+void gen_rmat(uint64_t count, uint32_t scale, double a, double b, double c, int32_t seed, uint32_t *src, uint32_t *dst,
+              hipStream_t s);
+void gen_ba(uint64_t count, uint64_t d, int32_t seed, uint32_t *src, uint32_t *dst, hipStream_t s);
+// Lattices write n*dims slots: kept bonds, or self-loops for dropped ones.
+void gen_grid(uint64_t nx, uint64_t ny, uint64_t nz, int dims, int periodic, double p, int32_t seed, uint32_t *src,
+              uint32_t *dst, hipStream_t s);
+// CSR construction from an edge list: degrees (self-loops skipped), scatter of
+// both directions into row buckets.
+void deg_count(uint64_t count, const uint32_t *src, const uint32_t *dst, uint32_t *deg, hipStream_t s);
+void edge_scatter(uint64_t count, const uint32_t *src, const uint32_t *dst, const uint64_t *off, uint32_t *fill,
+                  uint32_t *bucket, hipStream_t s);
+void nonzero_rows(uint64_t n, const uint64_t *off, unsigned long long *counter, hipStream_t s);
 
 }  // namespace kc

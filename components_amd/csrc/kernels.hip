@@ -246,15 +246,15 @@ __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
 __global__ void k_cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab,
                                 uint32_t *cnt, uint32_t *radius, int with_radius) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  uint32_t r = 0;
+  uint32_t r = 0, l = 0, d = 0;
   if (v < n) {
     const uint64_t k = key[v];
-    const uint32_t l = static_cast<uint32_t>(k);
+    l = static_cast<uint32_t>(k);
     lab[v] = l;
-    const uint32_t d = deg_inter[v];
-    if (d) atomicAdd(&cnt[l], d);
+    d = deg_inter[v];
     if (with_radius) r = static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[l] >> 32);
   }
+  peeled_fetch_add(cnt, l, d, d != 0);
   if (with_radius) {
     for (int o = 32; o > 0; o >>= 1) r = max(r, static_cast<uint32_t>(__shfl_xor(r, o)));
     if (lane_id() == 0 && r) atomicMax(radius, r);
@@ -436,10 +436,9 @@ __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active,
   uint32_t lv = 0;
   uint64_t pos = 0;
   active = active && d > 0;
-  if (active) {
-    lv = static_cast<uint32_t>(a.key[v]);
-    if (gl == 0) pos = a.bucket_off[lv] + atomicAdd(&a.fill[lv], d);
-  }
+  if (active) lv = static_cast<uint32_t>(a.key[v]);
+  const uint32_t slot = peeled_fetch_add(a.fill, lv, d, active && gl == 0);
+  if (active && gl == 0) pos = a.bucket_off[lv] + slot;
   if (W > 1) pos = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), gbase, 64));
   if (W == 1) {
     if (!active) return;
@@ -505,12 +504,18 @@ __global__ __launch_bounds__(kBlock) void k_scatter(ScatterArgs a) {
 constexpr int kWaveTab = 2 * kDedupWaveCap;   // 1024 slots per wave
 constexpr int kBlockTab = 2 * kDedupBlockCap;  // 16384 slots per block (64 KB)
 
+// Open addressing with linear probing; a slot is read before the CAS so that
+// duplicates (frequent in hub buckets) cost a load instead of an atomic.
 __device__ __forceinline__ bool lds_insert(uint32_t *tab, uint32_t mask, uint32_t b) {
   uint32_t h = hash32(b) & mask;
   while (true) {
-    uint32_t prev = atomicCAS(&tab[h], kNone, b);
-    if (prev == kNone) return true;
-    if (prev == b) return false;
+    const uint32_t cur = tab[h];
+    if (cur == b) return false;
+    if (cur == kNone) {
+      const uint32_t prev = atomicCAS(&tab[h], kNone, b);
+      if (prev == kNone) return true;
+      if (prev == b) return false;
+    }
     h = (h + 1) & mask;
   }
 }
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
   uint32_t *tab = tabs[w];
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
   for (uint64_t a2 = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + w; a2 < n_next; a2 += waves) {
-    const uint32_t a = old_of[a2];
+    const uint32_t a = old_of ? old_of[a2] : static_cast<uint32_t>(a2);
     const uint32_t s = cnt[a];
     const uint64_t base = bucket_off[a];
     if (s > kDedupWaveCap) {
@@ -544,8 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
       }
       continue;
     }
-    if (s == 1) {
-      if (ln == 0) uniq[a2] = 1;
+    if (s <= 1) {
+      if (ln == 0) uniq[a2] = s;
       continue;
     }
     uint32_t tsz = 2;
@@ -580,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, cons
   const int tid = threadIdx.x;
   for (uint32_t i = blockIdx.x; i < n_medium; i += gridDim.x) {
     const uint32_t a2 = medium[i];
-    const uint32_t a = old_of[a2];
+    const uint32_t a = old_of ? old_of[a2] : static_cast<uint32_t>(a2);
     const uint32_t s = cnt[a];
     const uint64_t base = bucket_off[a];
     uint32_t tsz = 2;
@@ -612,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_insert(uint32_t n_huge, c
                                                              const uint32_t *old_of, const uint32_t *cnt,
                                                              const uint64_t *bucket_off, const uint32_t *bucket) {
   for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
-  const uint32_t a = old_of[huge[h]];
+  const uint32_t a = old_of ? old_of[huge[h]] : huge[h];
   const uint32_t s = cnt[a];
   const uint64_t base = bucket_off[a];
   uint64_t tsz = 1;
@@ -624,8 +629,13 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_insert(uint32_t n_huge, c
     const uint32_t b = bucket[base + j];
     uint32_t k = hash32(b) & mask;
     while (true) {
-      uint32_t prev = atomicCAS(&tab[k], kNone, b);
-      if (prev == kNone || prev == b) break;
+      // a slot never changes once filled, so a non-empty read is exact
+      const uint32_t cur = tab[k];
+      if (cur == b) break;
+      if (cur == kNone) {
+        const uint32_t prev = atomicCAS(&tab[k], kNone, b);
+        if (prev == kNone || prev == b) break;
+      }
       k = (k + 1) & mask;
     }
   }
@@ -638,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(uint32_t n_huge, con
                                                            const uint64_t *bucket_off, uint32_t *bucket,
                                                            uint32_t *hcount) {
   for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
-  const uint32_t a = old_of[huge[h]];
+  const uint32_t a = old_of ? old_of[huge[h]] : huge[h];
   const uint32_t s = cnt[a];
   const uint64_t base = bucket_off[a];
   uint64_t tsz = 1;
@@ -660,7 +670,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_tail(uint32_t n_huge, con
                                                            uint32_t *uniq, const uint32_t *hcount) {
   for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
   const uint32_t a2 = huge[h];
-  const uint32_t a = old_of[a2];
+  const uint32_t a = old_of ? old_of[a2] : static_cast<uint32_t>(a2);
   const uint32_t s = cnt[a];
   const uint64_t base = bucket_off[a];
   const uint32_t u = hcount[h];
@@ -737,8 +747,8 @@ __global__ void k_rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t 
 
 __global__ void k_canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (v >= n) return;
-  atomicMin(&minid[rep[v]], static_cast<uint32_t>(v));
+  const bool live = v < n;
+  peeled_min(minid, live ? rep[v] : 0u, static_cast<uint32_t>(v), live);
 }
 
 __global__ void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,

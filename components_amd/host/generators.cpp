@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "csr_build.h"
+#include "gen_rng.h"
 #include "kacomp.h"
 
 namespace kc {
@@ -35,22 +36,14 @@ void set_error(const char *fmt, ...);
 
 namespace {
 
-inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  return z ^ (z >> 31);
-}
-
-struct Stream {
-  uint64_t base;
-  Stream(int32_t seed, uint64_t stream) : base(mix64(mix64(static_cast<uint64_t>(static_cast<uint32_t>(seed)) ^ (stream << 40)) + stream)) {}
-  uint64_t operator()(uint64_t i) const { return mix64(base + (i + 1) * 0x9e3779b97f4a7c15ull); }
+using Stream = kc::GenStream;
+inline uint64_t bounded(uint64_t z, uint64_t n) { return kc::gen_bounded(z, n); }
+inline double unit(uint64_t z) { return kc::gen_unit(z); }
+struct StreamFn {
+  Stream s;
+  StreamFn(int32_t seed, uint64_t id) : s(kc::gen_stream(seed, id)) {}
+  uint64_t operator()(uint64_t i) const { return kc::gen_value(s, i); }
 };
-
-inline uint64_t bounded(uint64_t z, uint64_t n) {
-  return static_cast<uint64_t>((static_cast<unsigned __int128>(z) * n) >> 64);
-}
-inline double unit(uint64_t z) { return static_cast<double>(z >> 11) * 0x1p-53; }
 
 struct EdgeList {
   std::vector<uint32_t> src, dst;
@@ -78,57 +71,29 @@ void merge(std::vector<EdgeList> &parts, EdgeList &out) {
 
 void gen_gnm(const kc_gen_params &p, EdgeList &el) {
   const uint64_t n = p.n, m = p.m;
-  Stream rs(p.seed, 1);
+  const Stream rs = kc::gen_stream(p.seed, 1);
   el.src.resize(m);
   el.dst.resize(m);
 #pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < static_cast<int64_t>(m); ++i) {
-    el.src[i] = static_cast<uint32_t>(bounded(rs(2 * i), n));
-    el.dst[i] = static_cast<uint32_t>(bounded(rs(2 * i + 1), n));
-  }
+  for (int64_t i = 0; i < static_cast<int64_t>(m); ++i) kc::gnm_edge(rs, i, n, &el.src[i], &el.dst[i]);
 }
 
 void gen_rmat(const kc_gen_params &p, EdgeList &el, uint32_t scale) {
   const uint64_t m = p.m;
-  Stream rs(p.seed, 2);
-  const double a = p.rmat_a, b = p.rmat_b, c = p.rmat_c;
-  const uint64_t ta = static_cast<uint64_t>(a * 4294967296.0);
-  const uint64_t tab = static_cast<uint64_t>((a + b) * 4294967296.0);
-  const uint64_t tabc = static_cast<uint64_t>((a + b + c) * 4294967296.0);
-  const uint64_t words = (scale + 1) / 2;
+  const Stream rs = kc::gen_stream(p.seed, 2);
+  const kc::RmatThresholds t = kc::rmat_thresholds(p.rmat_a, p.rmat_b, p.rmat_c);
   el.src.resize(m);
   el.dst.resize(m);
 #pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < static_cast<int64_t>(m); ++i) {
-    uint32_t u = 0, v = 0;
-    for (uint32_t l = 0; l < scale; ++l) {
-      const uint64_t z = rs(static_cast<uint64_t>(i) * words + l / 2);
-      const uint64_t r = (l & 1) ? (z >> 32) : (z & 0xffffffffull);
-      uint32_t bu = 0, bv = 0;
-      if (r < ta) {
-      } else if (r < tab) {
-        bv = 1;
-      } else if (r < tabc) {
-        bu = 1;
-      } else {
-        bu = 1;
-        bv = 1;
-      }
-      u = (u << 1) | bu;
-      v = (v << 1) | bv;
-    }
-    el.src[i] = u;
-    el.dst[i] = v;
-  }
+  for (int64_t i = 0; i < static_cast<int64_t>(m); ++i) kc::rmat_edge(rs, i, scale, t, &el.src[i], &el.dst[i]);
 }
 
 void gen_grid(const kc_gen_params &p, EdgeList &el, int dims) {
   const uint64_t nx = p.nx, ny = p.ny, nz = dims == 3 ? p.nz : 1;
   const uint64_t n = nx * ny * nz;
-  Stream rs(p.seed, 3);
+  const Stream rs = kc::gen_stream(p.seed, 3);
   const double prob = p.p;
-  const uint64_t thresh = prob >= 1.0 ? ~0ull : static_cast<uint64_t>(prob * 18446744073709551616.0);
-  const bool keep_all = prob >= 1.0;
+  const uint64_t thresh = kc::grid_keep_threshold(prob);
   const int nt = omp_get_max_threads();
   std::vector<EdgeList> parts(nt);
 #pragma omp parallel
@@ -149,7 +114,7 @@ void gen_grid(const kc_gen_params &p, EdgeList &el, int dims) {
           continue;
         }
         const uint64_t bond = static_cast<uint64_t>(id) * dims + d;
-        if (!keep_all && rs(bond) >= thresh) continue;
+        if (!kc::grid_bond_kept(rs, bond, prob, thresh)) continue;
         mine.add(static_cast<uint32_t>(id), static_cast<uint32_t>((tz * ny + ty) * nx + tx));
       }
     }
@@ -160,7 +125,7 @@ void gen_grid(const kc_gen_params &p, EdgeList &el, int dims) {
 void gen_rgg(const kc_gen_params &p, EdgeList &el, int dims) {
   const uint64_t n = p.n;
   const double r = p.r;
-  Stream rs(p.seed, 4);
+  StreamFn rs(p.seed, 4);
   uint64_t cells = r > 0 ? static_cast<uint64_t>(std::floor(1.0 / r)) : 1;
   if (cells < 1) cells = 1;
   const uint64_t max_cells = dims == 2 ? (1ull << 26) : (1ull << 24);
@@ -249,7 +214,7 @@ void gen_rhg(const kc_gen_params &p, EdgeList &el) {
   // target radius from the average-degree approximation of Krioukov et al. (2010)
   const double xi = 2.0 * alpha / (2.0 * alpha - 1.0);
   const double R = 2.0 * std::log(2.0 * xi * xi * static_cast<double>(n) / (M_PI * kbar));
-  Stream rs(p.seed, 6);
+  StreamFn rs(p.seed, 6);
   struct Pt {
     double theta, r;
   };
@@ -333,20 +298,11 @@ void gen_rhg(const kc_gen_params &p, EdgeList &el) {
 void gen_ba(const kc_gen_params &p, EdgeList &el) {
   const uint64_t n = p.n;
   const uint64_t d = p.d >= 1 ? static_cast<uint64_t>(p.d) : 1;
-  Stream rs(p.seed, 5);
-  const uint64_t slots = 2 * n * d;
+  const Stream rs = kc::gen_stream(p.seed, 5);
   el.src.resize(n * d);
   el.dst.resize(n * d);
 #pragma omp parallel for schedule(static)
-  for (int64_t e = 0; e < static_cast<int64_t>(n * d); ++e) {
-    const uint64_t s = 2 * static_cast<uint64_t>(e);
-    // target slot uniform among earlier slots; odd slots resolve recursively
-    uint64_t t = bounded(rs(s + 1), s + 1);
-    while (t & 1) t = bounded(rs(t), t);
-    el.src[e] = static_cast<uint32_t>(s / 2 / d);
-    el.dst[e] = static_cast<uint32_t>(t / 2 / d);
-  }
-  (void)slots;
+  for (int64_t e = 0; e < static_cast<int64_t>(n * d); ++e) kc::ba_edge(rs, e, d, &el.src[e], &el.dst[e]);
 }
 
 }  // namespace

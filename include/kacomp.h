@@ -77,6 +77,23 @@ typedef struct kc_csr {
   uint32_t *targets; /* offsets[n] */
 } kc_csr;
 
+/* Generator parameters of kc_generate / kc_graph_generate (generator names:
+ * gnm_undirected, rgg_2d, rgg_3d, grid_2d, grid_3d, rmat, rhg, ba).  KaGen-style
+ * (app/parse_parameters.h:83-99): n, m (exact counts), r, p, gamma, d,
+ * periodic, plus rmat_a/b/c. */
+typedef struct kc_gen_params {
+  uint64_t n;
+  uint64_t m;
+  double r;
+  double p;
+  double gamma;
+  double d;
+  int32_t periodic;
+  int32_t seed;
+  double rmat_a, rmat_b, rmat_c;
+  uint64_t nx, ny, nz; /* grid dimensions (grid_2d: nx=n, ny=m) */
+} kc_gen_params;
+
 /* Options of one kc_find_components call.  kc_options_init fills the
  * reference's defaults (app/parse_parameters.h:30-100). */
 typedef struct kc_options {
@@ -148,6 +165,13 @@ void kc_options_init(kc_options *opt);
 int kc_graph_create(const kc_csr_host *csr, kc_graph **out);
 int kc_graph_destroy(kc_graph *g);
 int kc_graph_info(const kc_graph *g, uint64_t *n, uint64_t *m, uint64_t *non_isolated);
+/* Generates gnm_undirected, rmat, grid_2d, grid_3d or ba directly in HBM with
+ * the edge streams of kc_generate and builds the CSR on the device (same rows
+ * as kc_generate; the order inside a row is not fixed).  Input preparation
+ * only: IOUtility::LoadGraph's generator branch (lib/io/io_utils.h:46-51). */
+int kc_graph_generate(const char *name, const kc_gen_params *p, kc_graph **out);
+/* Copies the level-0 CSR of a graph to host memory (free with kc_csr_free). */
+int kc_graph_download(const kc_graph *g, kc_csr *out);
 
 /* Runs the selected variant.  labels_out (host, n entries) may be NULL: the
  * labels then stay on the device (kc_copy_labels fetches them). */
@@ -174,21 +198,6 @@ int kc_deviates(uint32_t gid0, uint32_t count, int32_t seed, uint32_t iteration,
 /* Host I/O and generators. */
 int kc_read_metis(const char *path, kc_csr *out);
 int kc_write_metis(const char *path, const kc_csr *g);
-/* name: gnm_undirected, rgg_2d, rgg_3d, grid_2d, grid_3d, rmat, rhg, ba.
- * params (KaGen-style, app/parse_parameters.h:83-99): n, m (exact counts),
- * r, p, gamma, d, periodic, plus rmat_a/b/c. */
-typedef struct kc_gen_params {
-  uint64_t n;
-  uint64_t m;
-  double r;
-  double p;
-  double gamma;
-  double d;
-  int32_t periodic;
-  int32_t seed;
-  double rmat_a, rmat_b, rmat_c;
-  uint64_t nx, ny, nz; /* grid dimensions (grid_2d: nx=n, ny=m) */
-} kc_gen_params;
 void kc_gen_params_init(kc_gen_params *p);
 int kc_generate(const char *name, const kc_gen_params *p, kc_csr *out);
 void kc_csr_free(kc_csr *g);

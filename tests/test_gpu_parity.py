@@ -214,3 +214,25 @@ def test_app_check_flag():
                             os.path.join(GOLDEN, "simple.graph"), "-i", "1", "-check"],
                            capture_output=True, text=True, timeout=300)
         assert r.returncode == 0 and "CHECK labels_bit_exact=yes" in r.stdout, app
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("gnm_undirected", dict(n=50000, m=200000, seed=1)),
+    ("rmat", dict(n=1 << 15, m=1 << 19, seed=3)),
+    ("grid_2d", dict(nx=300, ny=200, p=0.5, seed=2)),
+    ("grid_3d", dict(nx=20, ny=20, nz=20, p=0.3, periodic=1, seed=2)),
+    ("ba", dict(n=20000, d=4.0, seed=6)),
+])
+def test_device_generator_matches_host(name, kw):
+    host = ca.generate(name, **kw)
+    dg = ca.DeviceGraph.generate(name, **kw)
+    dev = dg.download()
+    assert np.array_equal(dev.offsets, host.offsets)
+    for v in range(dev.n):
+        a, b = int(dev.offsets[v]), int(dev.offsets[v + 1])
+        if b - a > 1:
+            dev.targets[a:b] = np.sort(dev.targets[a:b])
+    assert np.array_equal(dev.targets, host.targets)
+    lab, _ = dg.find_components(seed=1001)
+    assert np.array_equal(lab, oracle.canonical_labels(host.offsets, host.targets))
+    dg.close()
