@@ -99,11 +99,18 @@ __global__ void k_edge_scatter(uint64_t count, const uint32_t *src, const uint32
   }
 }
 
-__global__ void k_nonzero_rows(uint64_t n, const uint64_t *off, unsigned long long *counter) {
+// Non-empty rows per block (one atomic per block, on a 64-bit counter).
+__global__ __launch_bounds__(kBlock) void k_nonzero_rows(uint64_t n, const uint64_t *off, unsigned long long *counter) {
+  __shared__ uint32_t wc[kBlock / kWave];
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   const bool nz = v < n && off[v + 1] > off[v];
-  const uint64_t m = __ballot(nz);
-  if (m && __lane_id() == 0) atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
+  const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(nz)));
+  if (__lane_id() == 0) wc[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = wc[0] + wc[1] + wc[2] + wc[3];
+    if (t) atomicAdd(counter, static_cast<unsigned long long>(t));
+  }
 }
 
 }  // namespace

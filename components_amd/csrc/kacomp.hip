@@ -199,8 +199,9 @@ struct kc_graph {
   uint32_t *deg_inter = nullptr, *lists = nullptr, *hub_inter = nullptr;
   HubChunk *hubs = nullptr;
   uint64_t hubs_cap = 0;
-  uint32_t *cnt = nullptr, *fill = nullptr, *bucket = nullptr, *old_of = nullptr, *uniq = nullptr;
-  uint64_t *bucket_off = nullptr;
+  uint32_t *bucket = nullptr, *old_of = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
+  uint32_t *sk = nullptr, *sv = nullptr, *sk2 = nullptr, *sv2 = nullptr;  // (label, vertex) sort buffers
+  uint64_t *seg_off = nullptr, *sort_scratch = nullptr;
   uint32_t *rep_a = nullptr, *rep_b = nullptr, *minid = nullptr, *labels = nullptr;
   uint64_t *scan_scratch = nullptr;
   uint64_t *off_pp[2] = {nullptr, nullptr};
@@ -209,6 +210,10 @@ struct kc_graph {
   uint64_t *dl_huge_tab = nullptr;
   uint32_t *huge_tables = nullptr;
   uint64_t huge_cap = 0;
+  HugeChunk *d_chunks = nullptr;
+  uint64_t chunks_cap = 0;
+  uint64_t *d_huge_info = nullptr;
+  uint64_t huge_info_cap = 0;
   int *flags = nullptr;
   uint32_t *small = nullptr;  // [0..3] bin counters, [4] radius, [5] roots
   uint64_t *sizes = nullptr;  // [0] n_next [1] P [2] m2_next [3..5] dedup counters
@@ -311,10 +316,9 @@ Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg) 
   b.listS = g->lists + g->n;
   b.listW = g->lists + 2 * g->n;
   b.hubs = g->hubs;
-  fill_u32(g->small, 4, 0, g->stream);
   {
     TimedScope ts(g->timer, KC_K_BIN, 16 * n + 8);
-    bin_vertices(n, off, cfg, b.listT, b.listS, b.listW, b.hubs, g->small, g->deg_inter, g->stream);
+    bin_vertices(n, off, cfg, b.listT, b.listS, b.listW, b.hubs, g->small, g->deg_inter, g->scan_scratch, g->stream);
   }
   d2h(g, g->h_small, g->small, 4 * sizeof(uint32_t));
   sync(g);
@@ -372,6 +376,76 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
     if (i < batch) break;
   }
   return done;
+}
+
+// Dedup of buckets [seg_off[a2], seg_off[a2+1]) for a2 < nb: small buckets
+// in wave tables, medium ones in block tables, huge ones in global tables
+// processed in fixed-size chunks planned on the host.
+void dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
+                   uint32_t *medium, uint32_t *huge, uint64_t *huge_tab) {
+  hipStream_t s = g->stream;
+  fill_u64(g->sizes + 3, 3, 0, s);
+  DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
+  {
+    TimedScope ts(g->timer, KC_K_DEDUP, 0);
+    dedup_small(nb, seg_off, bucket, uniq, dl, s);
+  }
+  d2h(g, g->h_sizes + 3, g->sizes + 3, 3 * sizeof(uint64_t));
+  sync(g);
+  const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
+  if (n_medium) {
+    TimedScope ts(g->timer, KC_K_DEDUP, 0);
+    dedup_medium(static_cast<uint32_t>(n_medium), medium, seg_off, bucket, uniq, s);
+  }
+  if (!n_huge) return;
+  if (2 * n_huge > g->huge_info_cap) {
+    g->huge_info_cap = 2 * n_huge + 64;
+    g->d_huge_info = g->dalloc<uint64_t>(g->huge_info_cap);
+  }
+  huge_sizes(static_cast<uint32_t>(n_huge), huge, seg_off, g->d_huge_info, s);
+  std::vector<uint64_t> info(2 * n_huge), tab(n_huge);
+  std::vector<uint32_t> a2s(n_huge);
+  KC_HIP(hipMemcpyAsync(info.data(), g->d_huge_info, 16 * n_huge, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipMemcpyAsync(tab.data(), huge_tab, 8 * n_huge, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipMemcpyAsync(a2s.data(), huge, 4 * n_huge, hipMemcpyDeviceToHost, s));
+  sync(g);
+  std::vector<HugeChunk> elem, slot;
+  for (uint64_t h = 0; h < n_huge; ++h) {
+    const uint64_t size = info[2 * h + 1];
+    uint64_t tsz = 1;
+    while (tsz < 2 * size) tsz <<= 1;
+    HugeChunk c{info[2 * h], tab[h], 0, 0, static_cast<uint32_t>(tsz - 1), static_cast<uint32_t>(h), a2s[h], 0};
+    for (uint64_t lo = 0; lo < size; lo += kHugeChunk) {
+      c.lo = lo;
+      c.hi = std::min(size, lo + kHugeChunk);
+      elem.push_back(c);
+    }
+    for (uint64_t lo = 0; lo < tsz; lo += kHugeChunk) {
+      c.lo = lo;
+      c.hi = std::min(tsz, lo + kHugeChunk);
+      slot.push_back(c);
+    }
+  }
+  if (elem.size() + slot.size() > g->chunks_cap) {
+    g->chunks_cap = elem.size() + slot.size() + 1024;
+    g->d_chunks = g->dalloc<HugeChunk>(g->chunks_cap);
+  }
+  KC_HIP(hipMemcpyAsync(g->d_chunks, elem.data(), elem.size() * sizeof(HugeChunk), hipMemcpyHostToDevice, s));
+  KC_HIP(hipMemcpyAsync(g->d_chunks + elem.size(), slot.data(), slot.size() * sizeof(HugeChunk),
+                        hipMemcpyHostToDevice, s));
+  if (huge_slots > g->huge_cap) {
+    g->huge_cap = huge_slots + (huge_slots >> 2);
+    g->huge_tables = g->dalloc<uint32_t>(g->huge_cap);
+  }
+  if (n_huge > g->huge_info_cap) fail(KC_ERR_INVALID, "huge bucket list overflow");
+  fill_u32(g->huge_tables, huge_slots, kNone, s);
+  uint32_t *hcount = reinterpret_cast<uint32_t *>(g->d_huge_info);  // sizes already copied out
+  fill_u32(hcount, n_huge, 0, s);
+  TimedScope ts(g->timer, KC_K_DEDUP, 0);
+  dedup_huge(g->d_chunks, static_cast<uint32_t>(elem.size()), g->d_chunks + elem.size(),
+             static_cast<uint32_t>(slot.size()), g->huge_tables, bucket, uniq, hcount, s);
+  // the chunk vectors die here: wait until the uploads have been consumed
+  sync(g);
 }
 
 struct RunOut {
@@ -482,8 +556,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       continue;
     }
     if (b.nH) {
-      TimedScope ts(g->timer, KC_K_COUNT, 8ull * b.nH + 4ull * b.nH);
-      hub_fold(b, g->hub_inter, g->deg_inter, g->stream);
+      TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
+      hub_fold(b, g->hub_inter, g->deg_inter, g->hub_pos, g->stream);
     }
     LevelRec rec;
     rec.n = n_l;
@@ -497,61 +571,51 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       rec.keys_copy = g->arena.alloc_n<uint64_t>(n_l);
       KC_HIP(hipMemcpyAsync(rec.keys_copy, g->key, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
     }
-    fill_u32(g->cnt, n_l, 0, g->stream);
     fill_u32(g->small + 4, 1, 0, g->stream);
     {
-      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + 4 * n_l + 8 * n_l);
-      cluster_count(n_l, g->key, g->deg_inter, rec.lab, g->cnt, g->small + 4, !local, g->stream);
+      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
+      labels_radius(n_l, g->key, rec.lab, g->small + 4, !local, g->stream);
     }
+    // clusters as segments of the (label, vertex) list sorted by label
     {
-      TimedScope ts(g->timer, KC_K_SCAN, 4 * n_l + 8 * n_l + 4 * n_l + 12 * n_l);
-      kept_scan(n_l, g->cnt, gid_l, rec.newidx, g->bucket_off, gid_next, g->old_of, g->sizes, g->scan_scratch,
-                g->stream);
+      TimedScope ts(g->timer, KC_K_SCAN, 2 * 4 * n_l + 8 * n_l + 8 * n_l);
+      select_inter(n_l, g->key, g->deg_inter, g->sk, g->sv, g->sizes + 8, g->scan_scratch, g->stream);
+    }
+    d2h(g, g->h_sizes + 8, g->sizes + 8, sizeof(uint64_t));
+    sync(g);
+    const uint64_t N = g->h_sizes[8];
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < n_l) ++bits;
+    int which = 0;
+    {
+      TimedScope ts(g->timer, KC_K_SORT, 24 * N * static_cast<uint64_t>((bits + 7) / 8));
+      which = radix_sort_pairs(N, bits, g->sk, g->sv, g->sk2, g->sv2, g->sort_scratch, g->stream);
+    }
+    const uint32_t *sk = which ? g->sk2 : g->sk, *sv = which ? g->sv2 : g->sv;
+    fill_u32(rec.newidx, n_l, kNone, g->stream);
+    {
+      TimedScope ts(g->timer, KC_K_SCAN, 2 * (8 * N + 4 * N) + 4 * N + 20 * n_l / 8);
+      segments(N, sk, sv, g->deg_inter, gid_l, g->pos_of, rec.newidx, g->old_of, gid_next, g->seg_off, g->sizes,
+               g->scan_scratch, g->stream);
     }
     d2h(g, g->h_sizes, g->sizes, 2 * sizeof(uint64_t));
     d2h(g, g->h_small + 4, g->small + 4, sizeof(uint32_t));
     sync(g);
-    const uint64_t n_next = g->h_sizes[0];
-    const uint64_t P = g->h_sizes[1];
+    const uint64_t n_next = N ? g->h_sizes[0] : 0;
+    const uint64_t P = N ? g->h_sizes[1] : 0;
     rec.radius = local ? 0 : g->h_small[4];
     out.sweeps += sweeps;
     out.radius_sum += rec.radius;
     g->levels.push_back(rec);
     if (n_next == 0) break;  // every cluster is a whole component
 
-    // relabel: scatter cluster edges into the buckets of their centres
-    fill_u32(g->fill, n_l, 0, g->stream);
+    // relabel: every vertex writes the next-level labels of its cluster edges at its position
     {
-      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 12 * m2_l + 4 * n_l + 8 * P + 8 * n_l);
-      scatter(off_l, col_l, g->key, b, g->deg_inter, g->hub_inter, rec.newidx, g->bucket_off, g->fill, g->bucket,
+      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 12 * m2_l + 8 * n_l + 8 * P);
+      scatter(off_l, col_l, g->key, b, g->deg_inter, g->hub_inter, g->hub_pos, rec.newidx, g->pos_of, g->bucket,
               g->stream);
     }
-    // dedup of every bucket
-    fill_u64(g->sizes + 3, 3, 0, g->stream);
-    DedupLists dl{g->dl_medium, g->dl_huge, g->dl_huge_tab, g->sizes + 3};
-    {
-      TimedScope ts(g->timer, KC_K_DEDUP, 8 * P);
-      dedup_small(n_next, g->old_of, g->cnt, g->bucket_off, g->bucket, g->uniq, dl, g->stream);
-    }
-    d2h(g, g->h_sizes + 3, g->sizes + 3, 3 * sizeof(uint64_t));
-    sync(g);
-    const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
-    if (n_medium) {
-      TimedScope ts(g->timer, KC_K_DEDUP, 0);
-      dedup_medium(static_cast<uint32_t>(n_medium), g->dl_medium, g->old_of, g->cnt, g->bucket_off, g->bucket, g->uniq,
-                   g->stream);
-    }
-    if (n_huge) {
-      if (huge_slots > g->huge_cap) {
-        g->huge_cap = huge_slots + (huge_slots >> 2);
-        g->huge_tables = g->dalloc<uint32_t>(g->huge_cap);
-      }
-      fill_u32(g->huge_tables, huge_slots, kNone, g->stream);
-      fill_u32(g->hcount, n_huge, 0, g->stream);
-      TimedScope ts(g->timer, KC_K_DEDUP, 0);
-      dedup_huge(static_cast<uint32_t>(n_huge), g->dl_huge, g->dl_huge_tab, g->huge_tables, g->old_of, g->cnt,
-                 g->bucket_off, g->bucket, g->uniq, g->hcount, g->stream);
-    }
+    dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab);
     // next-level CSR
     uint64_t *off_next = g->off_pp[pp];
     uint32_t *col_next = g->col_pp[pp];
@@ -592,14 +656,14 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   }
   // canonical labels: minimum vertex id per component
   fill_u32(g->minid, g->n, kNone, g->stream);
-  fill_u32(g->small + 5, 1, 0, g->stream);
   {
     TimedScope ts(g->timer, KC_K_CANON, 8 * g->n);
     canon_min(g->n, rep_up, g->minid, g->stream);
   }
   {
     TimedScope ts(g->timer, KC_K_CANON, 12 * g->n);
-    canon_label(g->n, rep_up, g->minid, g->labels, g->small + 5, g->stream);
+    canon_label(g->n, rep_up, g->minid, g->labels, g->small + 5, reinterpret_cast<uint32_t *>(g->scan_scratch),
+                g->stream);
   }
 }
 
@@ -617,9 +681,8 @@ void run_propagation(kc_graph *g, const kc_options *o, RunOut &out) {
   out.sweeps = sweep_to_fixpoint(g, g->n, g->m2, g->off0, g->col0, g->key, 0, b, false,
                                  o->variant == KC_VARIANT_SHORTCUT, flag_base);
   out.levels = 0;
-  fill_u32(g->small + 5, 1, 0, g->stream);
   TimedScope ts(g->timer, KC_K_CANON, 12 * g->n);
-  labels_from_keys(g->n, g->key, g->labels, g->small + 5, g->stream);
+  labels_from_keys(g->n, g->key, g->labels, g->small + 5, reinterpret_cast<uint32_t *>(g->scan_scratch), g->stream);
 }
 
 }  // namespace
@@ -673,17 +736,22 @@ void init_work(kc_graph *g) {
   g->hubs_cap = m2 / (kMinWaveBin / 2) + 2;
   g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
   g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
-  g->cnt = g->dalloc<uint32_t>(n);
-  g->fill = g->dalloc<uint32_t>(n);
   g->bucket = g->dalloc<uint32_t>(m2);
   g->old_of = g->dalloc<uint32_t>(n);
   g->uniq = g->dalloc<uint32_t>(n);
-  g->bucket_off = g->dalloc<uint64_t>(n + 1);
+  g->pos_of = g->dalloc<uint32_t>(n);
+  g->hub_pos = g->dalloc<uint32_t>(g->hubs_cap);
+  g->sk = g->dalloc<uint32_t>(n);
+  g->sv = g->dalloc<uint32_t>(n);
+  g->sk2 = g->dalloc<uint32_t>(n);
+  g->sv2 = g->dalloc<uint32_t>(n);
+  g->seg_off = g->dalloc<uint64_t>(n + 1);
+  g->sort_scratch = g->dalloc<uint64_t>(radix_sort_scratch_words(n));
   g->rep_a = g->dalloc<uint32_t>(n);
   g->rep_b = g->dalloc<uint32_t>(n);
   g->minid = g->dalloc<uint32_t>(n);
   g->labels = g->dalloc<uint32_t>(n);
-  g->scan_scratch = g->dalloc<uint64_t>(scan_scratch_words(n + m2) + 1024);
+  g->scan_scratch = g->dalloc<uint64_t>(std::max(scan_scratch_words(n + m2) + 1024, bin_scratch_words(n)));
   for (int i = 0; i < 2; ++i) {
     g->off_pp[i] = g->dalloc<uint64_t>(n + 1);
     g->col_pp[i] = g->dalloc<uint32_t>(m2);
@@ -730,22 +798,7 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   uint32_t *medium = lists.as<uint32_t>();
   uint32_t *huge = medium + n + 2;
   uint64_t *huge_tab = reinterpret_cast<uint64_t *>(huge + n + 2) + 1;
-  fill_u64(g->sizes + 3, 3, 0, s);
-  DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
-  dedup_small(n, nullptr, deg.as<uint32_t>(), off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), dl, s);
-  KC_HIP(hipMemcpyAsync(g->h_sizes + 3, g->sizes + 3, 3 * 8, hipMemcpyDeviceToHost, s));
-  KC_HIP(hipStreamSynchronize(s));
-  const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
-  dedup_medium(static_cast<uint32_t>(n_medium), medium, nullptr, deg.as<uint32_t>(), off_tmp.as<uint64_t>(),
-               bucket.as<uint32_t>(), uniq.as<uint32_t>(), s);
-  if (n_huge) {
-    DevTemp tables(huge_slots * 4 + 64), hcount(n_huge * 4 + 64);
-    fill_u32(tables.as<uint32_t>(), huge_slots, kNone, s);
-    fill_u32(hcount.as<uint32_t>(), n_huge, 0, s);
-    dedup_huge(static_cast<uint32_t>(n_huge), huge, huge_tab, tables.as<uint32_t>(), nullptr, deg.as<uint32_t>(),
-               off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), hcount.as<uint32_t>(), s);
-    KC_HIP(hipStreamSynchronize(s));
-  }
+  dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), medium, huge, huge_tab);
   g->off0 = g->dalloc<uint64_t>(n + 1);
   csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));

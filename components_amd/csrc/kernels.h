@@ -61,54 +61,74 @@ void init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t i
                        hipStream_t s);
 void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, hipStream_t s);
 
-// Bins vertices of a CSR by degree; counters[0..3] (zeroed by the caller)
+// Bins vertices of a CSR by degree (stable, no global atomics); counters[0..3]
 // receive nT, nS, nW, nH.  deg_inter of degree-0 vertices is cleared.
+// scratch: bin_scratch_words(n) words.
 void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS,
-                  uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter,
+                  uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch,
                   hipStream_t s);
 void finish_bins(Bins &b);  // computes block offsets from the counts
+uint64_t bin_scratch_words(uint64_t n);
 
 // One binned sweep.  flag: set to 1 when a key decreased; prev_flag (may be
 // null): when it reads 0 the launch returns at once (converged earlier).
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin,
            uint64_t *kout, uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter,
            int *flag, const int *prev_flag, hipStream_t s);
-void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, hipStream_t s);
+// deg_inter of hubs from their chunk counts; hub_pos[c] = offset of chunk c in its hub's run.
+void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, hipStream_t s);
 
 // Pointer jumping on keys: key[v] <- min(key[v], key[label(v)]).
 void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
 
-// lab[v] = low half of key[v]; cnt[lab] += deg_inter[v]; max cluster radius.
-void cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab,
-                   uint32_t *cnt, uint32_t *radius, bool with_radius, hipStream_t s);
-
-// Kept clusters: newidx, bucket offsets, next-level gid, old index of each
-// kept cluster.  sizes[0] = n_next, sizes[1] = total bucket entries.
-void kept_scan(uint64_t n, const uint32_t *cnt, const uint32_t *gid, uint32_t *newidx,
-               uint64_t *bucket_off, uint32_t *gid_next, uint32_t *old_of, uint64_t *sizes,
-               uint64_t *scratch, hipStream_t s);
+// lab[v] = low half of key[v]; max cluster radius (atomicMax into *radius).
+void labels_radius(uint64_t n, const uint64_t *key, uint32_t *lab, uint32_t *radius, bool with_radius,
+                   hipStream_t s);
+// (label, vertex) of every vertex with a cluster edge; *count (device) = N.
+void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *sk, uint32_t *sv,
+                  uint64_t *count, uint64_t *scratch, hipStream_t s);
+// Over the label-sorted list: pos_of[v] = write position of v's cluster edges,
+// newidx[a] = next-level index of label a (caller pre-fills kNone), old_of and
+// gid_next by next-level index, seg_off[0..n_next] bucket offsets;
+// sizes[0] = n_next, sizes[1] = P (cluster edges).
+void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t *deg_inter, const uint32_t *gid,
+              uint32_t *pos_of, uint32_t *newidx, uint32_t *old_of, uint32_t *gid_next, uint64_t *seg_off,
+              uint64_t *sizes, uint64_t *scratch, hipStream_t s);
 
 void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b,
-             const uint32_t *deg_inter, const uint32_t *hub_inter, const uint32_t *newidx,
-             const uint64_t *bucket_off, uint32_t *fill, uint32_t *bucket, hipStream_t s);
+             const uint32_t *deg_inter, const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *newidx,
+             const uint32_t *pos_of, uint32_t *bucket, hipStream_t s);
 
-// Dedup of every kept bucket; lists[0..2] count medium/huge buckets and huge
-// table slots (zeroed by the caller).  old_of == nullptr: bucket a2 is row a2.
+// Dedup of every bucket a2 (elements [seg_off[a2], seg_off[a2+1])): small ones
+// in wave LDS tables; medium and huge ones are listed (counters zeroed by the
+// caller: [0] medium count, [1] huge count, [2] huge table slots).
 struct DedupLists {
-  uint32_t *medium;    // a' of buckets with wave_cap < size <= block_cap
-  uint32_t *huge;      // a' of buckets above block_cap
+  uint32_t *medium;    // a2 of buckets with wave_cap < size <= block_cap
+  uint32_t *huge;      // a2 of buckets above block_cap
   uint64_t *huge_tab;  // table offset per huge bucket
-  uint64_t *counters;  // [0] medium count, [1] huge count, [2] huge table slots
+  uint64_t *counters;
 };
 constexpr uint32_t kDedupWaveCap = 512;
 constexpr uint32_t kDedupBlockCap = 8192;
-void dedup_small(uint64_t n_next, const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off,
-                 uint32_t *bucket, uint32_t *uniq, DedupLists l, hipStream_t s);
-void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint32_t *old_of, const uint32_t *cnt,
-                  const uint64_t *bucket_off, uint32_t *bucket, uint32_t *uniq, hipStream_t s);
-void dedup_huge(uint32_t n_huge, const uint32_t *huge, const uint64_t *huge_tab, uint32_t *tables,
-                const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off, uint32_t *bucket,
-                uint32_t *uniq, uint32_t *hcount, hipStream_t s);
+constexpr uint64_t kHugeChunk = 16384;  // elements (or table slots) per huge-bucket chunk
+// A fixed-size slice of a huge bucket: elements or table slots [lo, hi).
+struct HugeChunk {
+  uint64_t base;      // bucket offset of the huge bucket
+  uint64_t tab;       // table offset
+  uint64_t lo, hi;
+  uint32_t tab_mask;  // table size - 1
+  uint32_t huge;      // index in the huge list
+  uint32_t a2;        // bucket (next-level vertex)
+  uint32_t pad;
+};
+void dedup_small(uint64_t n_next, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq, DedupLists l,
+                 hipStream_t s);
+void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, uint32_t *bucket,
+                  uint32_t *uniq, hipStream_t s);
+// out[2h] = bucket offset, out[2h+1] = size of huge bucket h.
+void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out, hipStream_t s);
+void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,
+                uint32_t *tables, uint32_t *bucket, uint32_t *uniq, uint32_t *hcount, hipStream_t s);
 
 // off_next = exclusive scan of uniq (n_next+1 entries); sizes[2] = total.
 void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint64_t *sizes,
@@ -116,15 +136,26 @@ void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint
 // Stream compaction of bucket[0..total) dropping kNone into col_next.
 void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, uint64_t *scratch, hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
+void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, uint64_t *total, uint64_t *scratch,
+                               hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits
+// (sort.hip).  Returns 0 when keys/vals hold the result, 1 for tkeys/tvals.
+uint64_t radix_sort_scratch_words(uint64_t n);
+int radix_sort_pairs(uint64_t n, int bits, uint32_t *keys, uint32_t *vals, uint32_t *tkeys, uint32_t *tvals,
+                     uint64_t *scratch, hipStream_t s);
 
 void undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_t *gid,
           const uint32_t *rep_up, uint32_t *rep_out, hipStream_t s);
 // rep of a base-case level: rep[v] = gid[low(key[v])].
 void rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_t *rep, hipStream_t s);
 void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s);
+// roots (device) = number of vertices that are their own label; partial needs
+// ceil(n / 256) words.
 void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,
-                 uint32_t *roots, hipStream_t s);
-void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, hipStream_t s);
+                 uint32_t *roots, uint32_t *partial, hipStream_t s);
+void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, uint32_t *partial,
+                      hipStream_t s);
 
 // Device generators (gen_kernels.hip): the edge streams of the host generators.
 void gen_gnm(uint64_t count, uint64_t n, int32_t seed, uint32_t *src, uint32_t *dst, hipStream_t s);

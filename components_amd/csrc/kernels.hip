@@ -90,40 +90,94 @@ __global__ void k_deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, ui
 }
 
 // ---------------------------------------------------------------------------
-// Degree binning.
-__global__ void k_bin(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS,
-                      uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter) {
-  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  uint64_t deg = 0, e0 = 0;
+// Degree binning, two passes without global atomics: per-block bin counts,
+// one exclusive scan over the bin-major count table, then a stable write.
+__device__ __forceinline__ int bin_of(uint64_t deg, const BinCfg &cfg) {
+  return deg == 0 ? -1 : deg <= cfg.tmax ? 0 : deg <= cfg.smax ? 1 : deg <= cfg.wmax ? 2 : 3;
+}
+
+// Per wave: rank of each lane among the lanes of its bin, and the wave's
+// per-bin totals in LDS (bins 0..2 count vertices, bin 3 counts hub chunks).
+__device__ __forceinline__ void bin_ranks(int bin, uint32_t chunks, uint32_t *rank, uint32_t wave_tot[4]) {
+  for (int r = 0; r < 3; ++r) {
+    const uint64_t m = __ballot(bin == r);
+    if (bin == r) *rank = static_cast<uint32_t>(__popcll(m & lanes_below()));
+    wave_tot[r] = static_cast<uint32_t>(__popcll(m));
+  }
+  uint32_t x = bin == 3 ? chunks : 0u, incl = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (static_cast<int>(lane_id()) >= o) incl += t;
+  }
+  if (bin == 3) *rank = incl - x;
+  wave_tot[3] = __shfl(incl, 63);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_count(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *counts,
+                                                     uint32_t nblocks) {
+  __shared__ uint32_t tot[kBlock / kWave][4];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+  const uint64_t deg = v < n ? off[v + 1] - off[v] : 0;
+  const int bin = bin_of(deg, cfg);
+  const uint32_t chunks = bin == 3 ? static_cast<uint32_t>(ceil_div(deg, cfg.wmax)) : 0u;
+  uint32_t rank = 0, wt[4];
+  bin_ranks(bin, chunks, &rank, wt);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0)
+    for (int r = 0; r < 4; ++r) tot[w][r] = wt[r];
+  __syncthreads();
+  if (threadIdx.x < 4)
+    counts[static_cast<uint64_t>(threadIdx.x) * nblocks + blockIdx.x] =
+        tot[0][threadIdx.x] + tot[1][threadIdx.x] + tot[2][threadIdx.x] + tot[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_write(uint64_t n, const uint64_t *off, BinCfg cfg,
+                                                     const uint64_t *prefix, uint32_t nblocks, uint32_t *listT,
+                                                     uint32_t *listS, uint32_t *listW, HubChunk *hubs,
+                                                     uint32_t *deg_inter) {
+  __shared__ uint32_t tot[kBlock / kWave][4];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+  uint64_t e0 = 0, deg = 0;
   if (v < n) {
     e0 = off[v];
     deg = off[v + 1] - e0;
     if (deg == 0 && deg_inter) deg_inter[v] = 0;
   }
-  const bool live = v < n && deg > 0;
-  const bool inT = live && deg <= cfg.tmax;
-  const bool inS = live && deg > cfg.tmax && deg <= cfg.smax;
-  const bool inW = live && deg > cfg.smax && deg <= cfg.wmax;
-  const bool inH = live && deg > cfg.wmax;
-  uint32_t p;
-  p = wave_append(inT, &counters[0]);
-  if (inT) listT[p] = static_cast<uint32_t>(v);
-  p = wave_append(inS, &counters[1]);
-  if (inS) listS[p] = static_cast<uint32_t>(v);
-  p = wave_append(inW, &counters[2]);
-  if (inW) listW[p] = static_cast<uint32_t>(v);
-  if (inH) {
-    uint32_t chunks = static_cast<uint32_t>(ceil_div(deg, cfg.wmax));
-    uint32_t base = atomicAdd(&counters[3], chunks);
+  const int bin = bin_of(deg, cfg);
+  const uint32_t chunks = bin == 3 ? static_cast<uint32_t>(ceil_div(deg, cfg.wmax)) : 0u;
+  uint32_t rank = 0, wt[4];
+  bin_ranks(bin, chunks, &rank, wt);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0)
+    for (int r = 0; r < 4; ++r) tot[w][r] = wt[r];
+  __syncthreads();
+  if (bin < 0) return;
+  uint64_t pos = prefix[static_cast<uint64_t>(bin) * nblocks + blockIdx.x] - prefix[static_cast<uint64_t>(bin) * nblocks];
+  for (int w2 = 0; w2 < w; ++w2) pos += tot[w2][bin];
+  pos += rank;
+  if (bin == 0) {
+    listT[pos] = static_cast<uint32_t>(v);
+  } else if (bin == 1) {
+    listS[pos] = static_cast<uint32_t>(v);
+  } else if (bin == 2) {
+    listW[pos] = static_cast<uint32_t>(v);
+  } else {
     for (uint32_t c = 0; c < chunks; ++c) {
       HubChunk h;
       h.v = static_cast<uint32_t>(v);
       h.first = c == 0;
       h.e0 = e0 + static_cast<uint64_t>(c) * cfg.wmax;
       h.e1 = min(e0 + deg, h.e0 + cfg.wmax);
-      hubs[base + c] = h;
+      hubs[pos + c] = h;
     }
   }
+}
+
+__global__ void k_bin_totals(const uint64_t *prefix, const uint64_t *total, uint32_t nblocks, uint32_t *counters) {
+  const int r = threadIdx.x;
+  if (r >= 4) return;
+  const uint64_t hi = r < 3 ? prefix[static_cast<uint64_t>(r + 1) * nblocks] : *total;
+  counters[r] = static_cast<uint32_t>(hi - prefix[static_cast<uint64_t>(r) * nblocks]);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,12 +272,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
   }
 }
 
-__global__ void k_hub_fold(HubChunk *hubs, uint32_t nH, const uint32_t *hub_inter, uint32_t *deg_inter) {
+// Sums the chunk counts of every hub into deg_inter and records each chunk's
+// offset inside its hub's run of cluster edges.
+__global__ void k_hub_fold(HubChunk *hubs, uint32_t nH, const uint32_t *hub_inter, uint32_t *deg_inter,
+                           uint32_t *hub_pos) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nH || !hubs[i].first) return;
   const uint32_t v = hubs[i].v;
   uint32_t sum = 0;
-  for (uint32_t c = i; c < nH && hubs[c].v == v; ++c) sum += hub_inter[c];
+  for (uint32_t c = i; c < nH && hubs[c].v == v; ++c) {
+    hub_pos[c] = sum;
+    sum += hub_inter[c];
+  }
   deg_inter[v] = sum;
 }
 
@@ -242,22 +302,19 @@ __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
 }
 
 // ---------------------------------------------------------------------------
-// Cluster counting.
-__global__ void k_cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab,
-                                uint32_t *cnt, uint32_t *radius, int with_radius) {
+// Centre labels and the maximum cluster radius (hops from the centre).
+__global__ void k_labels_radius(uint64_t n, const uint64_t *key, uint32_t *lab, uint32_t *radius, int with_radius) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  uint32_t r = 0, l = 0, d = 0;
+  uint32_t r = 0;
   if (v < n) {
     const uint64_t k = key[v];
-    l = static_cast<uint32_t>(k);
+    const uint32_t l = static_cast<uint32_t>(k);
     lab[v] = l;
-    d = deg_inter[v];
     if (with_radius) r = static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[l] >> 32);
   }
-  peeled_fetch_add(cnt, l, d, d != 0);
   if (with_radius) {
     for (int o = 32; o > 0; o >>= 1) r = max(r, static_cast<uint32_t>(__shfl_xor(r, o)));
-    if (lane_id() == 0 && r) atomicMax(radius, r);
+    if (lane_id() == 0 && r > *radius) atomicMax(radius, r);
   }
 }
 
@@ -335,6 +392,10 @@ struct ArrayLoad {
   const T *p;
   __device__ T operator()(uint64_t i) const { return p[i]; }
 };
+struct ArrLoadU32 {
+  const uint32_t *p;
+  __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
+};
 template <class T>
 struct ArrayStore {
   T *p;
@@ -381,37 +442,56 @@ void scan_exclusive(uint64_t n, Load load, Store store, T *total, uint64_t *scra
 }
 
 // ---------------------------------------------------------------------------
-// Kept clusters.
-struct KeptLoad {
-  const uint32_t *cnt;
-  __device__ Pair64 operator()(uint64_t i) const {
-    const uint32_t c = cnt[i];
-    return {c != 0 ? 1ull : 0ull, c};
-  }
+// Clusters from the (label, vertex) list sorted by label: every vertex with a
+// cluster edge is selected, the list is sorted, and one scan over the sorted
+// list yields each cluster's next-level index (segment number), its bucket
+// offset and every vertex's write position -- no atomics.
+struct SelectLoad {
+  const uint32_t *deg_inter;
+  __device__ uint64_t operator()(uint64_t i) const { return deg_inter[i] != 0 ? 1ull : 0ull; }
 };
-struct KeptStore {
-  const uint32_t *gid;
-  uint32_t *newidx;
-  uint64_t *bucket_off;
-  uint32_t *gid_next;
-  uint32_t *old_of;
-  __device__ void operator()(uint64_t i, Pair64 prefix, Pair64 v) const {
-    bucket_off[i] = prefix.b;
-    if (v.a) {
-      const uint32_t a2 = static_cast<uint32_t>(prefix.a);
-      newidx[i] = a2;
-      gid_next[a2] = gid ? gid[i] : static_cast<uint32_t>(i);
-      old_of[a2] = static_cast<uint32_t>(i);
-    } else {
-      newidx[i] = kNone;
+struct SelectStore {
+  const uint64_t *key;
+  uint32_t *sk, *sv;
+  __device__ void operator()(uint64_t i, uint64_t prefix, uint64_t v) const {
+    if (v) {
+      sk[prefix] = static_cast<uint32_t>(key[i]);
+      sv[prefix] = static_cast<uint32_t>(i);
     }
   }
 };
 
-__global__ void k_pair_to_sizes(const Pair64 *p, uint64_t *sizes) {
-  sizes[0] = p->a;
-  sizes[1] = p->b;
+struct SegLoad {
+  const uint32_t *sk, *sv, *deg_inter;
+  __device__ Pair64 operator()(uint64_t i) const {
+    const uint32_t a = sk[i];
+    const bool head = i == 0 || sk[i - 1] != a;
+    return {head ? 1ull : 0ull, deg_inter[sv[i]]};
+  }
+};
+struct SegStore {
+  const uint32_t *sk, *sv, *gid;
+  uint32_t *pos_of, *newidx, *old_of, *gid_next;
+  uint64_t *seg_off;
+  __device__ void operator()(uint64_t i, Pair64 prefix, Pair64 v) const {
+    pos_of[sv[i]] = static_cast<uint32_t>(prefix.b);
+    if (v.a) {
+      const uint32_t a = sk[i];
+      const uint32_t seg = static_cast<uint32_t>(prefix.a);
+      newidx[a] = seg;
+      old_of[seg] = a;
+      gid_next[seg] = gid ? gid[a] : a;
+      seg_off[seg] = prefix.b;
+    }
+  }
+};
+
+__global__ void k_seg_finish(const Pair64 *total, uint64_t *seg_off, uint64_t *sizes) {
+  seg_off[total->a] = total->b;
+  sizes[0] = total->a;
+  sizes[1] = total->b;
 }
+__global__ void k_copy_total(const uint64_t *t, uint64_t *dst) { *dst = *t; }
 
 // ---------------------------------------------------------------------------
 // Scatter of cluster edges into buckets (binned like the sweep).
@@ -422,23 +502,24 @@ struct ScatterArgs {
   Bins b;
   const uint32_t *deg_inter;
   const uint32_t *hub_inter;
+  const uint32_t *hub_pos;
   const uint32_t *newidx;
-  const uint64_t *bucket_off;
-  uint32_t *fill;
+  const uint32_t *pos_of;
   uint32_t *bucket;
 };
 
 template <int W>
 __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active, uint32_t v, uint64_t e0,
-                                              uint64_t e1, int gl, uint32_t d) {
+                                              uint64_t e1, int gl, uint32_t d, uint32_t extra) {
   // group base lane within the wave
   const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
   uint32_t lv = 0;
   uint64_t pos = 0;
   active = active && d > 0;
-  if (active) lv = static_cast<uint32_t>(a.key[v]);
-  const uint32_t slot = peeled_fetch_add(a.fill, lv, d, active && gl == 0);
-  if (active && gl == 0) pos = a.bucket_off[lv] + slot;
+  if (active) {
+    lv = static_cast<uint32_t>(a.key[v]);
+    if (gl == 0) pos = static_cast<uint64_t>(a.pos_of[v]) + extra;
+  }
   if (W > 1) pos = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), gbase, 64));
   if (W == 1) {
     if (!active) return;
@@ -478,24 +559,24 @@ __global__ __launch_bounds__(kBlock) void k_scatter(ScatterArgs a) {
     const bool act = i < a.b.nT;
     uint32_t v = act ? a.b.listT[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    scatter_group<1>(a, act, v, e0, e1, 0, act ? a.deg_inter[v] : 0);
+    scatter_group<1>(a, act, v, e0, e1, 0, act ? a.deg_inter[v] : 0, 0);
   } else if (blk < a.b.bW) {
     const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
     const bool act = i < a.b.nS;
     uint32_t v = act ? a.b.listS[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    scatter_group<16>(a, act, v, e0, e1, tid & 15, act ? a.deg_inter[v] : 0);
+    scatter_group<16>(a, act, v, e0, e1, tid & 15, act ? a.deg_inter[v] : 0, 0);
   } else if (blk < a.b.bH) {
     const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nW;
     uint32_t v = act ? a.b.listW[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    scatter_group<kWave>(a, act, v, e0, e1, tid & 63, act ? a.deg_inter[v] : 0);
+    scatter_group<kWave>(a, act, v, e0, e1, tid & 63, act ? a.deg_inter[v] : 0, 0);
   } else {
     const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nH;
     HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
-    scatter_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, act ? a.hub_inter[i] : 0);
+    scatter_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, act ? a.hub_inter[i] : 0, act ? a.hub_pos[i] : 0);
   }
 }
 
@@ -520,18 +601,16 @@ __device__ __forceinline__ bool lds_insert(uint32_t *tab, uint32_t mask, uint32_
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint32_t *old_of, const uint32_t *cnt,
-                                                       const uint64_t *bucket_off, uint32_t *bucket, uint32_t *uniq,
-                                                       DedupLists l) {
+__global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint64_t *seg_off, uint32_t *bucket,
+                                                       uint32_t *uniq, DedupLists l) {
   __shared__ uint32_t tabs[kBlock / kWave][kWaveTab];
   const int w = threadIdx.x / kWave;
   const int ln = threadIdx.x & 63;
   uint32_t *tab = tabs[w];
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
   for (uint64_t a2 = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + w; a2 < n_next; a2 += waves) {
-    const uint32_t a = old_of ? old_of[a2] : static_cast<uint32_t>(a2);
-    const uint32_t s = cnt[a];
-    const uint64_t base = bucket_off[a];
+    const uint64_t base = seg_off[a2];
+    const uint32_t s = static_cast<uint32_t>(seg_off[a2 + 1] - base);
     if (s > kDedupWaveCap) {
       if (ln == 0) {
         if (s <= kDedupBlockCap) {
@@ -577,17 +656,14 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
 }
 
 __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, const uint32_t *medium,
-                                                        const uint32_t *old_of, const uint32_t *cnt,
-                                                        const uint64_t *bucket_off, uint32_t *bucket,
-                                                        uint32_t *uniq) {
+                                                        const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq) {
   __shared__ uint32_t tab[kBlockTab];
   __shared__ uint32_t psum[kBlock];
   const int tid = threadIdx.x;
   for (uint32_t i = blockIdx.x; i < n_medium; i += gridDim.x) {
     const uint32_t a2 = medium[i];
-    const uint32_t a = old_of ? old_of[a2] : static_cast<uint32_t>(a2);
-    const uint32_t s = cnt[a];
-    const uint64_t base = bucket_off[a];
+    const uint64_t base = seg_off[a2];
+    const uint32_t s = static_cast<uint32_t>(seg_off[a2 + 1] - base);
     uint32_t tsz = 2;
     while (tsz < 2u * s) tsz <<= 1;
     const uint32_t mask = tsz - 1;
@@ -611,73 +687,88 @@ __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, cons
   }
 }
 
-// Huge buckets: one global open-addressing table per bucket (pre-filled with kNone).
-__global__ __launch_bounds__(kBlock) void k_dedup_huge_insert(uint32_t n_huge, const uint32_t *huge,
-                                                             const uint64_t *huge_tab, uint32_t *tables,
-                                                             const uint32_t *old_of, const uint32_t *cnt,
-                                                             const uint64_t *bucket_off, const uint32_t *bucket) {
-  for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
-  const uint32_t a = old_of ? old_of[huge[h]] : huge[h];
-  const uint32_t s = cnt[a];
-  const uint64_t base = bucket_off[a];
-  uint64_t tsz = 1;
-  while (tsz < 2ull * s) tsz <<= 1;
-  const uint32_t mask = static_cast<uint32_t>(tsz - 1);
-  uint32_t *tab = tables + huge_tab[h];
-  for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < s;
-       j += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    const uint32_t b = bucket[base + j];
-    uint32_t k = hash32(b) & mask;
-    while (true) {
-      // a slot never changes once filled, so a non-empty read is exact
-      const uint32_t cur = tab[k];
-      if (cur == b) break;
-      if (cur == kNone) {
-        const uint32_t prev = atomicCAS(&tab[k], kNone, b);
-        if (prev == kNone || prev == b) break;
+// Huge buckets: one global open-addressing table per bucket (pre-filled with
+// kNone), processed in fixed-size chunks so that every block has the same work.
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_insert(const HugeChunk *chunks, uint32_t n_chunks,
+                                                             uint32_t *tables, const uint32_t *bucket) {
+  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const HugeChunk h = chunks[c];
+    uint32_t *tab = tables + h.tab;
+    const uint32_t mask = h.tab_mask;
+    for (uint64_t j = h.lo + threadIdx.x; j < h.hi; j += kBlock) {
+      const uint32_t b = bucket[h.base + j];
+      uint32_t k = hash32(b) & mask;
+      while (true) {
+        // a slot never changes once filled, so a non-empty read is exact
+        const uint32_t cur = tab[k];
+        if (cur == b) break;
+        if (cur == kNone) {
+          const uint32_t prev = atomicCAS(&tab[k], kNone, b);
+          if (prev == kNone || prev == b) break;
+        }
+        k = (k + 1) & mask;
       }
-      k = (k + 1) & mask;
     }
   }
+}
+
+// Table slots [lo, hi) of a huge bucket.  Each wave reads its slots in
+// coalesced 64-slot rows twice: first to count the kept values (one atomic per
+// block reserves the output range), then to write them with ballot prefixes.
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(const HugeChunk *chunks, uint32_t n_chunks,
+                                                           const uint32_t *tables, uint32_t *bucket, uint32_t *hcount) {
+  __shared__ uint32_t wave_tot[kBlock / kWave];
+  __shared__ uint32_t at;
+  const int w = threadIdx.x / kWave;
+  const uint32_t ln = lane_id();
+  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const HugeChunk h = chunks[c];
+    const uint32_t *tab = tables + h.tab;
+    uint32_t cnt = 0;
+    for (uint64_t r = h.lo + static_cast<uint64_t>(w) * kWave; r < h.hi; r += kBlock) {
+      const uint64_t j = r + ln;
+      cnt += __popcll(__ballot(j < h.hi && tab[j] != kNone));
+    }
+    if (ln == 0) wave_tot[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t tot = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+      at = tot ? atomicAdd(&hcount[h.huge], tot) : 0u;
+    }
+    __syncthreads();
+    uint64_t out = at;
+    for (int w2 = 0; w2 < w; ++w2) out += wave_tot[w2];
+    for (uint64_t r = h.lo + static_cast<uint64_t>(w) * kWave; r < h.hi; r += kBlock) {
+      const uint64_t j = r + ln;
+      const uint32_t val = j < h.hi ? tab[j] : kNone;
+      const bool keep = val != kNone;
+      const uint64_t m = __ballot(keep);
+      if (keep) bucket[h.base + out + __popcll(m & lanes_below())] = val;
+      out += __popcll(m);
+    }
+    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(uint32_t n_huge, const uint32_t *huge,
-                                                           const uint64_t *huge_tab, const uint32_t *tables,
-                                                           const uint32_t *old_of, const uint32_t *cnt,
-                                                           const uint64_t *bucket_off, uint32_t *bucket,
-                                                           uint32_t *hcount) {
-  for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
-  const uint32_t a = old_of ? old_of[huge[h]] : huge[h];
-  const uint32_t s = cnt[a];
-  const uint64_t base = bucket_off[a];
-  uint64_t tsz = 1;
-  while (tsz < 2ull * s) tsz <<= 1;
-  const uint32_t *tab = tables + huge_tab[h];
-  for (uint64_t j0 = static_cast<uint64_t>(blockIdx.x) * kBlock; j0 < tsz; j0 += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    const uint64_t j = j0 + threadIdx.x;
-    const uint32_t val = j < tsz ? tab[j] : kNone;
-    const bool keep = val != kNone;
-    const uint32_t p = wave_append(keep, &hcount[h]);
-    if (keep) bucket[base + p] = val;
-  }
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_dedup_huge_tail(uint32_t n_huge, const uint32_t *huge,
-                                                           const uint32_t *old_of, const uint32_t *cnt,
-                                                           const uint64_t *bucket_off, uint32_t *bucket,
-                                                           uint32_t *uniq, const uint32_t *hcount) {
-  for (uint32_t h = blockIdx.y; h < n_huge; h += gridDim.y) {
+// (offset, size) of every huge bucket, for the host to plan the chunks.
+__global__ void k_huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= n_huge) return;
   const uint32_t a2 = huge[h];
-  const uint32_t a = old_of ? old_of[a2] : static_cast<uint32_t>(a2);
-  const uint32_t s = cnt[a];
-  const uint64_t base = bucket_off[a];
-  const uint32_t u = hcount[h];
-  if (blockIdx.x == 0 && threadIdx.x == 0) uniq[a2] = u;
-  for (uint64_t j = u + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < s;
-       j += static_cast<uint64_t>(gridDim.x) * kBlock)
-    bucket[base + j] = kNone;
+  out[2 * h] = seg_off[a2];
+  out[2 * h + 1] = seg_off[a2 + 1] - seg_off[a2];
+}
+
+// Element chunks again: positions at or after the unique count become kNone;
+// the first chunk of a bucket stores the count.
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_tail(const HugeChunk *chunks, uint32_t n_chunks,
+                                                           uint32_t *bucket, uint32_t *uniq, const uint32_t *hcount) {
+  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const HugeChunk h = chunks[c];
+    const uint32_t u = hcount[h.huge];
+    if (h.lo == 0 && threadIdx.x == 0) uniq[h.a2] = u;
+    const uint64_t from = h.lo > u ? h.lo : u;
+    for (uint64_t j = from + threadIdx.x; j < h.hi; j += kBlock) bucket[h.base + j] = kNone;
   }
 }
 
@@ -695,8 +786,6 @@ struct OffStore {
     if (i + 1 == n) off[n] = prefix + v;
   }
 };
-__global__ void k_copy_total(const uint64_t *t, uint64_t *dst) { *dst = *t; }
-
 // Compaction: count non-kNone per tile, scan, write.
 __global__ __launch_bounds__(kBlock) void k_compact_count(uint64_t total, const uint32_t *src, uint64_t *counts) {
   __shared__ uint64_t smem[kBlock];
@@ -751,8 +840,17 @@ __global__ void k_canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid) {
   peeled_min(minid, live ? rep[v] : 0u, static_cast<uint32_t>(v), live);
 }
 
-__global__ void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,
-                              uint32_t *roots) {
+// Block count of the lanes with pred set, written to partial[blockIdx.x].
+__device__ __forceinline__ void block_count(bool pred, uint32_t *partial) {
+  __shared__ uint32_t wc[kBlock / kWave];
+  const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(pred)));
+  if (lane_id() == 0) wc[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid,
+                                                       uint32_t *labels, uint32_t *partial) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   bool root = false;
   if (v < n) {
@@ -760,11 +858,11 @@ __global__ void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *m
     labels[v] = l;
     root = l == static_cast<uint32_t>(v);
   }
-  const uint64_t m = __ballot(root);
-  if (m && lane_id() == 0) atomicAdd(roots, static_cast<uint32_t>(__popcll(m)));
+  block_count(root, partial);
 }
 
-__global__ void k_labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots) {
+__global__ __launch_bounds__(kBlock) void k_labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels,
+                                                            uint32_t *partial) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   bool root = false;
   if (v < n) {
@@ -772,8 +870,17 @@ __global__ void k_labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *la
     labels[v] = l;
     root = l == static_cast<uint32_t>(v);
   }
-  const uint64_t m = __ballot(root);
-  if (m && lane_id() == 0) atomicAdd(roots, static_cast<uint32_t>(__popcll(m)));
+  block_count(root, partial);
+}
+
+// out = sum of partial[0..count) (one block).
+__global__ __launch_bounds__(kBlock) void k_reduce_sum(const uint32_t *partial, uint64_t count, uint32_t *out) {
+  __shared__ uint64_t smem[kBlock];
+  uint64_t acc = 0;
+  for (uint64_t i = threadIdx.x; i < count; i += kBlock) acc += partial[i];
+  uint64_t tot;
+  block_exclusive_scan_shift<uint64_t>(acc, smem, &tot);
+  if (threadIdx.x == 0) *out = static_cast<uint32_t>(tot);
 }
 
 }  // namespace
@@ -814,10 +921,23 @@ void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, 
 }
 
 void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS, uint32_t *listW,
-                  HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, hipStream_t s) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(n)), dim3(kBlock), 0, s, n, off, cfg, listT, listS, listW, hubs, counters,
+                  HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch, hipStream_t s) {
+  if (!n) {
+    fill_u32(counters, 4, 0, s);
+    return;
+  }
+  const uint32_t nb = static_cast<uint32_t>(ceil_div(n, kBlock));
+  uint32_t *counts = reinterpret_cast<uint32_t *>(scratch);
+  uint64_t *prefix = scratch + ceil_div(4ull * nb, 2) + 8;
+  uint64_t *total = prefix + 4ull * nb + 8;
+  uint64_t *rest = total + 8;
+  hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(kBlock), 0, s, n, off, cfg, counts, nb);
+  KC_LAUNCH_CHECK();
+  scan_exclusive<uint64_t>(4ull * nb, ArrLoadU32{counts}, ArrayStore<uint64_t>{prefix}, total, rest, s);
+  hipLaunchKernelGGL(k_bin_write, dim3(nb), dim3(kBlock), 0, s, n, off, cfg, prefix, nb, listT, listS, listW, hubs,
                      deg_inter);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bin_totals, dim3(1), dim3(64), 0, s, prefix, total, nb, counters);
   KC_LAUNCH_CHECK();
 }
 
@@ -849,9 +969,9 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
   KC_LAUNCH_CHECK();
 }
 
-void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, hipStream_t s) {
+void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, hipStream_t s) {
   if (!b.nH) return;
-  hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, hub_inter, deg_inter);
+  hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, hub_inter, deg_inter, hub_pos);
   KC_LAUNCH_CHECK();
 }
 
@@ -861,65 +981,82 @@ void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s) {
   KC_LAUNCH_CHECK();
 }
 
-void cluster_count(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *lab, uint32_t *cnt,
-                   uint32_t *radius, bool with_radius, hipStream_t s) {
+uint64_t bin_scratch_words(uint64_t n) {
+  const uint64_t nb = ceil_div(n, kBlock) + 1;
+  return 6 * nb + 64 + scan_words_for(4 * nb, 1) + 64;
+}
+
+void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out, hipStream_t s) {
+  if (!n_huge) return;
+  hipLaunchKernelGGL(k_huge_sizes, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s, n_huge,
+                     huge, seg_off, out);
+  KC_LAUNCH_CHECK();
+}
+
+void labels_radius(uint64_t n, const uint64_t *key, uint32_t *lab, uint32_t *radius, bool with_radius, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_cluster_count, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, deg_inter, lab, cnt, radius,
-                     with_radius ? 1 : 0);
+  hipLaunchKernelGGL(k_labels_radius, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, lab, radius, with_radius ? 1 : 0);
   KC_LAUNCH_CHECK();
 }
 
 uint64_t scan_scratch_words(uint64_t n) { return scan_words_for(n, 2) + 64; }
 
-void kept_scan(uint64_t n, const uint32_t *cnt, const uint32_t *gid, uint32_t *newidx, uint64_t *bucket_off,
-               uint32_t *gid_next, uint32_t *old_of, uint64_t *sizes, uint64_t *scratch, hipStream_t s) {
+void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *sk, uint32_t *sv,
+                  uint64_t *count, uint64_t *scratch, hipStream_t s) {
+  scan_exclusive<uint64_t>(n, SelectLoad{deg_inter}, SelectStore{key, sk, sv}, count, scratch, s);
+}
+
+void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t *deg_inter, const uint32_t *gid,
+              uint32_t *pos_of, uint32_t *newidx, uint32_t *old_of, uint32_t *gid_next, uint64_t *seg_off,
+              uint64_t *sizes, uint64_t *scratch, hipStream_t s) {
   Pair64 *total = reinterpret_cast<Pair64 *>(scratch);
-  scan_exclusive<Pair64>(n, KeptLoad{cnt}, KeptStore{gid, newidx, bucket_off, gid_next, old_of}, total, scratch + 4, s);
-  hipLaunchKernelGGL(k_pair_to_sizes, dim3(1), dim3(1), 0, s, total, sizes);
+  scan_exclusive<Pair64>(N, SegLoad{sk, sv, deg_inter}, SegStore{sk, sv, gid, pos_of, newidx, old_of, gid_next, seg_off},
+                         total, scratch + 4, s);
+  hipLaunchKernelGGL(k_seg_finish, dim3(1), dim3(1), 0, s, total, seg_off, sizes);
   KC_LAUNCH_CHECK();
 }
 
+void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, uint64_t *total, uint64_t *scratch,
+                               hipStream_t s) {
+  scan_exclusive<uint64_t>(n, ArrLoadU32{in}, ArrayStore<uint64_t>{out}, total, scratch, s);
+}
+
 void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, const uint32_t *deg_inter,
-             const uint32_t *hub_inter, const uint32_t *newidx, const uint64_t *bucket_off, uint32_t *fill,
+             const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *newidx, const uint32_t *pos_of,
              uint32_t *bucket, hipStream_t s) {
   if (b.blocks == 0) return;
-  ScatterArgs a{off, col, key, b, deg_inter, hub_inter, newidx, bucket_off, fill, bucket};
+  ScatterArgs a{off, col, key, b, deg_inter, hub_inter, hub_pos, newidx, pos_of, bucket};
   hipLaunchKernelGGL(k_scatter, dim3(b.blocks), dim3(kBlock), 0, s, a);
   KC_LAUNCH_CHECK();
 }
 
-void dedup_small(uint64_t n_next, const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off,
-                 uint32_t *bucket, uint32_t *uniq, DedupLists l, hipStream_t s) {
+void dedup_small(uint64_t n_next, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq, DedupLists l,
+                 hipStream_t s) {
   if (!n_next) return;
   uint64_t blocks = ceil_div(n_next, kBlock / kWave);
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_dedup_small, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n_next, old_of, cnt,
-                     bucket_off, bucket, uniq, l);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_dedup_small, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n_next, seg_off, bucket,
+                     uniq, l);
   KC_LAUNCH_CHECK();
 }
 
-void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint32_t *old_of, const uint32_t *cnt,
-                  const uint64_t *bucket_off, uint32_t *bucket, uint32_t *uniq, hipStream_t s) {
+void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
+                  hipStream_t s) {
   if (!n_medium) return;
-  unsigned blocks = n_medium > 1024 ? 1024 : n_medium;
-  hipLaunchKernelGGL(k_dedup_medium, dim3(blocks), dim3(kBlock), 0, s, n_medium, medium, old_of, cnt, bucket_off,
-                     bucket, uniq);
+  unsigned blocks = n_medium > 2048 ? 2048 : n_medium;
+  hipLaunchKernelGGL(k_dedup_medium, dim3(blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off, bucket, uniq);
   KC_LAUNCH_CHECK();
 }
 
-void dedup_huge(uint32_t n_huge, const uint32_t *huge, const uint64_t *huge_tab, uint32_t *tables,
-                const uint32_t *old_of, const uint32_t *cnt, const uint64_t *bucket_off, uint32_t *bucket,
-                uint32_t *uniq, uint32_t *hcount, hipStream_t s) {
-  if (!n_huge) return;
-  dim3 g(64, n_huge < 4096 ? n_huge : 4096);
-  hipLaunchKernelGGL(k_dedup_huge_insert, g, dim3(kBlock), 0, s, n_huge, huge, huge_tab, tables, old_of, cnt,
-                     bucket_off, bucket);
+void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,
+                uint32_t *tables, uint32_t *bucket, uint32_t *uniq, uint32_t *hcount, hipStream_t s) {
+  if (!n_elem) return;
+  const unsigned ge = n_elem < 8192 ? n_elem : 8192, gs = n_slot < 8192 ? n_slot : 8192;
+  hipLaunchKernelGGL(k_dedup_huge_insert, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, tables, bucket);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dedup_huge_emit, g, dim3(kBlock), 0, s, n_huge, huge, huge_tab, tables, old_of, cnt, bucket_off,
-                     bucket, hcount);
+  hipLaunchKernelGGL(k_dedup_huge_emit, dim3(gs), dim3(kBlock), 0, s, slot_chunks, n_slot, tables, bucket, hcount);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dedup_huge_tail, g, dim3(kBlock), 0, s, n_huge, huge, old_of, cnt, bucket_off, bucket, uniq,
-                     hcount);
+  hipLaunchKernelGGL(k_dedup_huge_tail, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, bucket, uniq, hcount);
   KC_LAUNCH_CHECK();
 }
 
@@ -971,15 +1108,20 @@ void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s) 
 }
 
 void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels, uint32_t *roots,
-                 hipStream_t s) {
+                 uint32_t *partial, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_canon_label, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, minid, labels, roots);
+  hipLaunchKernelGGL(k_canon_label, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, minid, labels, partial);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(grid_for(n)), roots);
   KC_LAUNCH_CHECK();
 }
 
-void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, hipStream_t s) {
+void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, uint32_t *partial,
+                      hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_labels_from_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, labels, roots);
+  hipLaunchKernelGGL(k_labels_from_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, labels, partial);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(grid_for(n)), roots);
   KC_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,119 @@
+// Stable LSD radix sort of (u32 key, u32 value) pairs, 8-bit digits.
+//
+// Per pass: (1) every tile of 4096 items builds its digit histogram in LDS;
+// (2) an exclusive scan over the digit-major histogram table gives each
+// (digit, tile) its output offset; (3) every tile ranks its items in index
+// order -- peers with the same digit inside a wave are found with 8 ballots,
+// waves are ordered through LDS counts -- and writes them out.  No global
+// atomics; traffic per pass is 8 B read twice and 8 B written per item.
+#include <hip/hip_runtime.h>
+
+#include "kc_common.h"
+#include "kernels.h"
+
+namespace kc {
+
+namespace {
+
+constexpr int kRadixBits = 8;
+constexpr int kDigits = 1 << kRadixBits;
+constexpr int kItemsPerThread = 16;
+constexpr int kTile = kBlock * kItemsPerThread;
+constexpr int kWaves = kBlock / kWave;
+
+__global__ __launch_bounds__(kBlock) void k_radix_hist(uint64_t n, const uint32_t *keys, int shift, uint32_t *hist,
+                                                      uint32_t tiles) {
+  __shared__ uint32_t cnt[kDigits];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
+  for (int r = 0; r < kItemsPerThread; ++r) {
+    const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kDigits - 1)], 1u);
+  }
+  __syncthreads();
+  hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(uint64_t n, const uint32_t *keys, const uint32_t *vals,
+                                                         int shift, const uint64_t *offsets, uint32_t tiles,
+                                                         uint32_t *okeys, uint32_t *ovals) {
+  __shared__ uint64_t digit_base[kDigits];
+  __shared__ uint32_t wave_cnt[kWaves][kDigits];
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  const uint32_t lane = lane_id();
+  digit_base[tid] = offsets[static_cast<uint64_t>(tid) * tiles + blockIdx.x];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
+  for (int r = 0; r < kItemsPerThread; ++r) {
+    for (int d = lane; d < kDigits; d += kWave) wave_cnt[w][d] = 0;
+    __syncthreads();
+    const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + tid;
+    const bool valid = i < n;
+    const uint32_t k = valid ? keys[i] : 0u;
+    const uint32_t v = valid ? vals[i] : 0u;
+    const uint32_t d = (k >> shift) & (kDigits - 1);
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lanes_below()));
+    if (valid && rank == 0) wave_cnt[w][d] = static_cast<uint32_t>(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint64_t off = digit_base[d] + rank;
+      for (int w2 = 0; w2 < w; ++w2) off += wave_cnt[w2][d];
+      okeys[off] = k;
+      ovals[off] = v;
+    }
+    __syncthreads();
+    digit_base[tid] += wave_cnt[0][tid] + wave_cnt[1][tid] + wave_cnt[2][tid] + wave_cnt[3][tid];
+    __syncthreads();
+  }
+}
+
+template <class T>
+struct ArrLoad {
+  const T *p;
+  __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
+};
+struct ArrStore64 {
+  uint64_t *p;
+  __device__ void operator()(uint64_t i, uint64_t prefix, uint64_t) const { p[i] = prefix; }
+};
+
+}  // namespace
+
+uint64_t radix_sort_scratch_words(uint64_t n) {
+  const uint64_t tiles = ceil_div(n, kTile) + 1;
+  return tiles * kDigits * 2 + scan_scratch_words(tiles * kDigits) + 64;
+}
+
+// Sorts pairs by the low `bits` bits of the key.  Returns the buffer pair
+// (0: keys/vals, 1: tkeys/tvals) that holds the result.
+int radix_sort_pairs(uint64_t n, int bits, uint32_t *keys, uint32_t *vals, uint32_t *tkeys, uint32_t *tvals,
+                     uint64_t *scratch, hipStream_t s) {
+  if (n <= 1 || bits <= 0) return 0;
+  const uint32_t tiles = static_cast<uint32_t>(ceil_div(n, kTile));
+  const uint64_t cells = static_cast<uint64_t>(tiles) * kDigits;
+  uint32_t *hist = reinterpret_cast<uint32_t *>(scratch);
+  uint64_t *offs = scratch + ceil_div(cells, 2) + 8;
+  uint64_t *total = offs + cells + 8;
+  uint64_t *rest = total + 8;
+  int cur = 0;
+  for (int shift = 0; shift < bits; shift += kRadixBits) {
+    uint32_t *ik = cur ? tkeys : keys, *iv = cur ? tvals : vals;
+    uint32_t *ok = cur ? keys : tkeys, *ov = cur ? vals : tvals;
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, s, n, ik, shift, hist, tiles);
+    KC_LAUNCH_CHECK();
+    scan_exclusive_u32_to_u64(cells, hist, offs, total, rest, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, s, n, ik, iv, shift, offs, tiles, ok, ov);
+    KC_LAUNCH_CHECK();
+    cur ^= 1;
+  }
+  return cur;
+}
+
+}  // namespace kc
