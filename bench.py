@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--stream", choices=["hashed", "reference"], default="hashed")
+    ap.add_argument("--schedule", choices=["rounds", "sweeps"], default="rounds",
+                    help="rounds: time-stepped push (each edge pushed once); sweeps: in-place pull sweeps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the label check")
     ap.add_argument("--cpu-scale", type=int, default=21, help="R-MAT scale of the bounded CPU sample")
     ap.add_argument("--tmax", type=int, default=0)
@@ -60,7 +62,8 @@ def main():
     gen_s = time.time() - t0
     n, m = dg.n, dg.m
     seq_limit = max(m // 256, 4096)  # ComputeSequentialLimit with one PE per compute unit
-    run_kw = dict(variant="exp", beta=0.1, seq_limit=seq_limit, stream=stream, tmax=a.tmax, smax=a.smax, wmax=a.wmax)
+    run_kw = dict(variant="exp", beta=0.1, seq_limit=seq_limit, stream=stream, tmax=a.tmax, smax=a.smax, wmax=a.wmax,
+                  use_bfs=a.schedule == "rounds")
     print(f"[bench] rmat scale {a.scale}: n={n} m={m} non_isolated={dg.non_isolated} built in {gen_s:.1f}s",
           file=sys.stderr, flush=True)
 
@@ -108,7 +111,8 @@ def main():
         "data": "synthetic (device-generated R-MAT, counter-based SplitMix64 streams)",
         "config": {"workload": f"rmat-scale{a.scale}-ef{a.edge_factor}-abcd(0.57,0.19,0.19,0.05)-seed3",
                    "n": n, "m": m, "non_isolated": dg.non_isolated, "form": "plain",
-                   "deviate_stream": a.stream, "beta": 0.1, "seq_limit": seq_limit, "parallelism": "1 GPU"},
+                   "deviate_stream": a.stream, "schedule": a.schedule, "beta": 0.1, "seq_limit": seq_limit,
+                   "parallelism": "1 GPU"},
         "levels": stats[-1]["levels"],
         "sweeps": stats[-1]["sweeps"],
         "radius_sum": stats[-1]["radius_sum"],

@@ -448,6 +448,45 @@ void dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *
   sync(g);
 }
 
+// Time-stepped schedule until no vertex lies above the current round, then
+// one counting pass for the cluster-edge counts.  Returns the rounds that did
+// work (the reported "sweeps" of this schedule).
+uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
+                            uint64_t *key, const Bins &b, uint32_t &flag_base) {
+  uint64_t done = 0;
+  uint32_t t = 0;
+  const uint32_t batch = 4;
+  while (true) {
+    if (flag_base + batch + 2 >= kMaxFlags) {
+      fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
+      flag_base = 0;
+    }
+    int slots[8];
+    for (uint32_t i = 0; i < batch; ++i) {
+      const uint32_t f = flag_base + i;
+      const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
+      slots[i] = g->timer.start(KC_K_SWEEP);
+      push_round(off, col, key, kOneHop, b, t + i, &g->flags[f], prev, g->stream);
+      g->timer.stop(slots[i], 8 * (n + 1) + 8 * n);
+    }
+    d2h(g, g->h_flags, &g->flags[flag_base], batch * sizeof(int));
+    sync(g);
+    uint32_t i = 0;
+    while (i < batch && g->h_flags[i] != 0) ++i;
+    const uint32_t worked = i < batch ? i + 1 : batch;
+    for (uint32_t j = worked; j < batch; ++j) g->timer.retag(slots[j], KC_K_OTHER, 0);
+    done += worked;
+    flag_base += batch;
+    t += batch;
+    if (i < batch) break;
+  }
+  // every edge was pushed once: its key read and (when smaller) its atomicMin
+  {
+    TimedScope ts(g->timer, KC_K_SWEEP, 12 * m2);
+  }
+  return done;
+}
+
 struct RunOut {
   uint64_t sweeps = 0, radius_sum = 0;
   uint32_t levels = 0;
@@ -526,16 +565,13 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         sweep(kSweepJacobi, false, off_l, col_l, kin, kout, 0, b, g->deg_inter, g->hub_inter,
               &g->flags[kMaxFlags - 1], nullptr, g->stream);
       }
-      {
-        TimedScope ts(g->timer, KC_K_COUNT, sweep_bytes(n_l, m2_l, true) - 8 * n_l);
-        sweep(kSweepCountOnly, true, off_l, col_l, g->key, g->key, 0, b, g->deg_inter, g->hub_inter,
-              &g->flags[kMaxFlags - 1], nullptr, g->stream);
-      }
       sweeps = 2;
+    } else if (o->use_bfs) {
+      sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, flag_base);
     } else {
-      sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, true, false, flag_base);
+      sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
     }
-    if (!local && gid_l && sweeps == 1) {
+    if (!local && !o->use_bfs && gid_l && sweeps == 1) {
       // Stall: the first sweep changed no key, so every vertex is its own
       // centre and keeps an edge; the contracted graph equals this one.  The
       // level is recorded and the next iteration draws new deviates on the
@@ -554,6 +590,13 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       g->levels.push_back(rec);
       ++iteration;
       continue;
+    }
+    // relabel: every edge slot gets the label of its far end (kNone inside a
+    // cluster); the slots live in the next level's column buffer until it is built
+    uint32_t *relab = g->col_pp[pp];
+    {
+      TimedScope ts(g->timer, KC_K_RELABEL, 8 * (n_l + 1) + 4 * m2_l + 8 * m2_l + 8 * n_l + 4 * m2_l + 4 * n_l);
+      relabel(off_l, col_l, g->key, b, relab, g->deg_inter, g->hub_inter, g->stream);
     }
     if (b.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
@@ -609,11 +652,10 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     g->levels.push_back(rec);
     if (n_next == 0) break;  // every cluster is a whole component
 
-    // relabel: every vertex writes the next-level labels of its cluster edges at its position
+    // gather: every row copies its cluster edges to its position in its cluster's bucket
     {
-      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 12 * m2_l + 8 * n_l + 8 * P);
-      scatter(off_l, col_l, g->key, b, g->deg_inter, g->hub_inter, g->hub_pos, rec.newidx, g->pos_of, g->bucket,
-              g->stream);
+      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l + 4 * P);
+      scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, g->bucket, g->stream);
     }
     dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab);
     // next-level CSR
@@ -628,7 +670,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     const uint64_t m2_next = g->h_sizes[2];
     {
       TimedScope ts(g->timer, KC_K_CSR, 4 * P + 4 * m2_next);
-      compact(P, g->bucket, col_next, g->scan_scratch, g->stream);
+      compact(P, g->bucket, col_next, rec.newidx, g->scan_scratch, g->stream);
     }
     n_l = n_next;
     m2_l = m2_next;
@@ -805,7 +847,7 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   KC_HIP(hipStreamSynchronize(s));
   g->m2 = g->h_sizes[2];
   g->col0 = g->dalloc<uint32_t>(g->m2);
-  compact(P, bucket.as<uint32_t>(), g->col0, scratch.as<uint64_t>(), s);
+  compact(P, bucket.as<uint32_t>(), g->col0, nullptr, scratch.as<uint64_t>(), s);
   unsigned long long *cnt = reinterpret_cast<unsigned long long *>(g->sizes + 6);
   fill_u64(g->sizes + 6, 1, 0, s);
   nonzero_rows(n, g->off0, cnt, s);

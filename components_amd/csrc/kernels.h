@@ -75,6 +75,11 @@ uint64_t bin_scratch_words(uint64_t n);
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin,
            uint64_t *kout, uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter,
            int *flag, const int *prev_flag, hipStream_t s);
+// Round t of the time-stepped schedule: vertices at key distance t push
+// key + hop; *pending = 1 while a vertex is still above distance t;
+// prev_pending (may be null) reading 0 makes the launch return at once.
+void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
+                int *pending, const int *prev_pending, hipStream_t s);
 // deg_inter of hubs from their chunk counts; hub_pos[c] = offset of chunk c in its hub's run.
 void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, hipStream_t s);
 
@@ -95,9 +100,14 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
               uint32_t *pos_of, uint32_t *newidx, uint32_t *old_of, uint32_t *gid_next, uint64_t *seg_off,
               uint64_t *sizes, uint64_t *scratch, hipStream_t s);
 
-void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b,
-             const uint32_t *deg_inter, const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *newidx,
-             const uint32_t *pos_of, uint32_t *bucket, hipStream_t s);
+// relab[e] = label of col[e] when it differs from row v's label, else kNone;
+// deg_inter[v] (hub chunks: hub_inter) = number of such edges.
+void relabel(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, uint32_t *relab,
+             uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s);
+// Copies the non-kNone relab slots of every row to bucket[pos_of[v] + ...].
+void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,
+             const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *pos_of, uint32_t *bucket,
+             hipStream_t s);
 
 // Dedup of every bucket a2 (elements [seg_off[a2], seg_off[a2+1])): small ones
 // in wave LDS tables; medium and huge ones are listed (counters zeroed by the
@@ -133,8 +143,10 @@ void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *
 // off_next = exclusive scan of uniq (n_next+1 entries); sizes[2] = total.
 void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint64_t *sizes,
                  uint64_t *scratch, hipStream_t s);
-// Stream compaction of bucket[0..total) dropping kNone into col_next.
-void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, uint64_t *scratch, hipStream_t s);
+// Stream compaction of bucket[0..total) dropping kNone into col_next, each
+// value mapped through map (identity when null).
+void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, const uint32_t *map, uint64_t *scratch,
+             hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
 void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, uint64_t *total, uint64_t *scratch,
                                hipStream_t s);

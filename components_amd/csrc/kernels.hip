@@ -274,6 +274,115 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
 
 // Sums the chunk counts of every hub into deg_inter and records each chunk's
 // offset inside its hub's run of cluster edges.
+// ---------------------------------------------------------------------------
+// Time-stepped schedule of the same fixed point (RunContractionBFS order with
+// the label tie-break): in round t every vertex whose key distance is t --
+// final, since only distance t+1 candidates are pushed in round t -- pushes
+// key + hop to its neighbours (read first, atomicMin when smaller).  Every
+// edge is pushed once per level.  pending[t] is set while some vertex still
+// has a distance above t; a round returns at once when the previous round
+// found none.
+template <int W>
+__device__ __forceinline__ bool round_group(const SweepArgs &a, bool active, uint32_t v, uint64_t e0, uint64_t e1,
+                                            int gl, uint32_t t) {
+  const uint64_t kv = active ? a.kin[v] : ~0ull;
+  const uint32_t dist = static_cast<uint32_t>(kv >> 32);
+  if (active && dist == t) {
+    const uint64_t cand = kv + a.hop;
+    for (uint64_t e = e0 + gl; e < e1; e += W) {
+      uint64_t *kw = a.kout + a.col[e];
+      if (cand < *kw) atomicMin(reinterpret_cast<unsigned long long *>(kw), static_cast<unsigned long long>(cand));
+    }
+  }
+  return active && dist > t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
+  if (a.prev_flag && *a.prev_flag == 0) return;
+  const uint32_t blk = blockIdx.x;
+  const int tid = threadIdx.x;
+  bool pending = false;
+  if (blk < a.b.bS) {
+    const uint32_t i = blk * kBlock + tid;
+    const bool act = i < a.b.nT;
+    uint32_t v = act ? a.b.listT[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    pending = round_group<1>(a, act, v, e0, e1, 0, t);
+  } else if (blk < a.b.bW) {
+    const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
+    const bool act = i < a.b.nS;
+    uint32_t v = act ? a.b.listS[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    pending = round_group<16>(a, act, v, e0, e1, tid & 15, t);
+  } else if (blk < a.b.bH) {
+    const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nW;
+    uint32_t v = act ? a.b.listW[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    pending = round_group<kWave>(a, act, v, e0, e1, tid & 63, t);
+  } else {
+    const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nH;
+    HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
+    pending = round_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, t);
+  }
+  if (__ballot(pending) != 0 && lane_id() == 0) *a.flag = 1;
+}
+
+// Relabel pass: relab[e] = label of col[e] when it differs from the row's
+// label, kNone otherwise; writes are coalesced (the CSR layout).  The row's
+// count of cluster edges goes to deg_inter (hub chunks: hub_inter).
+template <int W>
+__device__ __forceinline__ void relabel_group(const SweepArgs &a, uint32_t *relab, bool active, uint32_t v,
+                                              uint64_t e0, uint64_t e1, int gl, bool hub, uint32_t chunk) {
+  uint32_t cnt = 0;
+  if (active) {
+    const uint32_t lv = static_cast<uint32_t>(a.kin[v]);
+    for (uint64_t e = e0 + gl; e < e1; e += W) {
+      const uint32_t lw = static_cast<uint32_t>(a.kin[a.col[e]]);
+      const bool inter = lw != lv;
+      relab[e] = inter ? lw : kNone;
+      cnt += inter;
+    }
+  }
+  if (W > 1) cnt = group_sum_u32(cnt, W);
+  if (active && gl == 0) {
+    if (hub)
+      a.hub_inter[chunk] = cnt;
+    else
+      a.deg_inter[v] = cnt;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_relabel(SweepArgs a, uint32_t *relab) {
+  const uint32_t blk = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (blk < a.b.bS) {
+    const uint32_t i = blk * kBlock + tid;
+    const bool act = i < a.b.nT;
+    uint32_t v = act ? a.b.listT[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    relabel_group<1>(a, relab, act, v, e0, e1, 0, false, 0);
+  } else if (blk < a.b.bW) {
+    const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
+    const bool act = i < a.b.nS;
+    uint32_t v = act ? a.b.listS[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    relabel_group<16>(a, relab, act, v, e0, e1, tid & 15, false, 0);
+  } else if (blk < a.b.bH) {
+    const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nW;
+    uint32_t v = act ? a.b.listW[i] : 0;
+    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
+    relabel_group<kWave>(a, relab, act, v, e0, e1, tid & 63, false, 0);
+  } else {
+    const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nH;
+    HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
+    relabel_group<kWave>(a, relab, act, h.v, h.e0, h.e1, tid & 63, true, i);
+  }
+}
+
 __global__ void k_hub_fold(HubChunk *hubs, uint32_t nH, const uint32_t *hub_inter, uint32_t *deg_inter,
                            uint32_t *hub_pos) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,15 +604,16 @@ __global__ void k_copy_total(const uint64_t *t, uint64_t *dst) { *dst = *t; }
 
 // ---------------------------------------------------------------------------
 // Scatter of cluster edges into buckets (binned like the sweep).
+// Gather of every row's cluster edges (the non-kNone relabel slots) into the
+// bucket of its cluster, at the row's position: sequential reads of the row,
+// contiguous writes.
 struct ScatterArgs {
   const uint64_t *off;
-  const uint32_t *col;
-  const uint64_t *key;
+  const uint32_t *relab;
   Bins b;
   const uint32_t *deg_inter;
   const uint32_t *hub_inter;
   const uint32_t *hub_pos;
-  const uint32_t *newidx;
   const uint32_t *pos_of;
   uint32_t *bucket;
 };
@@ -513,19 +623,15 @@ __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active,
                                               uint64_t e1, int gl, uint32_t d, uint32_t extra) {
   // group base lane within the wave
   const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
-  uint32_t lv = 0;
   uint64_t pos = 0;
   active = active && d > 0;
-  if (active) {
-    lv = static_cast<uint32_t>(a.key[v]);
-    if (gl == 0) pos = static_cast<uint64_t>(a.pos_of[v]) + extra;
-  }
+  if (active && gl == 0) pos = static_cast<uint64_t>(a.pos_of[v]) + extra;
   if (W > 1) pos = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), gbase, 64));
   if (W == 1) {
     if (!active) return;
     for (uint64_t e = e0; e < e1; ++e) {
-      const uint32_t lw = static_cast<uint32_t>(a.key[a.col[e]]);
-      if (lw != lv) a.bucket[pos++] = a.newidx[lw];
+      const uint32_t lw = a.relab[e];
+      if (lw != kNone) a.bucket[pos++] = lw;
     }
     return;
   }
@@ -540,13 +646,13 @@ __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active,
   for (uint64_t r = 0; r < rounds; ++r) {
     const uint64_t e = e0 + r * W + gl;
     bool pred = false;
-    uint32_t lw = 0;
+    uint32_t lw = kNone;
     if (active && e < e1) {
-      lw = static_cast<uint32_t>(a.key[a.col[e]]);
-      pred = lw != lv;
+      lw = a.relab[e];
+      pred = lw != kNone;
     }
     const uint64_t m = __ballot(pred) & gmask;
-    if (pred) a.bucket[pos + __popcll(m & lanes_below())] = a.newidx[lw];
+    if (pred) a.bucket[pos + __popcll(m & lanes_below())] = lw;
     pos += __popcll(m);
   }
 }
@@ -801,7 +907,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(uint64_t total, const 
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_write(uint64_t total, const uint32_t *src, uint32_t *dst,
-                                                         const uint64_t *prefix) {
+                                                         const uint64_t *prefix, const uint32_t *map) {
   __shared__ uint64_t smem[kBlock];
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
   uint64_t out = prefix[blockIdx.x];
@@ -811,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(uint64_t total, const 
     const uint64_t keep = val != kNone;
     uint64_t tot;
     uint64_t p = block_exclusive_scan_shift<uint64_t>(keep, smem, &tot);
-    if (keep) dst[out + p] = val;
+    if (keep) dst[out + p] = map ? map[val] : val;
     out += tot;
   }
 }
@@ -969,6 +1075,14 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
   KC_LAUNCH_CHECK();
 }
 
+void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
+                int *pending, const int *prev_pending, hipStream_t s) {
+  if (b.blocks == 0) return;
+  SweepArgs a{off, col, key, key, hop, b, nullptr, nullptr, pending, prev_pending};
+  hipLaunchKernelGGL(k_round, dim3(b.blocks), dim3(kBlock), 0, s, a, t);
+  KC_LAUNCH_CHECK();
+}
+
 void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, hipStream_t s) {
   if (!b.nH) return;
   hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, hub_inter, deg_inter, hub_pos);
@@ -1021,11 +1135,19 @@ void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, ui
   scan_exclusive<uint64_t>(n, ArrLoadU32{in}, ArrayStore<uint64_t>{out}, total, scratch, s);
 }
 
-void scatter(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, const uint32_t *deg_inter,
-             const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *newidx, const uint32_t *pos_of,
-             uint32_t *bucket, hipStream_t s) {
+void relabel(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, uint32_t *relab,
+             uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s) {
   if (b.blocks == 0) return;
-  ScatterArgs a{off, col, key, b, deg_inter, hub_inter, hub_pos, newidx, pos_of, bucket};
+  SweepArgs a{off, col, key, nullptr, 0, b, deg_inter, hub_inter, nullptr, nullptr};
+  hipLaunchKernelGGL(k_relabel, dim3(b.blocks), dim3(kBlock), 0, s, a, relab);
+  KC_LAUNCH_CHECK();
+}
+
+void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,
+             const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *pos_of, uint32_t *bucket,
+             hipStream_t s) {
+  if (b.blocks == 0) return;
+  ScatterArgs a{off, relab, b, deg_inter, hub_inter, hub_pos, pos_of, bucket};
   hipLaunchKernelGGL(k_scatter, dim3(b.blocks), dim3(kBlock), 0, s, a);
   KC_LAUNCH_CHECK();
 }
@@ -1073,7 +1195,8 @@ void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint
   KC_LAUNCH_CHECK();
 }
 
-void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, uint64_t *scratch, hipStream_t s) {
+void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, const uint32_t *map, uint64_t *scratch,
+             hipStream_t s) {
   if (!total) return;
   const uint64_t blocks = ceil_div(total, kScanTile);
   uint64_t *counts = scratch;
@@ -1084,7 +1207,7 @@ void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, uint64_
   KC_LAUNCH_CHECK();
   scan_exclusive<uint64_t>(blocks, ArrayLoad<uint64_t>{counts}, ArrayStore<uint64_t>{prefix}, tot, rest, s);
   hipLaunchKernelGGL(k_compact_write, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, total, bucket, col_next,
-                     prefix);
+                     prefix, map);
   KC_LAUNCH_CHECK();
 }
 

@@ -124,14 +124,14 @@ typedef struct kc_options {
 #define KC_STREAM_REFERENCE 0
 #define KC_STREAM_HASHED 1
 
-#define KC_KERNEL_CLASSES 13
+#define KC_KERNEL_CLASSES 14
 /* Kernel classes accounted in kc_stats.kernel_* (index = class). */
 #define KC_K_DEVIATES 0  /* deviate sampling + key init */
 #define KC_K_SWEEP 1     /* hooking sweep (decomposition) */
 #define KC_K_BIN 2       /* degree binning */
 #define KC_K_COUNT 3     /* cluster labels, radius, hub chunk folding */
 #define KC_K_SCAN 4      /* prefix sums: selection and cluster segments */
-#define KC_K_SCATTER 5   /* relabel: cluster edges written at their segment positions */
+#define KC_K_SCATTER 5   /* gather of cluster edges into their cluster's bucket */
 #define KC_K_DEDUP 6     /* LDS-staged bucket dedup */
 #define KC_K_CSR 7       /* next-level CSR compaction */
 #define KC_K_UNDO 8      /* undo gather */
@@ -139,6 +139,7 @@ typedef struct kc_options {
 #define KC_K_JUMP 10     /* pointer jumping */
 #define KC_K_OTHER 11
 #define KC_K_SORT 12     /* radix sort of (label, vertex) pairs */
+#define KC_K_RELABEL 13  /* edge slots relabelled with their far end's cluster */
 
 typedef struct kc_stats {
   double time_ms;       /* device time of the whole call (HIP events), labels resident on device */

@@ -94,9 +94,10 @@ def test_sampled_graphs_6_7_vertices():
     check_labels(union(graphs), seed=1002, seq_limit=0, stream=ca.STREAM_HASHED)
 
 
+@pytest.mark.parametrize("bfs", [False, True])
 @pytest.mark.parametrize("stream", [ca.STREAM_REFERENCE, ca.STREAM_HASHED])
 @pytest.mark.parametrize("case", ["gnm", "grid2", "grid3", "rmat", "rgg", "path", "star"])
-def test_level_trace_matches_oracle(case, stream):
+def test_level_trace_matches_oracle(case, stream, bfs):
     rng = np.random.default_rng(abs(hash(case)) % 1000)
     if case == "gnm":
         g = ca.generate("gnm_undirected", n=1 << 14, m=1 << 15, seed=3)
@@ -115,7 +116,8 @@ def test_level_trace_matches_oracle(case, stream):
     seq = 64
     run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1005, seq_limit=seq, keep_keys=True, stream=stream)
     dg = ca.DeviceGraph(g)
-    lab, st = dg.find_components(variant="exp", seed=1005, seq_limit=seq, record_trace=True, stream=stream)
+    lab, st = dg.find_components(variant="exp", seed=1005, seq_limit=seq, record_trace=True, stream=stream,
+                                 use_bfs=bfs)
     tr = dg.trace()
     dg.close()
     want_levels = run.levels()
@@ -167,6 +169,7 @@ def test_edge_cases(case):
     for stream in (ca.STREAM_REFERENCE, ca.STREAM_HASHED):
         check_labels(g, seed=1001, stream=stream)
     check_labels(g, variants=["exp"], seed=1001, seq_limit=0, stream=ca.STREAM_HASHED)
+    check_labels(g, variants=["exp"], seed=1001, seq_limit=0, stream=ca.STREAM_HASHED, use_bfs=True)
 
 
 @pytest.mark.parametrize("wmax,smax,tmax", [(256, 64, 4), (4096, 256, 16), (1 << 16, 1024, 32)])
@@ -190,8 +193,26 @@ def test_config0_full_size():
     ref = oracle.canonical_labels(g.offsets, g.targets)
     dg = ca.DeviceGraph(g)
     for stream in (ca.STREAM_REFERENCE, ca.STREAM_HASHED):
-        lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=16384, stream=stream)
+        for bfs in (False, True):
+            lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=16384, stream=stream, use_bfs=bfs)
+            assert np.array_equal(lab, ref)
+    dg.close()
+
+
+def test_rmat_scale22_schedules_agree():
+    dg = ca.DeviceGraph.generate("rmat", n=1 << 22, m=1 << 26, seed=3)
+    g = dg.download()
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    traces = []
+    for bfs in (False, True):
+        lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
+                                     use_bfs=bfs, record_trace=True)
         assert np.array_equal(lab, ref)
+        traces.append(dg.trace())
+    assert len(traces[0]) == len(traces[1])
+    for (i0, k0, g0), (i1, k1, g1) in zip(*traces):
+        assert i0[0] == i1[0] and i0[1] == i1[1] and i0[3] == i1[3]
+        assert np.array_equal(k0, k1) and np.array_equal(g0, g1)
     dg.close()
 
 
