@@ -191,13 +191,22 @@ struct kc_graph {
   int device = 0;
   hipStream_t stream = nullptr;
   uint64_t n = 0, m2 = 0, non_isolated = 0;
-  // level-0 CSR
+  // input CSR
   uint64_t *off0 = nullptr;
   uint32_t *col0 = nullptr;
+  // level 0 of the contraction: the n0 vertices of degree > 0 in id order,
+  // gid0 their input ids (null when no vertex is isolated: then off0c/col0c
+  // are off0/col0)
+  uint64_t n0 = 0;
+  uint32_t *gid0 = nullptr;
+  uint64_t *off0c = nullptr;
+  uint32_t *col0c = nullptr;
   // work buffers sized for level 0
   uint64_t *key = nullptr, *key2 = nullptr;
   uint32_t *deg_inter = nullptr, *lists = nullptr, *hub_inter = nullptr;
   HubChunk *hubs = nullptr;
+  uint32_t *flists = nullptr;  // bins of the unsettled vertices (time-stepped schedule)
+  HubChunk *fhubs = nullptr;
   uint64_t hubs_cap = 0;
   uint32_t *bucket = nullptr, *old_of = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
   uint32_t *sk = nullptr, *sv = nullptr, *sk2 = nullptr, *sv2 = nullptr;  // (label, vertex) sort buffers
@@ -309,16 +318,20 @@ BinCfg bin_cfg(const kc_options *o) {
   return c;
 }
 
-// Bins of the level CSR (one sync to size the binned launches).
-Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg) {
+// Bins of the level CSR (one sync to size the binned launches).  With key,
+// only the vertices at key distance >= tmin, into the second set of lists.
+Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg, const uint64_t *key = nullptr,
+               uint32_t tmin = 0) {
   Bins b;
-  b.listT = g->lists;
-  b.listS = g->lists + g->n;
-  b.listW = g->lists + 2 * g->n;
-  b.hubs = g->hubs;
+  uint32_t *lists = key ? g->flists : g->lists;
+  b.listT = lists;
+  b.listS = lists + g->n;
+  b.listW = lists + 2 * g->n;
+  b.hubs = key ? g->fhubs : g->hubs;
   {
-    TimedScope ts(g->timer, KC_K_BIN, 16 * n + 8);
-    bin_vertices(n, off, cfg, b.listT, b.listS, b.listW, b.hubs, g->small, g->deg_inter, g->scan_scratch, g->stream);
+    TimedScope ts(g->timer, KC_K_BIN, (key ? 24 : 16) * n + 8);
+    bin_vertices(n, off, cfg, b.listT, b.listS, b.listW, b.hubs, g->small, key ? nullptr : g->deg_inter,
+                 g->scan_scratch, g->stream, key, tmin);
   }
   d2h(g, g->h_small, g->small, 4 * sizeof(uint32_t));
   sync(g);
@@ -452,11 +465,18 @@ void dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *
 // one counting pass for the cluster-edge counts.  Returns the rounds that did
 // work (the reported "sweeps" of this schedule).
 uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
-                            uint64_t *key, const Bins &b, uint32_t &flag_base) {
+                            uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base) {
   uint64_t done = 0;
   uint32_t t = 0;
   const uint32_t batch = 4;
+  Bins b = level_bins;
+  uint32_t next_rebin = batch;
   while (true) {
+    if (t >= next_rebin) {
+      // drop the settled vertices from the rounds' bins (distances below t)
+      b = make_bins(g, n, off, cfg, key, t);
+      next_rebin *= 2;
+    }
     if (flag_base + batch + 2 >= kMaxFlags) {
       fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
       flag_base = 0;
@@ -496,10 +516,14 @@ struct RunOut {
 void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   const bool local = o->variant == KC_VARIANT_LOCAL;
   const BinCfg cfg = bin_cfg(o);
-  uint64_t n_l = g->n, m2_l = g->m2;
-  const uint64_t *off_l = g->off0;
-  const uint32_t *col_l = g->col0;
-  uint32_t *gid_l = nullptr;
+  // degree-0 vertices are their own components and take no part in the
+  // contraction (the reference sets them inactive at load, graph_io.h:380)
+  identity_u32(g->n, g->labels, g->stream);
+  fill_u32(g->small + 5, 1, 0, g->stream);
+  uint64_t n_l = g->n0, m2_l = g->m2;
+  const uint64_t *off_l = g->off0c;
+  const uint32_t *col_l = g->col0c;
+  uint32_t *gid_l = g->gid0;
   uint32_t iteration = 1;
   uint32_t flag_base = 0;
   int pp = 0;
@@ -567,15 +591,15 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       }
       sweeps = 2;
     } else if (o->use_bfs) {
-      sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, flag_base);
+      sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base);
     } else {
       sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
     }
-    if (!local && !o->use_bfs && gid_l && sweeps == 1) {
+    if (!local && !o->use_bfs && sweeps == 1) {
       // Stall: the first sweep changed no key, so every vertex is its own
-      // centre and keeps an edge; the contracted graph equals this one.  The
-      // level is recorded and the next iteration draws new deviates on the
-      // same graph (level >= 1 graphs have no isolated vertices).
+      // centre and keeps an edge (no level holds an isolated vertex); the
+      // contracted graph equals this one.  The level is recorded and the next
+      // iteration draws new deviates on the same graph.
       LevelRec rec;
       rec.stall = true;
       rec.n = n_l;
@@ -590,17 +614,6 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       g->levels.push_back(rec);
       ++iteration;
       continue;
-    }
-    // relabel: every edge slot gets the label of its far end (kNone inside a
-    // cluster); the slots live in the next level's column buffer until it is built
-    uint32_t *relab = g->col_pp[pp];
-    {
-      TimedScope ts(g->timer, KC_K_RELABEL, 8 * (n_l + 1) + 4 * m2_l + 8 * m2_l + 8 * n_l + 4 * m2_l + 4 * n_l);
-      relabel(off_l, col_l, g->key, b, relab, g->deg_inter, g->hub_inter, g->stream);
-    }
-    if (b.nH) {
-      TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
-      hub_fold(b, g->hub_inter, g->deg_inter, g->hub_pos, g->stream);
     }
     LevelRec rec;
     rec.n = n_l;
@@ -618,6 +631,17 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
       labels_radius(n_l, g->key, rec.lab, g->small + 4, !local, g->stream);
+    }
+    // relabel: every edge slot gets the label of its far end (kNone inside a
+    // cluster); the slots live in the next level's column buffer until it is built
+    uint32_t *relab = g->col_pp[pp];
+    {
+      TimedScope ts(g->timer, KC_K_RELABEL, 8 * (n_l + 1) + 4 * m2_l + 4 * m2_l + 4 * n_l + 4 * m2_l + 4 * n_l);
+      relabel(off_l, col_l, rec.lab, b, relab, g->deg_inter, g->hub_inter, g->stream);
+    }
+    if (b.nH) {
+      TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
+      hub_fold(b, g->hub_inter, g->deg_inter, g->hub_pos, g->stream);
     }
     // clusters as segments of the (label, vertex) list sorted by label
     {
@@ -697,14 +721,16 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     return;
   }
   // canonical labels: minimum vertex id per component
+  const uint64_t n0 = g->n0;
+  const uint64_t gid_bytes = g->gid0 ? 4 * n0 : 0;
   fill_u32(g->minid, g->n, kNone, g->stream);
   {
-    TimedScope ts(g->timer, KC_K_CANON, 8 * g->n);
-    canon_min(g->n, rep_up, g->minid, g->stream);
+    TimedScope ts(g->timer, KC_K_CANON, 8 * n0 + gid_bytes);
+    canon_min(n0, rep_up, g->gid0, g->minid, g->stream);
   }
   {
-    TimedScope ts(g->timer, KC_K_CANON, 12 * g->n);
-    canon_label(g->n, rep_up, g->minid, g->labels, g->small + 5, reinterpret_cast<uint32_t *>(g->scan_scratch),
+    TimedScope ts(g->timer, KC_K_CANON, 12 * n0 + gid_bytes);
+    canon_label(n0, rep_up, g->gid0, g->minid, g->labels, g->small + 5, reinterpret_cast<uint32_t *>(g->scan_scratch),
                 g->stream);
   }
 }
@@ -777,6 +803,8 @@ void init_work(kc_graph *g) {
   g->lists = g->dalloc<uint32_t>(3 * n);
   g->hubs_cap = m2 / (kMinWaveBin / 2) + 2;
   g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
+  g->flists = g->dalloc<uint32_t>(3 * n);
+  g->fhubs = g->dalloc<HubChunk>(g->hubs_cap);
   g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
   g->bucket = g->dalloc<uint32_t>(m2);
   g->old_of = g->dalloc<uint32_t>(n);
@@ -803,6 +831,25 @@ void init_work(kc_graph *g) {
   g->dl_huge_tab = g->dalloc<uint64_t>(n);
   g->hcount = g->dalloc<uint32_t>(n);
   g->flags = g->dalloc<int>(kMaxFlags);
+  // level 0 without the degree-0 vertices (rep_a holds the id map meanwhile)
+  if (g->non_isolated == n) {
+    g->n0 = n;
+    g->gid0 = nullptr;
+    g->off0c = g->off0;
+    g->col0c = g->col0;
+  } else {
+    g->gid0 = g->dalloc<uint32_t>(n);
+    g->off0c = g->dalloc<uint64_t>(n + 1);
+    g->col0c = g->dalloc<uint32_t>(m2);
+    drop_isolated(n, g->off0, g->col0, m2, g->rep_a, g->gid0, g->off0c, g->col0c, g->sizes + 9, g->scan_scratch,
+                  g->stream);
+    d2h(g, g->h_sizes + 9, g->sizes + 9, sizeof(uint64_t));
+    sync(g);
+    g->n0 = g->h_sizes[9];
+    if (g->n0 != g->non_isolated)
+      fail(KC_ERR_INVALID, "internal: %llu vertices of degree > 0, expected %llu", (unsigned long long)g->n0,
+           (unsigned long long)g->non_isolated);
+  }
   KC_HIP(hipDeviceSynchronize());
 }
 
@@ -998,6 +1045,8 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
     d2h(g, g->h_small + 5, g->small + 5, sizeof(uint32_t));
     sync(g);
     g->components = g->n ? g->h_small[5] : 0;
+    if (o.variant == KC_VARIANT_EXP || o.variant == KC_VARIANT_LOCAL)
+      g->components += static_cast<uint32_t>(g->n - g->n0);  // degree-0 vertices
     g->have_labels = true;
     float ms = 0.f;
     KC_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));

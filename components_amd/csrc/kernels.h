@@ -64,9 +64,10 @@ void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, 
 // Bins vertices of a CSR by degree (stable, no global atomics); counters[0..3]
 // receive nT, nS, nW, nH.  deg_inter of degree-0 vertices is cleared.
 // scratch: bin_scratch_words(n) words.
+// With key != null only vertices at key distance >= tmin are binned.
 void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS,
                   uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch,
-                  hipStream_t s);
+                  hipStream_t s, const uint64_t *key = nullptr, uint32_t tmin = 0);
 void finish_bins(Bins &b);  // computes block offsets from the counts
 uint64_t bin_scratch_words(uint64_t n);
 
@@ -100,9 +101,9 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
               uint32_t *pos_of, uint32_t *newidx, uint32_t *old_of, uint32_t *gid_next, uint64_t *seg_off,
               uint64_t *sizes, uint64_t *scratch, hipStream_t s);
 
-// relab[e] = label of col[e] when it differs from row v's label, else kNone;
+// relab[e] = lab[col[e]] when it differs from lab[v], else kNone;
 // deg_inter[v] (hub chunks: hub_inter) = number of such edges.
-void relabel(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, uint32_t *relab,
+void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
              uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s);
 // Copies the non-kNone relab slots of every row to bucket[pos_of[v] + ...].
 void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,
@@ -161,11 +162,18 @@ void undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_
           const uint32_t *rep_up, uint32_t *rep_out, hipStream_t s);
 // rep of a base-case level: rep[v] = gid[low(key[v])].
 void rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_t *rep, hipStream_t s);
-void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s);
+// minid[rep[v]] = min over v of gid[v] (identity gid when null).
+void canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid, uint32_t *minid, hipStream_t s);
 // roots (device) = number of vertices that are their own label; partial needs
 // ceil(n / 256) words.
-void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels,
+// labels[gid[v]] = minid[rep[v]]; roots counts the vertices that are their own label.
+void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid, const uint32_t *minid, uint32_t *labels,
                  uint32_t *roots, uint32_t *partial, hipStream_t s);
+// Level 0 without degree-0 vertices (the reference sets them inactive at
+// load): newid (n), gid0 (kept), off_c (kept + 1), col_c (m2, mapped), *n_out.
+void drop_isolated(uint64_t n, const uint64_t *off, const uint32_t *col, uint64_t m2, uint32_t *newid, uint32_t *gid0,
+                   uint64_t *off_c, uint32_t *col_c, uint64_t *n_out, uint64_t *scratch, hipStream_t s);
+void identity_u32(uint64_t n, uint32_t *p, hipStream_t s);
 void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, uint32_t *partial,
                       hipStream_t s);
 

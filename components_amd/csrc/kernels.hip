@@ -113,11 +113,20 @@ __device__ __forceinline__ void bin_ranks(int bin, uint32_t chunks, uint32_t *ra
   wave_tot[3] = __shfl(incl, 63);
 }
 
+// With key != null only vertices at key distance >= tmin are binned (the
+// unsettled ones of the time-stepped schedule).
+__device__ __forceinline__ uint64_t binned_degree(uint64_t v, uint64_t n, const uint64_t *off, const uint64_t *key,
+                                                  uint32_t tmin) {
+  if (v >= n) return 0;
+  if (key && static_cast<uint32_t>(key[v] >> 32) < tmin) return 0;
+  return off[v + 1] - off[v];
+}
+
 __global__ __launch_bounds__(kBlock) void k_bin_count(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *counts,
-                                                     uint32_t nblocks) {
+                                                     uint32_t nblocks, const uint64_t *key, uint32_t tmin) {
   __shared__ uint32_t tot[kBlock / kWave][4];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-  const uint64_t deg = v < n ? off[v + 1] - off[v] : 0;
+  const uint64_t deg = binned_degree(v, n, off, key, tmin);
   const int bin = bin_of(deg, cfg);
   const uint32_t chunks = bin == 3 ? static_cast<uint32_t>(ceil_div(deg, cfg.wmax)) : 0u;
   uint32_t rank = 0, wt[4];
@@ -134,15 +143,12 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(uint64_t n, const uint64_t
 __global__ __launch_bounds__(kBlock) void k_bin_write(uint64_t n, const uint64_t *off, BinCfg cfg,
                                                      const uint64_t *prefix, uint32_t nblocks, uint32_t *listT,
                                                      uint32_t *listS, uint32_t *listW, HubChunk *hubs,
-                                                     uint32_t *deg_inter) {
+                                                     uint32_t *deg_inter, const uint64_t *key, uint32_t tmin) {
   __shared__ uint32_t tot[kBlock / kWave][4];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
-  uint64_t e0 = 0, deg = 0;
-  if (v < n) {
-    e0 = off[v];
-    deg = off[v + 1] - e0;
-    if (deg == 0 && deg_inter) deg_inter[v] = 0;
-  }
+  const uint64_t deg = binned_degree(v, n, off, key, tmin);
+  const uint64_t e0 = deg ? off[v] : 0;
+  if (v < n && deg == 0 && deg_inter) deg_inter[v] = 0;
   const int bin = bin_of(deg, cfg);
   const uint32_t chunks = bin == 3 ? static_cast<uint32_t>(ceil_div(deg, cfg.wmax)) : 0u;
   uint32_t rank = 0, wt[4];
@@ -284,10 +290,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
 // found none.
 template <int W>
 __device__ __forceinline__ bool round_group(const SweepArgs &a, bool active, uint32_t v, uint64_t e0, uint64_t e1,
-                                            int gl, uint32_t t) {
+                                            int gl, uint32_t t, bool hub) {
   const uint64_t kv = active ? a.kin[v] : ~0ull;
   const uint32_t dist = static_cast<uint32_t>(kv >> 32);
   if (active && dist == t) {
+    if (!hub) {
+      e0 = a.off[v];
+      e1 = a.off[v + 1];
+    }
     const uint64_t cand = kv + a.hop;
     for (uint64_t e = e0 + gl; e < e1; e += W) {
       uint64_t *kw = a.kout + a.col[e];
@@ -305,26 +315,20 @@ __global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
   if (blk < a.b.bS) {
     const uint32_t i = blk * kBlock + tid;
     const bool act = i < a.b.nT;
-    uint32_t v = act ? a.b.listT[i] : 0;
-    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    pending = round_group<1>(a, act, v, e0, e1, 0, t);
+    pending = round_group<1>(a, act, act ? a.b.listT[i] : 0, 0, 0, 0, t, false);
   } else if (blk < a.b.bW) {
     const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
     const bool act = i < a.b.nS;
-    uint32_t v = act ? a.b.listS[i] : 0;
-    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    pending = round_group<16>(a, act, v, e0, e1, tid & 15, t);
+    pending = round_group<16>(a, act, act ? a.b.listS[i] : 0, 0, 0, tid & 15, t, false);
   } else if (blk < a.b.bH) {
     const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nW;
-    uint32_t v = act ? a.b.listW[i] : 0;
-    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    pending = round_group<kWave>(a, act, v, e0, e1, tid & 63, t);
+    pending = round_group<kWave>(a, act, act ? a.b.listW[i] : 0, 0, 0, tid & 63, t, false);
   } else {
     const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nH;
     HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
-    pending = round_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, t);
+    pending = round_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, t, true);
   }
   if (__ballot(pending) != 0 && lane_id() == 0) *a.flag = 1;
 }
@@ -332,15 +336,32 @@ __global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
 // Relabel pass: relab[e] = label of col[e] when it differs from the row's
 // label, kNone otherwise; writes are coalesced (the CSR layout).  The row's
 // count of cluster edges goes to deg_inter (hub chunks: hub_inter).
+// Groups of 16 or 64 lanes also drop repeats of a label inside their row
+// through a small direct-mapped LDS filter (best effort: a label is dropped
+// only when the filter holds it, so every kept set still covers the row's
+// distinct labels; the bucket dedup removes the rest).
 template <int W>
-__device__ __forceinline__ void relabel_group(const SweepArgs &a, uint32_t *relab, bool active, uint32_t v,
-                                              uint64_t e0, uint64_t e1, int gl, bool hub, uint32_t chunk) {
+__device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t *__restrict__ lab,
+                                              uint32_t *relab, bool active, uint32_t v,
+                                              uint64_t e0, uint64_t e1, int gl, bool hub, uint32_t chunk,
+                                              uint32_t *filt, uint32_t fmask) {
   uint32_t cnt = 0;
+  if (W > 1) {
+    for (uint32_t j = gl; j <= fmask; j += W) filt[j] = kNone;
+    wave_fence();
+  }
   if (active) {
-    const uint32_t lv = static_cast<uint32_t>(a.kin[v]);
+    const uint32_t lv = lab[v];
     for (uint64_t e = e0 + gl; e < e1; e += W) {
-      const uint32_t lw = static_cast<uint32_t>(a.kin[a.col[e]]);
-      const bool inter = lw != lv;
+      const uint32_t lw = lab[a.col[e]];
+      bool inter = lw != lv;
+      if (W > 1 && inter) {
+        const uint32_t slot = hash32(lw) & fmask;
+        if (filt[slot] == lw)
+          inter = false;
+        else
+          filt[slot] = lw;
+      }
       relab[e] = inter ? lw : kNone;
       cnt += inter;
     }
@@ -354,7 +375,10 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, uint32_t *rela
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_relabel(SweepArgs a, uint32_t *relab) {
+constexpr int kFilterWords = 4096;  // 16 KB of LDS per block
+
+__global__ __launch_bounds__(kBlock) void k_relabel(SweepArgs a, const uint32_t *lab, uint32_t *relab) {
+  __shared__ uint32_t filt[kFilterWords];
   const uint32_t blk = blockIdx.x;
   const int tid = threadIdx.x;
   if (blk < a.b.bS) {
@@ -362,24 +386,24 @@ __global__ __launch_bounds__(kBlock) void k_relabel(SweepArgs a, uint32_t *relab
     const bool act = i < a.b.nT;
     uint32_t v = act ? a.b.listT[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    relabel_group<1>(a, relab, act, v, e0, e1, 0, false, 0);
+    relabel_group<1>(a, lab, relab, act, v, e0, e1, 0, false, 0, nullptr, 0);
   } else if (blk < a.b.bW) {
     const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
     const bool act = i < a.b.nS;
     uint32_t v = act ? a.b.listS[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    relabel_group<16>(a, relab, act, v, e0, e1, tid & 15, false, 0);
+    relabel_group<16>(a, lab, relab, act, v, e0, e1, tid & 15, false, 0, filt + (tid / 16) * 256, 255);
   } else if (blk < a.b.bH) {
     const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nW;
     uint32_t v = act ? a.b.listW[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    relabel_group<kWave>(a, relab, act, v, e0, e1, tid & 63, false, 0);
+    relabel_group<kWave>(a, lab, relab, act, v, e0, e1, tid & 63, false, 0, filt + (tid / kWave) * 1024, 1023);
   } else {
     const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nH;
     HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
-    relabel_group<kWave>(a, relab, act, h.v, h.e0, h.e1, tid & 63, true, i);
+    relabel_group<kWave>(a, lab, relab, act, h.v, h.e0, h.e1, tid & 63, true, i, filt + (tid / kWave) * 1024, 1023);
   }
 }
 
@@ -940,10 +964,11 @@ __global__ void k_rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t 
   rep[v] = gid ? gid[a] : a;
 }
 
-__global__ void k_canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid) {
+__global__ void k_canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid, uint32_t *minid) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   const bool live = v < n;
-  peeled_min(minid, live ? rep[v] : 0u, static_cast<uint32_t>(v), live);
+  const uint32_t id = live ? (gid ? gid[v] : static_cast<uint32_t>(v)) : 0u;
+  peeled_min(minid, live ? rep[v] : 0u, id, live);
 }
 
 // Block count of the lanes with pred set, written to partial[blockIdx.x].
@@ -955,16 +980,51 @@ __device__ __forceinline__ void block_count(bool pred, uint32_t *partial) {
   if (threadIdx.x == 0) partial[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
-__global__ __launch_bounds__(kBlock) void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid,
-                                                       uint32_t *labels, uint32_t *partial) {
+__global__ __launch_bounds__(kBlock) void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid,
+                                                       const uint32_t *minid, uint32_t *labels, uint32_t *partial) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   bool root = false;
   if (v < n) {
+    const uint32_t id = gid ? gid[v] : static_cast<uint32_t>(v);
     const uint32_t l = minid[rep[v]];
-    labels[v] = l;
-    root = l == static_cast<uint32_t>(v);
+    labels[id] = l;
+    root = l == id;
   }
   block_count(root, partial);
+}
+
+// Level 0 without isolated vertices: newid[v] (kNone for degree 0), gid of
+// every kept vertex, its row offset, and the columns mapped through newid
+// (rows keep their order, so the edge order is unchanged).
+struct IsoLoad {
+  const uint64_t *off;
+  __device__ uint64_t operator()(uint64_t v) const { return off[v + 1] > off[v] ? 1ull : 0ull; }
+};
+struct IsoStore {
+  const uint64_t *off;
+  uint32_t *newid, *gid0;
+  uint64_t *off_c;
+  __device__ void operator()(uint64_t v, uint64_t prefix, uint64_t keep) const {
+    newid[v] = keep ? static_cast<uint32_t>(prefix) : kNone;
+    if (keep) {
+      gid0[prefix] = static_cast<uint32_t>(v);
+      off_c[prefix] = off[v];
+    }
+  }
+};
+__global__ void k_map_cols(uint64_t m2, const uint32_t *col, const uint32_t *newid, uint32_t *out) {
+  for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < m2;
+       e += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    out[e] = newid[col[e]];
+}
+__global__ void k_iso_finish(const uint64_t *total, uint64_t m2, uint64_t *off_c, uint64_t *n_out) {
+  off_c[*total] = m2;
+  *n_out = *total;
+}
+__global__ void k_identity(uint64_t n, uint32_t *p) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = static_cast<uint32_t>(i);
 }
 
 __global__ __launch_bounds__(kBlock) void k_labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels,
@@ -1027,7 +1087,8 @@ void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, 
 }
 
 void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS, uint32_t *listW,
-                  HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch, hipStream_t s) {
+                  HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch, hipStream_t s,
+                  const uint64_t *key, uint32_t tmin) {
   if (!n) {
     fill_u32(counters, 4, 0, s);
     return;
@@ -1037,11 +1098,11 @@ void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, 
   uint64_t *prefix = scratch + ceil_div(4ull * nb, 2) + 8;
   uint64_t *total = prefix + 4ull * nb + 8;
   uint64_t *rest = total + 8;
-  hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(kBlock), 0, s, n, off, cfg, counts, nb);
+  hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(kBlock), 0, s, n, off, cfg, counts, nb, key, tmin);
   KC_LAUNCH_CHECK();
   scan_exclusive<uint64_t>(4ull * nb, ArrLoadU32{counts}, ArrayStore<uint64_t>{prefix}, total, rest, s);
   hipLaunchKernelGGL(k_bin_write, dim3(nb), dim3(kBlock), 0, s, n, off, cfg, prefix, nb, listT, listS, listW, hubs,
-                     deg_inter);
+                     deg_inter, key, tmin);
   KC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bin_totals, dim3(1), dim3(64), 0, s, prefix, total, nb, counters);
   KC_LAUNCH_CHECK();
@@ -1135,11 +1196,11 @@ void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, ui
   scan_exclusive<uint64_t>(n, ArrLoadU32{in}, ArrayStore<uint64_t>{out}, total, scratch, s);
 }
 
-void relabel(const uint64_t *off, const uint32_t *col, const uint64_t *key, const Bins &b, uint32_t *relab,
+void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
              uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s) {
   if (b.blocks == 0) return;
-  SweepArgs a{off, col, key, nullptr, 0, b, deg_inter, hub_inter, nullptr, nullptr};
-  hipLaunchKernelGGL(k_relabel, dim3(b.blocks), dim3(kBlock), 0, s, a, relab);
+  SweepArgs a{off, col, nullptr, nullptr, 0, b, deg_inter, hub_inter, nullptr, nullptr};
+  hipLaunchKernelGGL(k_relabel, dim3(b.blocks), dim3(kBlock), 0, s, a, lab, relab);
   KC_LAUNCH_CHECK();
 }
 
@@ -1224,18 +1285,38 @@ void rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_
   KC_LAUNCH_CHECK();
 }
 
-void canon_min(uint64_t n, const uint32_t *rep, uint32_t *minid, hipStream_t s) {
+void canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid, uint32_t *minid, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_canon_min, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, minid);
+  hipLaunchKernelGGL(k_canon_min, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, gid, minid);
   KC_LAUNCH_CHECK();
 }
 
-void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *minid, uint32_t *labels, uint32_t *roots,
-                 uint32_t *partial, hipStream_t s) {
+void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid, const uint32_t *minid, uint32_t *labels,
+                 uint32_t *roots, uint32_t *partial, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_canon_label, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, minid, labels, partial);
+  hipLaunchKernelGGL(k_canon_label, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, gid, minid, labels, partial);
   KC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(grid_for(n)), roots);
+  KC_LAUNCH_CHECK();
+}
+
+void drop_isolated(uint64_t n, const uint64_t *off, const uint32_t *col, uint64_t m2, uint32_t *newid, uint32_t *gid0,
+                   uint64_t *off_c, uint32_t *col_c, uint64_t *n_out, uint64_t *scratch, hipStream_t s) {
+  if (!n) return;
+  uint64_t *total = scratch;
+  scan_exclusive<uint64_t>(n, IsoLoad{off}, IsoStore{off, newid, gid0, off_c}, total, scratch + 4, s);
+  hipLaunchKernelGGL(k_iso_finish, dim3(1), dim3(1), 0, s, total, m2, off_c, n_out);
+  KC_LAUNCH_CHECK();
+  if (m2) {
+    hipLaunchKernelGGL(k_map_cols, dim3(grid_for(m2) > 65536 ? 65536 : grid_for(m2)), dim3(kBlock), 0, s, m2, col, newid,
+                       col_c);
+    KC_LAUNCH_CHECK();
+  }
+}
+
+void identity_u32(uint64_t n, uint32_t *p, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_identity, dim3(grid_for(n) > 65536 ? 65536 : grid_for(n)), dim3(kBlock), 0, s, n, p);
   KC_LAUNCH_CHECK();
 }
 

@@ -208,11 +208,25 @@ void contract(const Csr &g, const std::vector<uint32_t> &lab, std::vector<uint32
 
 void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool fma, int stream,
                      uint64_t seq_limit, bool keep_keys, or_result &res) {
-  uint64_t rng_offset = 0;
-  for (uint64_t v = 0; v < g0.n; ++v) rng_offset += g0.off[v + 1] > g0.off[v];
-  Csr cur = g0;
-  std::vector<uint32_t> gid(g0.n);
-  for (uint64_t v = 0; v < g0.n; ++v) gid[v] = static_cast<uint32_t>(v);
+  // Level 0 holds the vertices of degree > 0 in id order: the reference sets
+  // degree-0 vertices inactive at load (lib/io/graph_io.h:380) and its vertex
+  // loops skip inactive vertices (lib/data_structures/dynamic_graph.h:113-117).
+  std::vector<uint32_t> newid(g0.n, kNone), gid;
+  for (uint64_t v = 0; v < g0.n; ++v)
+    if (g0.off[v + 1] > g0.off[v]) {
+      newid[v] = static_cast<uint32_t>(gid.size());
+      gid.push_back(static_cast<uint32_t>(v));
+    }
+  const std::vector<uint32_t> gid0 = gid;
+  const uint64_t rng_offset = gid.size();
+  Csr cur;
+  cur.n = gid.size();
+  cur.off.assign(cur.n + 1, 0);
+  cur.tgt.reserve(g0.off[g0.n]);
+  for (uint64_t a = 0; a < cur.n; ++a) {
+    for (uint64_t e = g0.off[gid[a]]; e < g0.off[gid[a] + 1]; ++e) cur.tgt.push_back(newid[g0.tgt[e]]);
+    cur.off[a + 1] = cur.tgt.size();
+  }
   uint32_t iteration = 1;
   std::vector<uint32_t> top_rep;
   bool have_top = false;
@@ -287,11 +301,13 @@ void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool
     rep.swap(down);
   }
   (void)have_top;
-  // ApplyToLocalComponents with canonical (minimum id) labels.
-  res.labels.assign(g0.n, 0);
+  // ApplyToLocalComponents with canonical (minimum id) labels; a degree-0
+  // vertex is its own component.
+  res.labels.resize(g0.n);
+  for (uint64_t v = 0; v < g0.n; ++v) res.labels[v] = static_cast<uint32_t>(v);
   std::vector<uint32_t> minid(g0.n, kNone);
-  for (uint64_t v = 0; v < g0.n; ++v) minid[rep[v]] = std::min<uint32_t>(minid[rep[v]], static_cast<uint32_t>(v));
-  for (uint64_t v = 0; v < g0.n; ++v) res.labels[v] = minid[rep[v]];
+  for (uint64_t a = 0; a < gid0.size(); ++a) minid[rep[a]] = std::min(minid[rep[a]], gid0[a]);
+  for (uint64_t a = 0; a < gid0.size(); ++a) res.labels[gid0[a]] = minid[rep[a]];
 }
 
 // Propagation (propagation.h:72-149) and ShortcutPropagation

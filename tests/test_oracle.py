@@ -176,11 +176,15 @@ def test_oracle_exp_levels_shrink_and_radius():
     run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1001, keep_keys=True)
     lv = run.levels()
     assert len(lv) >= 1
-    assert lv[0][0] == g.n and lv[0][1] == g.m
+    # level 0 holds the vertices of degree > 0, in id order
+    deg = np.diff(g.offsets)
+    assert (deg == 0).any()
+    assert lv[0][0] == int((deg > 0).sum()) and lv[0][1] == g.m
     for a, b in zip(lv, lv[1:]):
         assert b[0] < a[0]
     # radius = max hops from the centre, recomputed from the keys
     keys, gid = run.level_keys(0)
+    assert np.array_equal(gid, np.flatnonzero(deg > 0))
     lab = (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
     hops = (keys >> np.uint64(32)).astype(np.int64) - (keys[lab] >> np.uint64(32)).astype(np.int64)
     assert hops.max() == lv[0][3]
