@@ -30,7 +30,7 @@ VARIANT_EXP, VARIANT_LOCAL, VARIANT_SHORTCUT, VARIANT_LABELS = 0, 1, 2, 3
 VARIANTS = {"exp": 0, "local": 1, "shortcut": 2, "labels": 3}
 
 KERNEL_CLASSES = ["deviates", "sweep", "bin", "count", "scan", "scatter", "dedup", "csr", "undo", "canon",
-                  "jump", "other", "sort", "relabel"]
+                  "jump", "other", "sort", "relabel", "round"]
 _NK = len(KERNEL_CLASSES)
 
 
@@ -58,8 +58,8 @@ class Stats(ctypes.Structure):
                 ("send_volume", ctypes.c_uint64), ("recv_volume", ctypes.c_uint64), ("n", ctypes.c_uint64),
                 ("m", ctypes.c_uint64), ("levels", ctypes.c_uint32), ("components", ctypes.c_uint32),
                 ("sweeps", ctypes.c_uint64), ("radius_sum", ctypes.c_uint64), ("alg_bytes", ctypes.c_uint64),
-                ("kernel_ms", ctypes.c_double * 14), ("kernel_launches", ctypes.c_uint64 * 14),
-                ("kernel_bytes", ctypes.c_uint64 * 14)]
+                ("kernel_ms", ctypes.c_double * _NK), ("kernel_launches", ctypes.c_uint64 * _NK),
+                ("kernel_bytes", ctypes.c_uint64 * _NK)]
 
 
 class GenParams(ctypes.Structure):

@@ -383,7 +383,9 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
     uint32_t i = 0;
     while (i < batch && g->h_flags[i] != 0) ++i;
     const uint32_t worked = i < batch ? i + 1 : batch;
-    for (uint32_t j = worked; j < batch; ++j) g->timer.retag(slots[j], KC_K_OTHER, 0);
+    // launches after the last change returned at once: they keep their class
+    // (so launch counts match a kernel trace) but carry no bytes
+    for (uint32_t j = worked; j < batch; ++j) g->timer.retag(slots[j], KC_K_SWEEP, 0);
     done += worked;
     flag_base += batch;
     if (i < batch) break;
@@ -393,14 +395,17 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
 
 // Dedup of buckets [seg_off[a2], seg_off[a2+1]) for a2 < nb: small buckets
 // in wave tables, medium ones in block tables, huge ones in global tables
-// processed in fixed-size chunks planned on the host.
-void dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
+// processed in fixed-size chunks planned on the host.  Returns the timer slot
+// of the first dedup launch (its algorithmic bytes are booked by the caller).
+int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
                    uint32_t *medium, uint32_t *huge, uint64_t *huge_tab) {
   hipStream_t s = g->stream;
   fill_u64(g->sizes + 3, 3, 0, s);
   DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
+  int first;
   {
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
+    first = ts.slot;
     dedup_small(nb, seg_off, bucket, uniq, dl, s);
   }
   d2h(g, g->h_sizes + 3, g->sizes + 3, 3 * sizeof(uint64_t));
@@ -410,7 +415,7 @@ void dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
     dedup_medium(static_cast<uint32_t>(n_medium), medium, seg_off, bucket, uniq, s);
   }
-  if (!n_huge) return;
+  if (!n_huge) return first;
   if (2 * n_huge > g->huge_info_cap) {
     g->huge_info_cap = 2 * n_huge + 64;
     g->d_huge_info = g->dalloc<uint64_t>(g->huge_info_cap);
@@ -459,6 +464,7 @@ void dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *
              static_cast<uint32_t>(slot.size()), g->huge_tables, bucket, uniq, hcount, s);
   // the chunk vectors die here: wait until the uploads have been consumed
   sync(g);
+  return first;
 }
 
 // Time-stepped schedule until no vertex lies above the current round, then
@@ -471,6 +477,7 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
   const uint32_t batch = 4;
   Bins b = level_bins;
   uint32_t next_rebin = batch;
+  int first_slot = -1;
   while (true) {
     if (t >= next_rebin) {
       // drop the settled vertices from the rounds' bins (distances below t)
@@ -485,25 +492,26 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
     for (uint32_t i = 0; i < batch; ++i) {
       const uint32_t f = flag_base + i;
       const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
-      slots[i] = g->timer.start(KC_K_SWEEP);
+      slots[i] = g->timer.start(KC_K_ROUND);
       push_round(off, col, key, kOneHop, b, t + i, &g->flags[f], prev, g->stream);
-      g->timer.stop(slots[i], 8 * (n + 1) + 8 * n);
+      g->timer.stop(slots[i], 0);
+      if (first_slot < 0) first_slot = slots[i];
     }
     d2h(g, g->h_flags, &g->flags[flag_base], batch * sizeof(int));
     sync(g);
     uint32_t i = 0;
     while (i < batch && g->h_flags[i] != 0) ++i;
     const uint32_t worked = i < batch ? i + 1 : batch;
-    for (uint32_t j = worked; j < batch; ++j) g->timer.retag(slots[j], KC_K_OTHER, 0);
     done += worked;
     flag_base += batch;
     t += batch;
     if (i < batch) break;
   }
-  // every edge was pushed once: its key read and (when smaller) its atomicMin
-  {
-    TimedScope ts(g->timer, KC_K_SWEEP, 12 * m2);
-  }
+  // The rounds together do the work of one sweep: every row is read once,
+  // when its vertex fires (offsets, neighbour ids, neighbour keys), and every
+  // key is read and settled once: 8(n+1) + 12 2m + 16n bytes, booked on the
+  // level's first round.
+  if (first_slot >= 0) g->timer.retag(first_slot, KC_K_ROUND, 8 * (n + 1) + 12 * m2 + 16 * n);
   return done;
 }
 
@@ -681,7 +689,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l + 4 * P);
       scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, g->bucket, g->stream);
     }
-    dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab);
+    const int dedup_slot =
+        dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab);
     // next-level CSR
     uint64_t *off_next = g->off_pp[pp];
     uint32_t *col_next = g->col_pp[pp];
@@ -692,6 +701,9 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     d2h(g, g->h_sizes + 2, g->sizes + 2, sizeof(uint64_t));
     sync(g);
     const uint64_t m2_next = g->h_sizes[2];
+    // dedup: every bucket slot read once, the unique ones written, per
+    // cluster its bucket bounds and its unique count
+    g->timer.retag(dedup_slot, KC_K_DEDUP, 4 * P + 4 * m2_next + 8 * (n_next + 1) + 4 * n_next);
     {
       TimedScope ts(g->timer, KC_K_CSR, 4 * P + 4 * m2_next);
       compact(P, g->bucket, col_next, rec.newidx, g->scan_scratch, g->stream);

@@ -124,10 +124,10 @@ typedef struct kc_options {
 #define KC_STREAM_REFERENCE 0
 #define KC_STREAM_HASHED 1
 
-#define KC_KERNEL_CLASSES 14
+#define KC_KERNEL_CLASSES 15
 /* Kernel classes accounted in kc_stats.kernel_* (index = class). */
 #define KC_K_DEVIATES 0  /* deviate sampling + key init */
-#define KC_K_SWEEP 1     /* hooking sweep (decomposition) */
+#define KC_K_SWEEP 1     /* pull sweep (decomposition, base case) */
 #define KC_K_BIN 2       /* degree binning */
 #define KC_K_COUNT 3     /* cluster labels, radius, hub chunk folding */
 #define KC_K_SCAN 4      /* prefix sums: selection and cluster segments */
@@ -140,6 +140,7 @@ typedef struct kc_options {
 #define KC_K_OTHER 11
 #define KC_K_SORT 12     /* radix sort of (label, vertex) pairs */
 #define KC_K_RELABEL 13  /* edge slots relabelled with their far end's cluster */
+#define KC_K_ROUND 14    /* time-stepped push round (decomposition) */
 
 typedef struct kc_stats {
   double time_ms;       /* device time of the whole call (HIP events), labels resident on device */

@@ -31,6 +31,18 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(ca._SIGNATURES), "ctypes signatures and header differ"
 
 
+def test_kernel_classes_match_header():
+    """The ctypes stats layout and class names follow the header's class table."""
+    text = open(os.path.join(ROOT, "include", "kacomp.h")).read()
+    count = int(re.search(r"#define KC_KERNEL_CLASSES (\d+)", text).group(1))
+    assert len(ca.KERNEL_CLASSES) == count
+    idx = {int(v): k.lower() for k, v in re.findall(r"#define KC_K_([A-Z]+) (\d+)", text)}
+    assert sorted(idx) == list(range(count))
+    assert [idx[i] for i in range(count)] == ca.KERNEL_CLASSES
+    for f in ("kernel_ms", "kernel_launches", "kernel_bytes"):
+        assert getattr(ca.Stats, f).size == count * 8
+
+
 def test_read_metis_simple():
     g = ca.read_metis(os.path.join(GOLDEN, "simple.graph"))
     assert g.n == 6
