@@ -70,6 +70,35 @@ class GenParams(ctypes.Structure):
                 ("nz", ctypes.c_uint64)]
 
 
+class BatchHost(ctypes.Structure):
+    _fields_ = [("graphs", ctypes.c_uint32), ("vtx_off", ctypes.POINTER(ctypes.c_uint64)),
+                ("edge_off", ctypes.POINTER(ctypes.c_uint64)), ("offsets", ctypes.POINTER(ctypes.c_uint64)),
+                ("targets", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class BatchCsr(ctypes.Structure):
+    _fields_ = [("graphs", ctypes.c_uint32), ("vtx_off", ctypes.POINTER(ctypes.c_uint64)),
+                ("edge_off", ctypes.POINTER(ctypes.c_uint64)), ("offsets", ctypes.POINTER(ctypes.c_uint64)),
+                ("targets", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class BatchGenParams(ctypes.Structure):
+    _fields_ = [("graphs", ctypes.c_uint32), ("log_n_min", ctypes.c_double), ("log_n_max", ctypes.c_double),
+                ("deg_min", ctypes.c_double), ("deg_max", ctypes.c_double), ("types", ctypes.c_int32),
+                ("seed", ctypes.c_int32)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("groups_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("time_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("n", ctypes.c_uint64),
+                ("m", ctypes.c_uint64), ("graphs", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
+                ("levels_sum", ctypes.c_uint64), ("sweeps_sum", ctypes.c_uint64), ("radius_sum", ctypes.c_uint64),
+                ("components", ctypes.c_uint64), ("tuning", BatchTuning)]
+
+
 # Every symbol include/kacomp.h declares, with (restype, argtypes).
 _P = ctypes.POINTER
 _SIGNATURES = {
@@ -92,6 +121,17 @@ _SIGNATURES = {
     "kc_csr_free": (None, [_P(Csr)]),
     "kc_graph_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(ctypes.c_void_p)]),
     "kc_graph_download": (ctypes.c_int, [ctypes.c_void_p, _P(Csr)]),
+    "kc_batch_gen_params_init": (None, [_P(BatchGenParams)]),
+    "kc_batch_generate": (ctypes.c_int, [_P(BatchGenParams), _P(BatchCsr)]),
+    "kc_batch_free": (None, [_P(BatchCsr)]),
+    "kc_batch_create": (ctypes.c_int, [_P(BatchHost), _P(ctypes.c_void_p)]),
+    "kc_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "kc_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(BatchTuning)]),
+    "kc_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, _P(BatchTuning)]),
+    "kc_batch_run": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(ctypes.c_uint32), _P(BatchStats)]),
+    "kc_batch_graph_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32), _P(ctypes.c_uint32),
+                                            _P(ctypes.c_uint32)]),
+    "kc_batch_find_components": (ctypes.c_int, [_P(BatchHost), _P(Options), _P(ctypes.c_uint32), _P(BatchStats)]),
 }
 
 _lib = None
@@ -362,3 +402,112 @@ class ShortcutPropagation(_Component):
 
 class Propagation(_Component):
     variant = "labels"
+
+
+# ---------------------------------------------------------------------------
+# Batched form.
+@dataclass
+class BatchGraphs:
+    """Packed CSR segments: graph g owns vertices vtx_off[g]:vtx_off[g+1] and
+    edge slots edge_off[g]:edge_off[g+1]; offsets are global, targets local."""
+    vtx_off: np.ndarray
+    edge_off: np.ndarray
+    offsets: np.ndarray
+    targets: np.ndarray
+
+    @property
+    def graphs(self):
+        return len(self.vtx_off) - 1
+
+    def graph(self, g):
+        """Graph g as a standalone host CSR with local ids."""
+        a, z = int(self.vtx_off[g]), int(self.vtx_off[g + 1])
+        e0, e1 = int(self.edge_off[g]), int(self.edge_off[g + 1])
+        return Graph((self.offsets[a:z + 1] - np.uint64(e0)).astype(np.uint64), self.targets[e0:e1].copy())
+
+    @staticmethod
+    def pack(graphs):
+        vtx = np.zeros(len(graphs) + 1, np.uint64)
+        eo = np.zeros(len(graphs) + 1, np.uint64)
+        for i, g in enumerate(graphs):
+            vtx[i + 1] = vtx[i] + g.n
+            eo[i + 1] = eo[i] + int(g.offsets[-1])
+        offs = [np.asarray(g.offsets[:-1], np.uint64) + eo[i] for i, g in enumerate(graphs)]
+        offsets = np.concatenate(offs + [np.array([eo[-1]], np.uint64)])
+        targets = np.concatenate([np.asarray(g.targets, np.uint32) for g in graphs] + [np.zeros(0, np.uint32)])
+        return BatchGraphs(vtx, eo, offsets, targets)
+
+
+def batch_generate(graphs=4096, log_n_min=8.0, log_n_max=18.0, deg_min=2.0, deg_max=16.0, types=7, seed=4):
+    """kc_batch_generate: a mixed batch of G(n,m), 2D geometric and percolated grids."""
+    p = BatchGenParams()
+    lib().kc_batch_gen_params_init(ctypes.byref(p))
+    p.graphs, p.log_n_min, p.log_n_max = graphs, log_n_min, log_n_max
+    p.deg_min, p.deg_max, p.types, p.seed = deg_min, deg_max, types, seed
+    c = BatchCsr()
+    _check(lib().kc_batch_generate(ctypes.byref(p), ctypes.byref(c)))
+    G = int(c.graphs)
+    vtx = np.ctypeslib.as_array(c.vtx_off, shape=(G + 1,)).copy()
+    eo = np.ctypeslib.as_array(c.edge_off, shape=(G + 1,)).copy()
+    n, m2 = int(vtx[-1]), int(eo[-1])
+    off = np.ctypeslib.as_array(c.offsets, shape=(n + 1,)).copy()
+    tgt = np.ctypeslib.as_array(c.targets, shape=(max(m2, 1),))[:m2].copy()
+    lib().kc_batch_free(ctypes.byref(c))
+    return BatchGraphs(vtx, eo, off, tgt)
+
+
+def batch_stats_dict(st):
+    d = {k: getattr(st, k) for k, _ in BatchStats._fields_ if k != "tuning"}
+    t = st.tuning
+    d["tuning"] = {"block": t.block, "groups_per_cu": t.groups_per_cu, "use_graph": t.use_graph}
+    return d
+
+
+class DeviceBatch:
+    """kc_batch handle: the packed batch copied once to HBM."""
+
+    def __init__(self, b: BatchGraphs):
+        self.h = ctypes.c_void_p()
+        self.graphs = b.graphs
+        self.n = int(b.vtx_off[-1])
+        self._keep = [np.ascontiguousarray(b.vtx_off, np.uint64), np.ascontiguousarray(b.edge_off, np.uint64),
+                      np.ascontiguousarray(b.offsets, np.uint64), np.ascontiguousarray(b.targets, np.uint32)]
+        if len(self._keep[3]) == 0:
+            self._keep[3] = np.zeros(1, np.uint32)
+        bh = BatchHost(self.graphs, *(_u64p(a) for a in self._keep[:3]), _u32p(self._keep[3]))
+        _check(lib().kc_batch_create(ctypes.byref(bh), ctypes.byref(self.h)))
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            lib().kc_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, block=256, groups_per_cu=4, use_graph=True):
+        t = BatchTuning(block, groups_per_cu, 1 if use_graph else 0)
+        _check(lib().kc_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, **kw):
+        t = BatchTuning()
+        _check(lib().kc_batch_autotune(self.h, ctypes.byref(options(**kw)), ctypes.byref(t)))
+        return {"block": t.block, "groups_per_cu": t.groups_per_cu, "use_graph": t.use_graph}
+
+    def run(self, copy_labels=True, **kw):
+        st = BatchStats()
+        labels = np.empty(max(self.n, 1), np.uint32)
+        _check(lib().kc_batch_run(self.h, ctypes.byref(options(**kw)), _u32p(labels) if copy_labels else None,
+                                  ctypes.byref(st)))
+        return (labels[: self.n] if copy_labels else None), batch_stats_dict(st)
+
+    def graph_stats(self):
+        lv = np.empty(max(self.graphs, 1), np.uint32)
+        sw = np.empty_like(lv)
+        rd = np.empty_like(lv)
+        _check(lib().kc_batch_graph_stats(self.h, _u32p(lv), _u32p(sw), _u32p(rd)))
+        return lv[: self.graphs], sw[: self.graphs], rd[: self.graphs]

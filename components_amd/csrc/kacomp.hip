@@ -149,7 +149,7 @@ static uint64_t truncated_deviate(double w, double beta) {
   return static_cast<uint64_t>(-std::log(w) / beta);
 }
 
-static std::vector<double> build_deviate_table(double beta) {
+std::vector<double> build_deviate_table(double beta) {
   const double wmin = 0x1p-53;  // smallest 1-u the canonical draw can produce
   const uint64_t kmax = truncated_deviate(wmin, beta);
   std::vector<double> t(kmax);
@@ -271,21 +271,6 @@ namespace {
 constexpr uint32_t kMaxFlags = 1u << 16;
 constexpr uint32_t kMinWaveBin = 256;
 
-template <class F>
-int guarded(F &&f) {
-  try {
-    f();
-    return KC_OK;
-  } catch (const Failure &e) {
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    set_error("host allocation failed");
-    return KC_ERR_OOM;
-  } catch (...) {
-    set_error("unexpected exception");
-    return KC_ERR_INVALID;
-  }
-}
 
 void sync(kc_graph *g) { KC_HIP(hipStreamSynchronize(g->stream)); }
 

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <new>
 #include <string>
 #include <utility>
 #include <vector>
@@ -38,6 +39,23 @@ void set_error(const char *fmt, ...);
   } while (0)
 
 #define KC_LAUNCH_CHECK() KC_HIP(hipGetLastError())
+
+// Runs f and turns a Failure (or any exception) into the C ABI's return code.
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return KC_OK;
+  } catch (const Failure &e) {
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return KC_ERR_OOM;
+  } catch (...) {
+    set_error("unexpected exception");
+    return KC_ERR_INVALID;
+  }
+}
 
 __host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
@@ -71,7 +89,7 @@ class Arena {
 // Kernel timing with HIP events recorded on the launch stream.  Every launch
 // group is one record (class, algorithmic bytes, optional event pair); a
 // record can be re-tagged once its outcome is known (speculative sweeps that
-// returned at once are moved to KC_K_OTHER with zero bytes).
+// returned at once keep their class with zero bytes).
 class KernelTimer {
  public:
   ~KernelTimer();

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace kc {
 
@@ -50,6 +51,10 @@ struct DeviateArgs {
 struct LearnedArgs {
   const uint16_t *scale_q8;  // null: plain deviates
 };
+
+// Thresholds W_k (k = 1..k_max) of the truncated exponential deviate with
+// rate beta: the deviate of a draw u is the number of W_k >= 1 - u.
+std::vector<double> build_deviate_table(double beta);
 
 void fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);

@@ -198,6 +198,89 @@ int kc_trace_level_keys(const kc_graph *g, uint32_t level, uint64_t *keys_out, u
 int kc_deviates(uint32_t gid0, uint32_t count, int32_t seed, uint32_t iteration, uint64_t rng_offset,
                 double beta, int32_t fma, int32_t stream, uint32_t *out);
 
+/* ------------------------------------------------------------------------
+ * Batched form: many independent graphs contracted in one persistent launch.
+ * No reference counterpart as a batch; per graph it computes what
+ * ExponentialContraction::FindComponents (lib/components/exponential_contraction.h:60-319)
+ * computes on that graph alone, with labels local to the graph (the minimum
+ * local vertex id of the component).
+ *
+ * Packed CSR segments: graph g owns vertices [vtx_off[g], vtx_off[g+1]) and
+ * edge slots [edge_off[g], edge_off[g+1]); offsets are global (offsets[vtx_off[g]]
+ * == edge_off[g]), targets are local ids (< n_g).  Every graph needs fewer than
+ * 2^31 edge slots. */
+typedef struct kc_batch_host {
+  uint32_t graphs;
+  const uint64_t *vtx_off;  /* graphs + 1 */
+  const uint64_t *edge_off; /* graphs + 1 */
+  const uint64_t *offsets;  /* vtx_off[graphs] + 1 */
+  const uint32_t *targets;  /* edge_off[graphs] */
+} kc_batch_host;
+
+/* A packed batch owned by the library (kc_batch_generate); free with kc_batch_free. */
+typedef struct kc_batch_csr {
+  uint32_t graphs;
+  uint64_t *vtx_off, *edge_off, *offsets;
+  uint32_t *targets;
+} kc_batch_csr;
+
+/* Mixed batch: n_g = round(2^(log_n_min + (log_n_max - log_n_min) u)), average
+ * degree uniform in [deg_min, deg_max], type uniform over the enabled ones
+ * (types bit 0: gnm_undirected with m = n d / 2; bit 1: rgg_2d with
+ * r = sqrt(d / (pi n)); bit 2: bond-percolated grid, 2D or 3D with equal
+ * probability, side round(n^(1/dims)), keep probability min(1, d / (2 dims))). */
+typedef struct kc_batch_gen_params {
+  uint32_t graphs;
+  double log_n_min, log_n_max;
+  double deg_min, deg_max;
+  int32_t types;
+  int32_t seed;
+} kc_batch_gen_params;
+
+/* Launch shape of the persistent kernel: threads per workgroup (256, 512 or
+ * 1024), workgroups per compute unit, and whether the launch sequence replays
+ * a captured HIP graph.  Every row of a graph is processed by one lane, so the
+ * batched form suits bounded-degree graphs; heavy-tailed graphs belong on the
+ * plain path (kc_find_components), whose degree bins split long rows. */
+typedef struct kc_batch_tuning {
+  int32_t block;
+  int32_t groups_per_cu;
+  int32_t use_graph;
+} kc_batch_tuning;
+
+typedef struct kc_batch_stats {
+  double time_ms;       /* device time of the launch sequence (HIP events) */
+  double total_ms;      /* host wall time of the call */
+  uint64_t n, m;        /* sums over the graphs (m undirected) */
+  uint32_t graphs;
+  uint32_t max_levels;  /* largest level count of a graph */
+  uint64_t levels_sum;  /* sums over the graphs of levels, sweeps, max cluster radii */
+  uint64_t sweeps_sum;
+  uint64_t radius_sum;
+  uint64_t components;  /* sum over the graphs */
+  kc_batch_tuning tuning;
+} kc_batch_stats;
+
+typedef struct kc_batch kc_batch;
+
+void kc_batch_gen_params_init(kc_batch_gen_params *p);
+int kc_batch_generate(const kc_batch_gen_params *p, kc_batch_csr *out);
+void kc_batch_free(kc_batch_csr *b);
+/* Copies the batch to the device (input arrays are not retained). */
+int kc_batch_create(const kc_batch_host *b, kc_batch **out);
+int kc_batch_destroy(kc_batch *b);
+/* Times every candidate launch shape on the batch and keeps the fastest. */
+int kc_batch_autotune(kc_batch *b, const kc_options *opt, kc_batch_tuning *chosen);
+int kc_batch_set_tuning(kc_batch *b, const kc_batch_tuning *t);
+/* Exponential contraction of every graph (variant KC_VARIANT_EXP only; no
+ * base case).  labels_out (host, vtx_off[graphs] entries) may be NULL. */
+int kc_batch_run(kc_batch *b, const kc_options *opt, uint32_t *labels_out, kc_batch_stats *stats);
+/* Per-graph levels, sweeps and summed max cluster radii of the last run (graphs entries each, any may be NULL). */
+int kc_batch_graph_stats(const kc_batch *b, uint32_t *levels, uint32_t *sweeps, uint32_t *radius);
+/* One-shot form: create, run with the default launch shape, destroy. */
+int kc_batch_find_components(const kc_batch_host *b, const kc_options *opt, uint32_t *labels_out,
+                             kc_batch_stats *stats);
+
 /* Host I/O and generators. */
 int kc_read_metis(const char *path, kc_csr *out);
 int kc_write_metis(const char *path, const kc_csr *g);

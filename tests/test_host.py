@@ -176,3 +176,46 @@ def test_cli_flag_parsing_without_device():
     assert re.search(r"\bn\s+256\b", r.stdout) and re.search(r"\bm\s+512\b", r.stdout)
     if not os.path.exists("/dev/kfd"):
         assert r.returncode != 0
+
+
+def test_batch_generator_valid_and_deterministic():
+    b = ca.batch_generate(graphs=64, log_n_min=2, log_n_max=10, deg_min=1, deg_max=12, seed=3)
+    b2 = ca.batch_generate(graphs=64, log_n_min=2, log_n_max=10, deg_min=1, deg_max=12, seed=3)
+    assert np.array_equal(b.offsets, b2.offsets) and np.array_equal(b.targets, b2.targets)
+    assert b.graphs == 64 and b.vtx_off[0] == 0 and b.edge_off[0] == 0
+    assert np.all(np.diff(b.vtx_off.astype(np.int64)) >= 1)
+    for g in range(b.graphs):
+        gr = b.graph(g)
+        assert gr.n >= 2 and gr.n <= 2 ** 10 + 64
+        assert int(b.offsets[int(b.vtx_off[g])]) == int(b.edge_off[g])
+        src = np.repeat(np.arange(gr.n), np.diff(gr.offsets).astype(np.int64))
+        assert np.all(gr.targets < gr.n) and np.all(src != gr.targets)
+        a = set(zip(src.tolist(), gr.targets.tolist()))
+        assert a == {(v, u) for u, v in a}, f"graph {g} not symmetric"
+    b3 = ca.batch_generate(graphs=64, log_n_min=2, log_n_max=10, deg_min=1, deg_max=12, seed=4)
+    assert not np.array_equal(b.vtx_off, b3.vtx_off)
+
+
+def test_batch_generator_types():
+    # one type at a time: G(n,m) keeps n exactly; grids have side^dims vertices
+    b = ca.batch_generate(graphs=16, log_n_min=6, log_n_max=6, deg_min=4, deg_max=4, types=1, seed=1)
+    assert np.all(np.diff(b.vtx_off.astype(np.int64)) == 64)
+    g = ca.batch_generate(graphs=16, log_n_min=6, log_n_max=6, deg_min=4, deg_max=4, types=4, seed=1)
+    assert set(np.diff(g.vtx_off.astype(np.int64)).tolist()) <= {64, 64}  # 8x8 and 4x4x4
+    with pytest.raises(ca.KcError):
+        ca.batch_generate(graphs=4, types=0)
+    with pytest.raises(ca.KcError):
+        ca.batch_generate(graphs=4, log_n_min=5, log_n_max=3)
+
+
+def test_batch_create_rejects_bad_input():
+    good = ca.BatchGraphs.pack([ca.Graph.from_edges(3, [(0, 1)]), ca.Graph.from_edges(2, [(0, 1)])])
+    bad_target = ca.BatchGraphs(good.vtx_off, good.edge_off, good.offsets, good.targets.copy())
+    bad_target.targets[-1] = 2  # graph 1 has two vertices
+    with pytest.raises(ca.KcError) as e:
+        ca.DeviceBatch(bad_target)
+    assert e.value.code == ca.KC_ERR_INVALID and "out of range" in str(e.value)
+    bad_off = ca.BatchGraphs(good.vtx_off, good.edge_off + np.uint64(1), good.offsets, good.targets)
+    with pytest.raises(ca.KcError) as e:
+        ca.DeviceBatch(bad_off)
+    assert e.value.code == ca.KC_ERR_INVALID
