@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--tmax", type=int, default=0)
     ap.add_argument("--smax", type=int, default=0)
     ap.add_argument("--wmax", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -170,7 +172,70 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"], out["labels_bit_exact"] = cpu_leg(a, dg, run_kw, stream)
     dg.close()
+    if not a.no_batched:
+        out["batched"] = batched_leg(a, stream)
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(a, stream):
+    """The batched form on the mixed batch (4096 graphs: n log-uniform in
+    [2^8, 2^18], average degree uniform in [2, 16], G(n,m) / 2D geometric /
+    percolated grid, seed 4), beside the plain path on the same graphs taken as
+    one disjoint union.  Labels of every graph are checked against the oracle's
+    union-find labels of the union."""
+    import numpy as np
+
+    import components_amd as ca
+    import oracle
+
+    t0 = time.time()
+    b = ca.batch_generate(graphs=a.batch_graphs, seed=4)
+    gen_s = time.time() - t0
+    n, m = int(b.vtx_off[-1]), int(b.edge_off[-1]) // 2
+    print(f"[bench] batch: {b.graphs} graphs, n={n} m={m} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    db = ca.DeviceBatch(b)
+    tune = db.autotune(seed=1001, stream=stream)
+    db.set_tuning(block=tune["block"], groups_per_cu=tune["groups_per_cu"], use_graph=True)
+    print(f"[bench] batch: tuned {tune}", file=sys.stderr, flush=True)
+    for i in range(a.warmup):
+        db.run(copy_labels=False, seed=1 + i + 1000, stream=stream)
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        _, st = db.run(copy_labels=False, seed=1 + i + 1000, stream=stream)
+        stats.append(st)
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    dev_ms = float(np.mean([s["time_ms"] for s in stats]))
+    lab, st = db.run(seed=1001, stream=stream)
+    db.close()
+    # the disjoint union with global ids: oracle labels and the plain path
+    per_vertex_base = np.repeat(b.vtx_off[:-1].astype(np.uint32), np.diff(b.vtx_off).astype(np.int64))
+    tg = b.targets + np.repeat(b.vtx_off[:-1].astype(np.uint32), np.diff(b.edge_off).astype(np.int64))
+    ref = oracle.canonical_labels(b.offsets, tg) - per_vertex_base
+    exact = bool(np.array_equal(lab, ref))
+    del ref
+    print(f"[bench] batch: {ms:.2f} ms/step, labels bit-exact {exact}", file=sys.stderr, flush=True)
+    dg = ca.DeviceGraph(ca.Graph(b.offsets, tg))
+    del tg
+    seq = max(m // 256, 4096)
+    plain = []
+    for i in range(a.warmup):
+        dg.find_components(copy_labels=False, seed=1 + i + 1000, seq_limit=seq, stream=stream, use_bfs=True)
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        plain.append(dg.find_components(copy_labels=False, seed=1 + i + 1000, seq_limit=seq, stream=stream,
+                                        use_bfs=True)[1])
+    pms = (time.perf_counter() - t0) * 1e3 / a.steps
+    dg.close()
+    return {"workload": f"batch-{b.graphs}-graphs-n2^8..2^18-deg2..16-gnm/rgg2d/grid-seed4", "graphs": b.graphs,
+            "n": n, "m": m, "value": round(m / (ms / 1e3) / 1e9, 4), "unit": "GTEPS", "ms_per_step": round(ms, 4),
+            "device_ms_per_step": round(dev_ms, 4), "levels_sum": stats[-1]["levels_sum"],
+            "max_levels": stats[-1]["max_levels"], "sweeps_sum": stats[-1]["sweeps_sum"],
+            "radius_sum": stats[-1]["radius_sum"], "tuning": stats[-1]["tuning"], "labels_bit_exact": exact,
+            "plain_union": {"value": round(m / (pms / 1e3) / 1e9, 4), "unit": "GTEPS", "ms_per_step": round(pms, 4),
+                            "levels": plain[-1]["levels"], "sweeps": plain[-1]["sweeps"],
+                            "schedule": "rounds", "seq_limit": seq},
+            "generation_s": round(gen_s, 1)}
 
 
 def cpu_leg(a, dg, run_kw, stream):

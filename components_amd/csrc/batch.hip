@@ -57,7 +57,7 @@ struct BatchDev {
   uint64_t *table;    // per graph at table_off[g]: cluster-edge hash set (all empty between calls)
   const uint64_t *table_off;
   uint32_t *stack;  // per graph at 10 * vtx_off[g]: level records for the undo
-  uint32_t *glevels, *gsweeps, *gradius, *gstatus;
+  uint32_t *glevels, *gsweeps, *gradius, *gstatus, *gcomp;
   uint32_t *queue;
   const BatchParams *params;
 };
@@ -126,6 +126,7 @@ struct Shared {
   uint32_t flag;
   uint32_t radius;
   uint32_t full;
+  uint32_t roots;
 };
 
 // One in-place sweep over the level (one lane per row); returns through
@@ -137,7 +138,15 @@ __device__ void sweep_level(uint32_t nl, const uint32_t *off, const uint32_t *co
     if (e1 == e0) continue;
     const uint64_t own = key[v];
     uint64_t best = own;
-    for (uint32_t e = e0; e < e1; ++e) {
+    uint32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {  // four independent gathers in flight
+      const uint32_t c0 = col[e], c1 = col[e + 1], c2 = col[e + 2], c3 = col[e + 3];
+      const uint64_t k0 = key[c0], k1 = key[c1], k2 = key[c2], k3 = key[c3];
+      const uint64_t a = (k0 < k1 ? k0 : k1), b = (k2 < k3 ? k2 : k3);
+      const uint64_t c = (a < b ? a : b) + kOneHop;
+      best = c < best ? c : best;
+    }
+    for (; e < e1; ++e) {
       const uint64_t c = key[col[e]] + kOneHop;
       best = c < best ? c : best;
     }
@@ -181,6 +190,7 @@ __device__ void contract_graph(const BatchDev &d, const BatchParams &p, uint32_t
       d.gsweeps[g] = 0;
       d.gradius[g] = 0;
       d.gstatus[g] = 0;
+      d.gcomp[g] = 0;
     }
     return;
   }
@@ -201,6 +211,7 @@ __device__ void contract_graph(const BatchDev &d, const BatchParams &p, uint32_t
       d.gsweeps[g] = 0;
       d.gradius[g] = 0;
       d.gstatus[g] = 0;
+      d.gcomp[g] = n0;
     }
     __syncthreads();
     return;
@@ -365,12 +376,21 @@ __device__ void contract_graph(const BatchDev &d, const BatchParams &p, uint32_t
   } else {
     // canonical: minimum local id per representative
     for (uint32_t v = tid; v < n0; v += B) cnt[v] = kNone;
+    if (tid == 0) s.roots = 0;
     __syncthreads();
     for (uint32_t v = tid; v < n0; v += B) atomicMin(&cnt[up[v]], v);
     __syncthreads();
-    for (uint32_t v = tid; v < n0; v += B) labels[v] = cnt[up[v]];
+    uint32_t roots = 0;
+    for (uint32_t v = tid; v < n0; v += B) {
+      const uint32_t l = cnt[up[v]];
+      labels[v] = l;
+      roots += l == v;
+    }
+    if (roots) atomicAdd(&s.roots, roots);
+    __syncthreads();
   }
   if (tid == 0) {
+    d.gcomp[g] = up ? s.roots : 0;
     d.glevels[g] = levels;
     d.gsweeps[g] = sweeps;
     d.gradius[g] = radius_sum;
@@ -669,6 +689,7 @@ int kc_batch_create(const kc_batch_host *h, kc_batch **out) {
     d.gsweeps = b->dalloc<uint32_t>(G);
     d.gradius = b->dalloc<uint32_t>(G);
     d.gstatus = b->dalloc<uint32_t>(G);
+    d.gcomp = b->dalloc<uint32_t>(G);
     b->queue = b->dalloc<uint32_t>(1);
     d.queue = b->queue;
     b->d_params = b->dalloc<BatchParams>(1);
@@ -712,25 +733,19 @@ int kc_batch_run(kc_batch *b, const kc_options *opt, uint32_t *labels_out, kc_ba
       stats->m = b->m2 / 2;
       stats->graphs = b->graphs;
       stats->tuning = b->tuning;
-      std::vector<uint32_t> lv(b->graphs), sw(b->graphs), rd(b->graphs);
+      std::vector<uint32_t> lv(b->graphs), sw(b->graphs), rd(b->graphs), cc(b->graphs);
       if (b->graphs) {
         KC_HIP(hipMemcpy(lv.data(), b->dev.glevels, 4ull * b->graphs, hipMemcpyDeviceToHost));
         KC_HIP(hipMemcpy(sw.data(), b->dev.gsweeps, 4ull * b->graphs, hipMemcpyDeviceToHost));
         KC_HIP(hipMemcpy(rd.data(), b->dev.gradius, 4ull * b->graphs, hipMemcpyDeviceToHost));
+        KC_HIP(hipMemcpy(cc.data(), b->dev.gcomp, 4ull * b->graphs, hipMemcpyDeviceToHost));
       }
       for (uint32_t g = 0; g < b->graphs; ++g) {
         stats->levels_sum += lv[g];
         stats->sweeps_sum += sw[g];
         stats->radius_sum += rd[g];
+        stats->components += cc[g];
         stats->max_levels = std::max(stats->max_levels, lv[g]);
-      }
-      if (b->n) {
-        // components: vertices that are their own label
-        std::vector<uint32_t> lab(b->n);
-        KC_HIP(hipMemcpy(lab.data(), b->dev.labels, 4ull * b->n, hipMemcpyDeviceToHost));
-        for (uint32_t g = 0; g < b->graphs; ++g)
-          for (uint64_t v = b->h_vtx_off[g]; v < b->h_vtx_off[g + 1]; ++v)
-            stats->components += lab[v] == v - b->h_vtx_off[g];
       }
     }
   });
