@@ -30,7 +30,7 @@ VARIANT_EXP, VARIANT_LOCAL, VARIANT_SHORTCUT, VARIANT_LABELS = 0, 1, 2, 3
 VARIANTS = {"exp": 0, "local": 1, "shortcut": 2, "labels": 3}
 
 KERNEL_CLASSES = ["deviates", "sweep", "bin", "count", "scan", "scatter", "dedup", "csr", "undo", "canon",
-                  "jump", "other", "sort", "relabel", "round"]
+                  "jump", "other", "sort", "relabel", "round", "learned"]
 _NK = len(KERNEL_CLASSES)
 
 
@@ -50,7 +50,8 @@ class Options(ctypes.Structure):
                 ("learned", ctypes.c_int32), ("record_trace", ctypes.c_int32), ("time_kernels", ctypes.c_int32),
                 ("max_levels", ctypes.c_int32), ("sweep_group_small", ctypes.c_int32),
                 ("sweep_group_mid", ctypes.c_int32), ("sweep_group_large", ctypes.c_int32),
-                ("deviate_stream", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("deviate_stream", ctypes.c_int32), ("learned_alpha", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 5)]
 
 
 class Stats(ctypes.Structure):
@@ -99,6 +100,21 @@ class BatchStats(ctypes.Structure):
                 ("components", ctypes.c_uint64), ("tuning", BatchTuning)]
 
 
+class TrainParams(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+                ("seed", ctypes.c_int32)]
+
+
+class TrainStats(ctypes.Structure):
+    _fields_ = [("first_loss", ctypes.c_double), ("last_loss", ctypes.c_double), ("time_ms", ctypes.c_double),
+                ("samples", ctypes.c_uint64), ("steps", ctypes.c_int32)]
+
+
+MLP_PARAMS = 4673
+MLP_FEATURES = 6
+
+
 # Every symbol include/kacomp.h declares, with (restype, argtypes).
 _P = ctypes.POINTER
 _SIGNATURES = {
@@ -132,6 +148,23 @@ _SIGNATURES = {
     "kc_batch_graph_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32), _P(ctypes.c_uint32),
                                             _P(ctypes.c_uint32)]),
     "kc_batch_find_components": (ctypes.c_int, [_P(BatchHost), _P(Options), _P(ctypes.c_uint32), _P(BatchStats)]),
+    "kc_mlp_create": (ctypes.c_int, [ctypes.c_int32, _P(ctypes.c_void_p)]),
+    "kc_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "kc_mlp_get_weights": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float)]),
+    "kc_mlp_set_weights": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float)]),
+    "kc_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float), ctypes.c_uint64, _P(ctypes.c_float)]),
+    "kc_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_uint64,
+                                        _P(ctypes.c_float), _P(ctypes.c_float)]),
+    "kc_mlp_collect": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _P(Options), ctypes.c_uint64]),
+    "kc_mlp_dataset_size": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64)]),
+    "kc_mlp_clear_dataset": (ctypes.c_int, [ctypes.c_void_p]),
+    "kc_mlp_add_dataset": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_uint64]),
+    "kc_mlp_get_dataset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, _P(ctypes.c_float),
+                                          _P(ctypes.c_float)]),
+    "kc_train_params_init": (None, [_P(TrainParams)]),
+    "kc_mlp_train": (ctypes.c_int, [ctypes.c_void_p, _P(TrainParams), _P(TrainStats)]),
+    "kc_graph_set_mlp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "kc_graph_features": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float)]),
 }
 
 _lib = None
@@ -241,7 +274,8 @@ STREAM_REFERENCE, STREAM_HASHED = 0, 1
 
 
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
-            tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE):
+            tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
+            alpha=1.0):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -255,6 +289,8 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     o.use_bfs = 1 if use_bfs else 0
     o.max_levels = max_levels
     o.deviate_stream = stream
+    o.learned = 1 if learned else 0
+    o.learned_alpha = alpha
     return o
 
 
@@ -320,6 +356,17 @@ class DeviceGraph:
         labels = np.empty(max(self.n, 1), np.uint32)
         _check(lib().kc_copy_labels(self.h, _u32p(labels)))
         return labels[: self.n]
+
+    def set_mlp(self, mlp):
+        """Model used when find_components(learned=True)."""
+        _check(lib().kc_graph_set_mlp(self.h, mlp.h if mlp is not None else None))
+        self._mlp = mlp
+
+    def features(self):
+        """Level-0 features (vertices of degree > 0 in id order), non_isolated x 6."""
+        X = np.empty((max(self.non_isolated, 1), MLP_FEATURES), np.float32)
+        _check(lib().kc_graph_features(self.h, X.ctypes.data_as(_P(ctypes.c_float))))
+        return X[: self.non_isolated]
 
     def trace(self):
         """Per-level (info, keys, gid) of the last run with record_trace=True."""
@@ -511,3 +558,84 @@ class DeviceBatch:
         rd = np.empty_like(lv)
         _check(lib().kc_batch_graph_stats(self.h, _u32p(lv), _u32p(sw), _u32p(rd)))
         return lv[: self.graphs], sw[: self.graphs], rd[: self.graphs]
+
+
+# ---------------------------------------------------------------------------
+# Learned centre proposals.
+def _f32p(a):
+    return a.ctypes.data_as(_P(ctypes.c_float))
+
+
+class Mlp:
+    """kc_mlp handle: MLP 6 -> 64 -> 64 -> 1 (bf16 weights, f32 accumulation) in HBM."""
+
+    def __init__(self, seed=1):
+        self.h = ctypes.c_void_p()
+        _check(lib().kc_mlp_create(seed, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().kc_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        w = np.empty(MLP_PARAMS, np.float32)
+        _check(lib().kc_mlp_get_weights(self.h, _f32p(w)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert len(w) == MLP_PARAMS
+        _check(lib().kc_mlp_set_weights(self.h, _f32p(w)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_FEATURES)
+        out = np.empty(max(len(X), 1), np.float32)
+        _check(lib().kc_mlp_forward(self.h, _f32p(X), len(X), _f32p(out)))
+        return out[: len(X)]
+
+    def gradients(self, X, y):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_FEATURES)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().kc_mlp_gradients(self.h, _f32p(X), _f32p(y), len(X), _f32p(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def collect(self, dg, stride=1, **kw):
+        _check(lib().kc_mlp_collect(self.h, dg.h, ctypes.byref(options(**kw)), stride))
+
+    def dataset_size(self):
+        n = ctypes.c_uint64()
+        _check(lib().kc_mlp_dataset_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def dataset(self, first=0, count=None):
+        n = self.dataset_size()
+        count = n - first if count is None else count
+        X = np.empty((max(count, 1), MLP_FEATURES), np.float32)
+        y = np.empty(max(count, 1), np.float32)
+        _check(lib().kc_mlp_get_dataset(self.h, first, count, _f32p(X), _f32p(y)))
+        return X[:count], y[:count]
+
+    def add_dataset(self, X, y):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_FEATURES)
+        y = np.ascontiguousarray(y, np.float32)
+        _check(lib().kc_mlp_add_dataset(self.h, _f32p(X), _f32p(y), len(X)))
+
+    def clear_dataset(self):
+        _check(lib().kc_mlp_clear_dataset(self.h))
+
+    def train(self, steps=400, batch=4096, lr=3e-3, seed=1):
+        p = TrainParams()
+        lib().kc_train_params_init(ctypes.byref(p))
+        p.steps, p.batch, p.lr, p.seed = steps, batch, lr, seed
+        st = TrainStats()
+        _check(lib().kc_mlp_train(self.h, ctypes.byref(p), ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in TrainStats._fields_}

@@ -26,6 +26,7 @@
 #include "deviates.h"
 #include "kc_common.h"
 #include "kernels.h"
+#include "learned.h"
 
 namespace kc {
 
@@ -244,6 +245,13 @@ struct kc_graph {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   uint32_t components = 0;
   bool have_labels = false;
+  // learned centre proposals: the model, and per-vertex buffers sized for level 0
+  kc_mlp *mlp = nullptr;
+  kc_mlp *collect = nullptr;  // set while kc_mlp_collect records a plain run
+  uint64_t collect_stride = 1;
+  float *feat = nullptr, *target = nullptr;
+  uint32_t *ldeg = nullptr, *csize = nullptr;
+  uint8_t *delay = nullptr;
   std::vector<void *> owned;
 
   template <class T>
@@ -567,7 +575,16 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         init_keys_uniform(n_l, gid_l, o->seed, iteration, g->key, g->stream);
       } else {
         da.iteration = iteration;
-        init_keys_exp(n_l, gid_l, da, LearnedArgs{nullptr}, g->key, g->stream);
+        LearnedArgs la{nullptr};
+        if (o->learned) {
+          // scores of this level's vertices delay their deviates
+          TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
+          level_degrees(n_l, off_l, g->ldeg, g->stream);
+          level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
+          mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
+          la.delay = g->delay;
+        }
+        init_keys_exp(n_l, gid_l, da, la, g->key, g->stream);
       }
     }
     uint64_t sweeps = 0;
@@ -624,6 +641,13 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
       labels_radius(n_l, g->key, rec.lab, g->small + 4, !local, g->stream);
+    }
+    if (g->collect) {
+      // trace of the plain method: features and the log2 size of each vertex's cluster
+      level_degrees(n_l, off_l, g->ldeg, g->stream);
+      level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
+      cluster_targets(n_l, rec.lab, g->csize, g->target, g->stream);
+      mlp_append(g->collect, n_l, g->feat, g->target, g->collect_stride, g->stream);
     }
     // relabel: every edge slot gets the label of its far end (kNone inside a
     // cluster); the slots live in the next level's column buffer until it is built
@@ -732,6 +756,17 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   }
 }
 
+// Buffers of the learned centre proposals (level-0 sized, allocated once).
+void ensure_learned_buffers(kc_graph *g) {
+  if (g->feat) return;
+  const uint64_t n = g->n0;
+  g->feat = g->dalloc<float>(n * kMlpIn);
+  g->target = g->dalloc<float>(n);
+  g->ldeg = g->dalloc<uint32_t>(n);
+  g->csize = g->dalloc<uint32_t>(n);
+  g->delay = g->dalloc<uint8_t>(n);
+}
+
 // Min-label propagation (labels) or propagation + pointer jumping (shortcuts).
 void run_propagation(kc_graph *g, const kc_options *o, RunOut &out) {
   const BinCfg cfg = bin_cfg(o);
@@ -768,6 +803,7 @@ void kc_options_init(kc_options *o) {
   o->deviate_fma = 1;
   o->deviate_stream = KC_STREAM_REFERENCE;
   o->max_levels = 4096;
+  o->learned_alpha = 1.0f;
 }
 
 }  // extern "C"
@@ -1026,7 +1062,12 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
     else
       kc_options_init(&o);
     if (o.variant < KC_VARIANT_EXP || o.variant > KC_VARIANT_LABELS) fail(KC_ERR_INVALID, "unknown variant %d", o.variant);
-    if (o.learned) fail(KC_ERR_UNSUPPORTED, "learned centre proposals need kc_mlp_set first");
+    if (o.learned) {
+      if (o.variant != KC_VARIANT_EXP) fail(KC_ERR_UNSUPPORTED, "learned centre proposals apply to exponential contraction only");
+      if (!g->mlp) fail(KC_ERR_INVALID, "learned centre proposals need a model (kc_graph_set_mlp)");
+      if (!(o.learned_alpha >= 0.f) || o.learned_alpha > 64.f) fail(KC_ERR_INVALID, "learned_alpha must be in [0, 64]");
+      ensure_learned_buffers(g);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     g->have_labels = false;
     if (o.variant == KC_VARIANT_EXP) ensure_table(g, o.beta);
@@ -1064,6 +1105,49 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
       for (int i = 0; i < KC_KERNEL_CLASSES; ++i) tot += stats->kernel_bytes[i];
       stats->alg_bytes = tot;
     }
+  });
+}
+
+int kc_graph_set_mlp(kc_graph *g, kc_mlp *m) {
+  return guarded([&] {
+    if (!g) fail(KC_ERR_INVALID, "null graph");
+    if (m) KC_HIP(hipStreamSynchronize(m->stream));
+    g->mlp = m;
+  });
+}
+
+int kc_graph_features(kc_graph *g, float *features) {
+  return guarded([&] {
+    if (!g || !features) fail(KC_ERR_INVALID, "null argument");
+    if (!g->n0) return;
+    ensure_learned_buffers(g);
+    level_degrees(g->n0, g->off0c, g->ldeg, g->stream);
+    level_features(g->n0, g->off0c, g->col0c, g->ldeg, 0, g->feat, g->stream);
+    KC_HIP(hipMemcpyAsync(features, g->feat, g->n0 * kMlpIn * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    sync(g);
+  });
+}
+
+int kc_mlp_collect(kc_mlp *m, kc_graph *g, const kc_options *opt, uint64_t stride) {
+  return guarded([&] {
+    if (!m || !g) fail(KC_ERR_INVALID, "null argument");
+    kc_options o;
+    if (opt)
+      o = *opt;
+    else
+      kc_options_init(&o);
+    if (o.variant != KC_VARIANT_EXP || o.learned) fail(KC_ERR_INVALID, "traces come from the plain exponential contraction");
+    ensure_learned_buffers(g);
+    KC_HIP(hipStreamSynchronize(m->stream));
+    g->collect = m;
+    g->collect_stride = stride ? stride : 1;
+    struct Reset {
+      kc_graph *g;
+      ~Reset() { g->collect = nullptr; }
+    } reset{g};
+    const int rc = kc_find_components(g, &o, nullptr, nullptr);
+    if (rc != KC_OK) throw Failure{rc};
+    sync(g);
   });
 }
 

@@ -47,9 +47,10 @@ struct DeviateArgs {
   uint32_t k_max;
 };
 
-// Learned centre proposals: per-vertex deviate divisor in 1/256 units (256 = 1.0).
+// Learned centre proposals: per-vertex delay added to the deviate (the vertex
+// starts its wave that many rounds later).
 struct LearnedArgs {
-  const uint16_t *scale_q8;  // null: plain deviates
+  const uint8_t *delay;  // null: plain deviates
 };
 
 // Thresholds W_k (k = 1..k_max) of the truncated exponential deviate with

@@ -67,7 +67,7 @@ __global__ void k_init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, 
   if (v >= n) return;
   uint32_t g = gid ? gid[v] : static_cast<uint32_t>(v);
   uint32_t dev = exp_deviate(g, d.seed, d.iteration, d.rng_offset, d.fused != 0, d.stream, d.table, d.k_max);
-  if (la.scale_q8) dev = static_cast<uint32_t>((static_cast<uint64_t>(dev) * 256u) / la.scale_q8[v]);
+  if (la.delay) dev += la.delay[v];
   key[v] = (static_cast<uint64_t>(dev) << 32) | static_cast<uint32_t>(v);
 }
 

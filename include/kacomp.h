@@ -106,7 +106,7 @@ typedef struct kc_options {
   int32_t deviate_fma;  /* 1: canonical draw summed with a fused multiply-add, as
                            libstdc++ compiled with -O3 -march=native on FMA hardware;
                            0: separate multiply and add */
-  int32_t learned;      /* 1: centre proposals scored by the MLP loaded with kc_mlp_set */
+  int32_t learned;      /* 1: deviates delayed by the scores of the MLP set with kc_graph_set_mlp */
   int32_t record_trace; /* keep per-level sizes/keys for kc_trace_* */
   int32_t time_kernels; /* bracket every launch with HIP events (per-kernel stats) */
   int32_t max_levels;   /* safety cap on contraction levels (default 4096) */
@@ -114,7 +114,8 @@ typedef struct kc_options {
   int32_t sweep_group_mid;
   int32_t sweep_group_large;
   int32_t deviate_stream; /* KC_STREAM_REFERENCE (default) or KC_STREAM_HASHED */
-  int32_t reserved[6];
+  float learned_alpha;    /* learned: delay = round(learned_alpha * max(score, 0)) rounds (default 1) */
+  int32_t reserved[5];
 } kc_options;
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit
@@ -124,7 +125,7 @@ typedef struct kc_options {
 #define KC_STREAM_REFERENCE 0
 #define KC_STREAM_HASHED 1
 
-#define KC_KERNEL_CLASSES 15
+#define KC_KERNEL_CLASSES 16
 /* Kernel classes accounted in kc_stats.kernel_* (index = class). */
 #define KC_K_DEVIATES 0  /* deviate sampling + key init */
 #define KC_K_SWEEP 1     /* pull sweep (decomposition, base case) */
@@ -141,6 +142,7 @@ typedef struct kc_options {
 #define KC_K_SORT 12     /* radix sort of (label, vertex) pairs */
 #define KC_K_RELABEL 13  /* edge slots relabelled with their far end's cluster */
 #define KC_K_ROUND 14    /* time-stepped push round (decomposition) */
+#define KC_K_LEARNED 15  /* features and MLP scores of learned centre proposals */
 
 typedef struct kc_stats {
   double time_ms;       /* device time of the whole call (HIP events), labels resident on device */
@@ -280,6 +282,69 @@ int kc_batch_graph_stats(const kc_batch *b, uint32_t *levels, uint32_t *sweeps, 
 /* One-shot form: create, run with the default launch shape, destroy. */
 int kc_batch_find_components(const kc_batch_host *b, const kc_options *opt, uint32_t *labels_out,
                              kc_batch_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * Learned centre proposals.  No counterpart in the reference (its hook is the
+ * commented-out TIEBREAK_DEGREE weighting in RunContractionLP,
+ * lib/components/exponential_contraction.h:459-466).  A per-vertex MLP
+ * 6 -> 64 -> 64 -> 1 (bf16 weights, f32 accumulation) scores every vertex of a
+ * level from six features (log2(1+deg), log2(1+mean) and log2(1+max) of
+ * sampled neighbour degrees, log2 of a 2-hop size estimate, a sampled
+ * clustering rate, the round index); it is trained on traces of the plain
+ * method to predict log2 of the size of the cluster the vertex ends in.  With
+ * kc_options.learned the deviate of every vertex is delayed by
+ * round(learned_alpha * max(score, 0)) rounds.  Labels stay exact: only the
+ * choice of centres changes.
+ *
+ * Parameter order of kc_mlp_get_weights / set_weights / kc_mlp_gradients
+ * (KC_MLP_PARAMS floats): W1[6][64] (input-major), b1[64], W2[64][64]
+ * (hidden-1-major), b2[64], w3[64], b3. */
+#define KC_MLP_FEATURES 6
+#define KC_MLP_PARAMS 4673
+
+typedef struct kc_mlp kc_mlp;
+
+typedef struct kc_train_params {
+  int32_t steps;  /* Adam steps (default 400) */
+  int32_t batch;  /* samples per step (default 4096) */
+  float lr;       /* default 3e-3 */
+  float beta1, beta2, eps;
+  int32_t seed;   /* mini-batch sampling stream */
+} kc_train_params;
+
+typedef struct kc_train_stats {
+  double first_loss, last_loss; /* 0.5 * mean squared error of the first and last step */
+  double time_ms;
+  uint64_t samples;             /* dataset rows */
+  int32_t steps;
+} kc_train_stats;
+
+int kc_mlp_create(int32_t seed, kc_mlp **out); /* He-uniform initial weights from seed */
+int kc_mlp_destroy(kc_mlp *m);
+int kc_mlp_get_weights(const kc_mlp *m, float *w);
+int kc_mlp_set_weights(kc_mlp *m, const float *w);
+/* Scores of n feature rows (n x 6, row-major). */
+int kc_mlp_forward(kc_mlp *m, const float *features, uint64_t n, float *scores);
+/* Gradient of 0.5 * mean((score - target)^2) over the n rows, without an update. */
+int kc_mlp_gradients(kc_mlp *m, const float *features, const float *targets, uint64_t n, float *grads, float *loss);
+/* Runs the plain method on g (opt with learned = 0) and appends the features
+ * and targets of every contracted level (every stride-th vertex) to the
+ * model's dataset. */
+int kc_mlp_collect(kc_mlp *m, kc_graph *g, const kc_options *opt, uint64_t stride);
+int kc_mlp_dataset_size(const kc_mlp *m, uint64_t *n);
+/* Copies dataset rows [first, first + count): features (count x 6) and targets. */
+int kc_mlp_get_dataset(const kc_mlp *m, uint64_t first, uint64_t count, float *features, float *targets);
+int kc_mlp_clear_dataset(kc_mlp *m);
+/* Appends n rows of caller features (n x 6) and targets to the dataset. */
+int kc_mlp_add_dataset(kc_mlp *m, const float *features, const float *targets, uint64_t n);
+void kc_train_params_init(kc_train_params *p);
+/* Adam on the dataset with the project's MFMA forward and backward kernels. */
+int kc_mlp_train(kc_mlp *m, const kc_train_params *p, kc_train_stats *stats);
+/* Model used by kc_find_components on g when kc_options.learned is set (NULL detaches). */
+int kc_graph_set_mlp(kc_graph *g, kc_mlp *m);
+/* Features of the level-0 vertices of g (its vertices of degree > 0 in id
+ * order), non_isolated x 6 floats. */
+int kc_graph_features(kc_graph *g, float *features);
 
 /* Host I/O and generators. */
 int kc_read_metis(const char *path, kc_csr *out);

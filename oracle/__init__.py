@@ -42,6 +42,9 @@ def lib():
             "or_total_sweeps": (u64, [ctypes.c_void_p]),
             "or_seconds": (dbl, [ctypes.c_void_p]),
             "or_uniform_deviate_100": (u32, [u32, i32, u32]),
+            "or_mlp_forward": (None, [_P(dbl), _P(ctypes.c_float), u64, i32, _P(dbl)]),
+            "or_mlp_grad": (None, [_P(dbl), _P(ctypes.c_float), _P(ctypes.c_float), u64, i32, _P(dbl), _P(dbl)]),
+            "or_level_features": (None, [u64, _P(u64), _P(u32), u32, _P(ctypes.c_float)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -141,3 +144,38 @@ class Run:
             lib().or_free(self.h)
         except Exception:
             pass
+
+
+MLP_PARAMS = 4673
+
+
+def mlp_forward(w, X, bf16=True):
+    """Scores of the rows of X (n x 6) for parameters w (public order), FP64."""
+    w = np.ascontiguousarray(w, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.empty(max(len(X), 1), np.float64)
+    lib().or_mlp_forward(w.ctypes.data_as(_P(ctypes.c_double)), X.ctypes.data_as(_P(ctypes.c_float)), len(X),
+                         1 if bf16 else 0, out.ctypes.data_as(_P(ctypes.c_double)))
+    return out[: len(X)]
+
+
+def mlp_grad(w, X, y, bf16=True):
+    """(gradient in public order, loss) of 0.5 * mean((score - y)^2), FP64."""
+    w = np.ascontiguousarray(w, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.empty(MLP_PARAMS, np.float64)
+    loss = ctypes.c_double()
+    lib().or_mlp_grad(w.ctypes.data_as(_P(ctypes.c_double)), X.ctypes.data_as(_P(ctypes.c_float)),
+                      y.ctypes.data_as(_P(ctypes.c_float)), len(X), 1 if bf16 else 0,
+                      g.ctypes.data_as(_P(ctypes.c_double)), ctypes.byref(loss))
+    return g, loss.value
+
+
+def level_features(offsets, targets, round_index=0):
+    off, tgt = _arrs(offsets, targets)
+    n = len(off) - 1
+    X = np.empty((max(n, 1), 6), np.float32)
+    lib().or_level_features(n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
+                            round_index, X.ctypes.data_as(_P(ctypes.c_float)))
+    return X[:n]

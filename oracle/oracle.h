@@ -16,6 +16,11 @@
  *   or_run (EXP/LOCAL)    labels against or_canonical_labels; per-level sizes and
  *                         keys against independent recomputation in the tests
  *   or_run (LABELS/SHORTCUT) labels against or_canonical_labels
+ *   or_mlp_grad (FP64)    central finite differences of its own loss
+ *   or_mlp_* (bf16 points) the same network with the device's bf16 rounding;
+ *                         pinned through the FP64 form (rounding a network whose
+ *                         weights are already bf16 values changes nothing else)
+ *   or_level_features     hand-computed features of small graphs
  */
 #ifndef KC_ORACLE_H_
 #define KC_ORACLE_H_
@@ -65,6 +70,14 @@ double or_seconds(const or_result *r);
 uint32_t or_hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta);
 /* Uniform deviate in [0,99] used by local contraction (SplitMix64). */
 uint32_t or_uniform_deviate_100(uint32_t gid, int32_t seed, uint32_t iteration);
+
+/* Learned centre proposals (no reference counterpart).  w: KC_ORACLE_MLP_PARAMS
+ * parameters in the library's public order (W1[6][64], b1, W2[64][64], b2, w3, b3). */
+#define KC_ORACLE_MLP_PARAMS 4673
+void or_mlp_forward(const double *w, const float *X, uint64_t n, int32_t bf16_points, double *scores);
+void or_mlp_grad(const double *w, const float *X, const float *y, uint64_t n, int32_t bf16_points, double *grads,
+                 double *loss);
+void or_level_features(uint64_t n, const uint64_t *off, const uint32_t *tgt, uint32_t round, float *X);
 
 #ifdef __cplusplus
 }
