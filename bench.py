@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--smax", type=int, default=0)
     ap.add_argument("--wmax", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
+    ap.add_argument("--learned-rhg-log-n", type=int, default=24)
+    ap.add_argument("--learned-rmat-scale", type=int, default=25)
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (HBM counter calibration) and exit")
@@ -174,6 +177,8 @@ def main():
     dg.close()
     if not a.no_batched:
         out["batched"] = batched_leg(a, stream)
+    if not a.no_learned:
+        out["learned"] = learned_leg(a, stream, run_kw)
     print(json.dumps(out), flush=True)
 
 
@@ -236,6 +241,92 @@ def batched_leg(a, stream):
                             "levels": plain[-1]["levels"], "sweeps": plain[-1]["sweeps"],
                             "schedule": "rounds", "seq_limit": seq},
             "generation_s": round(gen_s, 1)}
+
+
+def learned_leg(a, stream, run_kw):
+    """Learned centre proposals: the MLP is trained on traces of the plain
+    method on the plain workloads (G(n,m) 2^20/2^22, percolated 3D grid 128^3
+    p = 0.3, 2D geometric 2^20 at average degree 8, R-MAT scale 22), then the
+    plain and learned forms run at the same seeds on the hyperbolic graph
+    n = 2^24, gamma 2.8, average degree 16, and on R-MAT scale 25."""
+    import numpy as np
+
+    import components_amd as ca
+    import oracle
+
+    t0 = time.time()
+    mlp = ca.Mlp(seed=1)
+    train = [("gnm_undirected", dict(n=1 << 20, m=1 << 22, seed=1), 16),
+             ("grid_3d", dict(nx=128, ny=128, nz=128, p=0.3, seed=2), 16),
+             ("rmat", dict(n=1 << 22, m=16 << 22, seed=3, rmat_a=0.57, rmat_b=0.19, rmat_c=0.19), 16)]
+    for name, kw, stride in train:
+        g = ca.DeviceGraph.generate(name, **kw)
+        mlp.collect(g, stride=stride, seed=11, seq_limit=0, stream=stream)
+        g.close()
+    rgg = ca.generate("rgg_2d", n=1 << 20, r=float(np.sqrt(8 / (np.pi * (1 << 20)))), seed=2)
+    g = ca.DeviceGraph(rgg)
+    mlp.collect(g, stride=16, seed=11, seq_limit=0, stream=stream)
+    g.close()
+    tr = mlp.train(steps=600, batch=8192, lr=3e-3)
+    X, y = mlp.dataset()
+    const = 0.5 * float(np.var(y.astype(np.float64)))
+    _, full_loss = mlp.gradients(X, y)
+    del X, y
+    train_s = time.time() - t0
+    print(f"[bench] learned: {tr['samples']} trace rows, loss {full_loss:.3f} (constant {const:.3f}), "
+          f"{train_s:.1f}s", file=sys.stderr, flush=True)
+    evals = {}
+    graphs = [("rhg-n2^%d-gamma2.8-deg16-seed5" % a.learned_rhg_log_n, "host", "rhg",
+               dict(n=1 << a.learned_rhg_log_n, gamma=2.8, d=16.0, seed=5)),
+              ("rmat-scale%d-ef16-seed3" % a.learned_rmat_scale, "device", "rmat",
+               dict(n=1 << a.learned_rmat_scale, m=16 << a.learned_rmat_scale, seed=3, rmat_a=0.57, rmat_b=0.19,
+                    rmat_c=0.19))]
+    for label, where, name, kw in graphs:
+        t1 = time.time()
+        if where == "host":
+            host = ca.generate(name, **kw)
+            dg = ca.DeviceGraph(host)
+        else:
+            host = None
+            dg = ca.DeviceGraph.generate(name, **kw)
+        dg.set_mlp(mlp)
+        m = dg.m
+        seq = max(m // 256, 4096)
+        res = {"n": dg.n, "m": m}
+        labels = {}
+        for form, learned in (("plain", False), ("learned", True)):
+            kw2 = dict(run_kw, seq_limit=seq, learned=learned)
+            for i in range(a.warmup):
+                dg.find_components(copy_labels=False, seed=1 + i + 1000, **kw2)
+            sts = []
+            t2 = time.perf_counter()
+            for i in range(a.steps):
+                sts.append(dg.find_components(copy_labels=False, seed=1 + i + 1000, **kw2)[1])
+            ms = (time.perf_counter() - t2) * 1e3 / a.steps
+            labels[form] = dg.find_components(seed=1001, **kw2)[0]
+            res[form] = {"value": round(m / (ms / 1e3) / 1e9, 4), "unit": "GTEPS", "ms_per_step": round(ms, 4),
+                         "levels": float(np.mean([s["levels"] for s in sts])),
+                         "sweeps": float(np.mean([s["sweeps"] for s in sts])),
+                         "radius_sum": float(np.mean([s["radius_sum"] for s in sts]))}
+        if host is not None:
+            ref = oracle.canonical_labels(host.offsets, host.targets)
+            res["labels_bit_exact"] = bool(np.array_equal(labels["plain"], ref) and
+                                           np.array_equal(labels["learned"], ref))
+            del ref
+        else:
+            res["labels_bit_exact"] = bool(np.array_equal(labels["plain"], labels["learned"]))
+            res["labels_note"] = "learned labels equal the plain path's (checked against the oracle elsewhere)"
+        dg.close()
+        evals[label] = res
+        print(f"[bench] learned: {label}: plain {res['plain']['ms_per_step']:.2f} ms {res['plain']['levels']} levels, "
+              f"learned {res['learned']['ms_per_step']:.2f} ms {res['learned']['levels']} levels "
+              f"({time.time() - t1:.1f}s)", file=sys.stderr, flush=True)
+    mlp.close()
+    return {"model": "MLP 6-64-64-1, bf16 weights, f32 accumulation (MFMA 16x16x16 bf16)",
+            "training": {"trace_rows": tr["samples"], "steps": tr["steps"], "batch": 8192,
+                         "full_data_loss": round(full_loss, 4), "constant_predictor_loss": round(const, 4),
+                         "seconds": round(train_s, 1)},
+            "alpha": 1.0, "eval": evals}
 
 
 def cpu_leg(a, dg, run_kw, stream):
