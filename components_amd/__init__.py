@@ -125,6 +125,8 @@ _SIGNATURES = {
     "kc_graph_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
     "kc_find_components": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(ctypes.c_uint32), _P(Stats)]),
     "kc_copy_labels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
+    "kc_component_histogram": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_uint64,
+                                              _P(ctypes.c_uint64)]),
     "kc_trace_levels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
     "kc_trace_level_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint64)]),
     "kc_trace_level_keys": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint64), _P(ctypes.c_uint32)]),
@@ -356,6 +358,14 @@ class DeviceGraph:
         labels = np.empty(max(self.n, 1), np.uint32)
         _check(lib().kc_copy_labels(self.h, _u32p(labels)))
         return labels[: self.n]
+
+    def component_histogram(self):
+        """(size, number of components of that size) pairs of the last run, increasing size."""
+        np_ = ctypes.c_uint64()
+        _check(lib().kc_component_histogram(self.h, None, 0, ctypes.byref(np_)))
+        out = np.empty(2 * max(np_.value, 1), np.uint64)
+        _check(lib().kc_component_histogram(self.h, _u64p(out), np_.value, ctypes.byref(np_)))
+        return out[: 2 * np_.value].reshape(-1, 2)
 
     def set_mlp(self, mlp):
         """Model used when find_components(learned=True)."""

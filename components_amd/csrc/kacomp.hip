@@ -1108,6 +1108,29 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
   });
 }
 
+int kc_component_histogram(kc_graph *g, uint64_t *pairs, uint64_t capacity, uint64_t *num_pairs) {
+  return guarded([&] {
+    if (!g || !num_pairs) fail(KC_ERR_INVALID, "null argument");
+    if (!g->have_labels) fail(KC_ERR_INVALID, "no labels: kc_find_components has not completed");
+    const uint64_t n = g->n;
+    // distinct sizes k satisfy k (k + 1) / 2 <= n
+    uint64_t kmax = 1;
+    while (kmax * (kmax + 1) / 2 <= n) ++kmax;
+    uint32_t *hist = g->arena.alloc_n<uint32_t>(n + 1);
+    uint64_t *dpairs = g->arena.alloc_n<uint64_t>(2 * kmax + 2);
+    component_histogram(n, g->labels, g->minid, hist, dpairs, g->sizes + 10, g->scan_scratch, g->stream);
+    d2h(g, g->h_sizes + 10, g->sizes + 10, sizeof(uint64_t));
+    sync(g);
+    const uint64_t np = g->h_sizes[10];
+    if (np > kmax) fail(KC_ERR_INVALID, "internal: %llu component sizes", (unsigned long long)np);
+    *num_pairs = np;
+    if (pairs && capacity) {
+      const uint64_t take = np < capacity ? np : capacity;
+      if (take) KC_HIP(hipMemcpy(pairs, dpairs, 16 * take, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
 int kc_graph_set_mlp(kc_graph *g, kc_mlp *m) {
   return guarded([&] {
     if (!g) fail(KC_ERR_INVALID, "null graph");

@@ -180,6 +180,11 @@ void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid, const uin
 void drop_isolated(uint64_t n, const uint64_t *off, const uint32_t *col, uint64_t m2, uint32_t *newid, uint32_t *gid0,
                    uint64_t *off_c, uint32_t *col_c, uint64_t *n_out, uint64_t *scratch, hipStream_t s);
 void identity_u32(uint64_t n, uint32_t *p, hipStream_t s);
+// (size, number of components of that size) pairs of a canonical labelling in
+// increasing size; count and hist are n and n + 1 words of scratch, pairs
+// holds 2 * (distinct sizes) words, *num_pairs the number of pairs.
+void component_histogram(uint64_t n, const uint32_t *labels, uint32_t *count, uint32_t *hist, uint64_t *pairs,
+                         uint64_t *num_pairs, uint64_t *scratch, hipStream_t s);
 void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, uint32_t *partial,
                       hipStream_t s);
 

@@ -1314,6 +1314,56 @@ void drop_isolated(uint64_t n, const uint64_t *off, const uint32_t *col, uint64_
   }
 }
 
+// Component-size histogram: count[label]++ over the vertices, then
+// hist[count[c]]++ over the labels with a count, then the non-zero entries of
+// hist as (size, number) pairs in increasing size.
+__global__ void k_count_labels(uint64_t n, const uint32_t *labels, uint32_t *count) {
+  for (uint64_t base = blockIdx.x * static_cast<uint64_t>(blockDim.x); base < n;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t v = base + threadIdx.x;
+    const bool live = v < n;
+    peeled_fetch_add(count, live ? labels[v] : 0u, 1u, live);
+  }
+}
+__global__ void k_size_hist(uint64_t n, const uint32_t *count, uint32_t *hist) {
+  for (uint64_t base = blockIdx.x * static_cast<uint64_t>(blockDim.x); base < n;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t v = base + threadIdx.x;
+    const uint32_t c = v < n ? count[v] : 0u;
+    peeled_fetch_add(hist, c, 1u, c != 0);
+  }
+}
+struct HistLoad {
+  const uint32_t *hist;
+  __device__ uint64_t operator()(uint64_t i) const { return hist[i] ? 1ull : 0ull; }
+};
+struct HistStore {
+  const uint32_t *hist;
+  uint64_t *pairs;
+  __device__ void operator()(uint64_t i, uint64_t prefix, uint64_t keep) const {
+    if (keep) {
+      pairs[2 * prefix] = i;
+      pairs[2 * prefix + 1] = hist[i];
+    }
+  }
+};
+
+void component_histogram(uint64_t n, const uint32_t *labels, uint32_t *count, uint32_t *hist, uint64_t *pairs,
+                         uint64_t *num_pairs, uint64_t *scratch, hipStream_t s) {
+  fill_u32(count, n, 0u, s);
+  fill_u32(hist, n + 1, 0u, s);
+  if (!n) {
+    fill_u64(num_pairs, 1, 0ull, s);
+    return;
+  }
+  const unsigned grid = grid_for(n) > 65536 ? 65536 : grid_for(n);
+  hipLaunchKernelGGL(k_count_labels, dim3(grid), dim3(kBlock), 0, s, n, labels, count);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_size_hist, dim3(grid), dim3(kBlock), 0, s, n, count, hist);
+  KC_LAUNCH_CHECK();
+  scan_exclusive<uint64_t>(n + 1, HistLoad{hist}, HistStore{hist, pairs}, num_pairs, scratch, s);
+}
+
 void identity_u32(uint64_t n, uint32_t *p, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_identity, dim3(grid_for(n) > 65536 ? 65536 : grid_for(n)), dim3(kBlock), 0, s, n, p);

@@ -266,6 +266,26 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner) {
     if (c.print_verbose)
       std::printf("ITERATION i=%u seed=%d time=%g device_ms=%g levels=%u sweeps=%llu components=%u\n", i, c.seed, secs,
                   st.time_ms, st.levels, static_cast<unsigned long long>(st.sweeps), st.components);
+    if (variant == KC_VARIANT_LABELS || variant == KC_VARIANT_LOCAL) {
+      // OutputComponents (lib/data_structures/static_graph.h:504-591), called by the
+      // labels and local_contraction programs after every run; its vertex loop
+      // skips the inactive degree-0 vertices, so size-1 components are not listed
+      uint64_t np = 0;
+      std::vector<uint64_t> pairs;
+      if (kc_component_histogram(dg, nullptr, 0, &np) == KC_OK) {
+        pairs.resize(2 * np + 2);
+        kc_component_histogram(dg, pairs.data(), np, &np);
+      }
+      uint64_t total = 0;
+      std::printf("COMPONENTS [ ");
+      for (uint64_t k = 0; k < np; ++k) {
+        if (pairs[2 * k] == 1) continue;
+        std::printf("size=%llu (num=%llu) ", static_cast<unsigned long long>(pairs[2 * k]),
+                    static_cast<unsigned long long>(pairs[2 * k + 1]));
+        total += pairs[2 * k + 1];
+      }
+      std::printf("]\nNUM COMPONENTS %llu\n", static_cast<unsigned long long>(total));
+    }
     if (c.check && i + 1 == c.iterations) {
       std::vector<uint32_t> ref = HostCanonicalLabels(g);
       const bool ok = ref == labels;

@@ -182,6 +182,13 @@ int kc_graph_download(const kc_graph *g, kc_csr *out);
  * labels then stay on the device (kc_copy_labels fetches them). */
 int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out, kc_stats *stats);
 int kc_copy_labels(kc_graph *g, uint32_t *labels_out);
+/* Component-size histogram of the last kc_find_components call, computed on
+ * the device (StaticGraph::OutputComponents, lib/data_structures/static_graph.h:504-591):
+ * (size, number of components of that size) pairs in increasing size, every
+ * vertex counted (a degree-0 vertex is a component of size 1).  Writes
+ * min(capacity, *num_pairs) pairs (2 words each) to pairs; pairs may be NULL
+ * to query *num_pairs. */
+int kc_component_histogram(kc_graph *g, uint64_t *pairs, uint64_t capacity, uint64_t *num_pairs);
 
 /* Per-level trace of the last kc_find_components call with record_trace=1.
  * level_sizes: n_l, m_l (undirected), sweeps_l, radius_l for level l. */

@@ -2,6 +2,7 @@
 import itertools
 import json
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -257,3 +258,48 @@ def test_device_generator_matches_host(name, kw):
     lab, _ = dg.find_components(seed=1001)
     assert np.array_equal(lab, oracle.canonical_labels(host.offsets, host.targets))
     dg.close()
+
+
+def histogram_of(labels):
+    _, counts = np.unique(labels, return_counts=True)
+    sizes, num = np.unique(counts, return_counts=True)
+    return np.column_stack([sizes, num]).astype(np.uint64)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_component_histogram(variant):
+    rng = np.random.default_rng(17)
+    g = union([rand_graph(rng, 3000, 2400), ca.generate("rmat", n=1 << 12, m=1 << 14, seed=3),
+               ca.Graph(np.zeros(6, np.uint64), np.zeros(0, np.uint32))])
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    dg = ca.DeviceGraph(g)
+    try:
+        dg.find_components(variant=variant, seed=4)
+        h = dg.component_histogram()
+    finally:
+        dg.close()
+    want = histogram_of(ref)
+    assert np.array_equal(h, want)
+    assert int((h[:, 0] * h[:, 1]).sum()) == g.n
+
+
+def test_app_prints_component_summary(tmp_path):
+    """labels and local_contraction print OutputComponents' lines after each
+    run: sizes other than 1 (degree-0 vertices are not listed) and their count."""
+    rng = np.random.default_rng(3)
+    g = union([rand_graph(rng, 400, 300), ca.Graph.from_edges(5, [(0, 1), (2, 3)])])
+    p = tmp_path / "g.graph"
+    ca.write_metis(p, g)
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    want = histogram_of(ref)
+    want = want[want[:, 0] != 1]
+    for prog in ("labels", "local_contraction"):
+        r = subprocess.run([os.path.join(ca.BIN_DIR, prog), "-input", "file", "-in", str(p), "-i", "2", "-seed", "3"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = [l for l in r.stdout.splitlines() if l.startswith("COMPONENTS [")]
+        assert len(lines) == 2, r.stdout
+        pairs = re.findall(r"size=(\d+) \(num=(\d+)\)", lines[0])
+        assert [(int(a), int(b)) for a, b in pairs] == [(int(a), int(b)) for a, b in want]
+        num = [l for l in r.stdout.splitlines() if l.startswith("NUM COMPONENTS")]
+        assert num and int(num[0].split()[-1]) == int(want[:, 1].sum())
