@@ -91,7 +91,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding is a design "
-              "in DESIGN.md, not a measured path", file=sys.stderr)
+              "only, not a measured path", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
@@ -330,8 +330,8 @@ def learned_leg(a, stream, run_kw):
 
 
 def cpu_leg(a, dg, run_kw, stream):
-    """The oracle as checker (union-find labels of the full workload against the
-    GPU labels) and as the CPU baseline (its exponential contraction timed on a
+    """The oracle to check the GPU labels (its union-find labels of the full
+    workload) and as the CPU baseline (its exponential contraction timed on a
     bounded sample of the same workload, R-MAT at --cpu-scale)."""
     import numpy as np
 

@@ -36,7 +36,7 @@ struct Config {
   uint64_t gen_n = 0, gen_m = 0, gen_z = 0;
   float gen_r = 0.001f, gen_gamma = 2.2f, gen_d = 16.0f, gen_p = 1.0f;
   bool gen_periodic = false;
-  // additions of this build (no reference counterpart)
+  // additions without a reference counterpart
   double rmat_a = 0.57, rmat_b = 0.19, rmat_c = 0.19;
   bool nofma = false, time_kernels = false;
   int tmax = 0, smax = 0, wmax = 0;

@@ -1,7 +1,7 @@
 """ctypes view of the CPU oracle (oracle/build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker and the CPU baseline.  The product
+bench.py's cpu_baseline leg, to check results and as the CPU baseline.  The product
 path never imports it.
 """
 from __future__ import annotations

@@ -1,8 +1,8 @@
 /*
  * oracle.h -- CPU restatement of the reference's connected-components
  * contraction path.  TEST INFRASTRUCTURE ONLY: it is linked or called by
- * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, as
- * the checker and as the CPU baseline; the product (libkacomp) never calls it.
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, to
+ * check results and as the CPU baseline; the product (libkacomp) never calls it.
  *
  * Pinned by (tests/test_oracle.py):
  *   or_canonical_labels   closed form (min id per component) against brute force
