@@ -181,6 +181,7 @@ using namespace kc;
 // Graph handle.
 struct LevelRec {
   bool stall = false;  // no key changed: the contraction is the identity and is skipped
+  const uint32_t *idx = nullptr;  // level stored in another order: position -> level index
   uint64_t n = 0, m2 = 0;
   uint32_t *lab = nullptr, *newidx = nullptr, *gid = nullptr;
   uint64_t sweeps = 0;
@@ -202,6 +203,12 @@ struct kc_graph {
   uint32_t *gid0 = nullptr;
   uint64_t *off0c = nullptr;
   uint32_t *col0c = nullptr;
+  // the same level 0 stored in degree-descending order (hub keys stay in
+  // cache): idx0 position -> level index, pos0 its inverse, gidp0 position ->
+  // input id, off0p/col0p the CSR with positions (null when not built)
+  uint32_t *idx0 = nullptr, *pos0 = nullptr, *gidp0 = nullptr;
+  uint64_t *off0p = nullptr;
+  uint32_t *col0p = nullptr;
   // work buffers sized for level 0
   uint64_t *key = nullptr, *key2 = nullptr;
   uint32_t *deg_inter = nullptr, *lists = nullptr, *hub_inter = nullptr;
@@ -277,6 +284,7 @@ struct kc_graph {
 namespace {
 
 constexpr uint32_t kMaxFlags = 1u << 16;
+constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep the input order
 constexpr uint32_t kMinWaveBin = 256;
 
 
@@ -525,6 +533,20 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   const uint64_t *off_l = g->off0c;
   const uint32_t *col_l = g->col0c;
   uint32_t *gid_l = g->gid0;
+  // Level 0 in degree order when built (and the level is not already the base
+  // case, whose pointer jumping indexes keys by level index): keys and rows by
+  // storage position, key labels and gid_l by level index as before.
+  const bool ordered = g->idx0 && !o->input_order && !(o->seq_limit > 0 && m2_l / 2 <= o->seq_limit) && m2_l > 0;
+  const uint32_t *idx_l = nullptr, *pos_l = nullptr;  // storage maps of the current level (null: identity)
+  const uint32_t *dev_gid_l = gid_l;                   // input id of each storage position
+  if (ordered) {
+    off_l = g->off0p;
+    col_l = g->col0p;
+    idx_l = g->idx0;
+    pos_l = g->pos0;
+    dev_gid_l = g->gidp0;
+  }
+  const uint32_t *canon_gid = ordered ? g->gidp0 : g->gid0;  // input id of each level-0 storage position
   uint32_t iteration = 1;
   uint32_t flag_base = 0;
   int pp = 0;
@@ -572,7 +594,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     {
       TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l + (gid_l ? 4 * n_l : 0));
       if (local) {
-        init_keys_uniform(n_l, gid_l, o->seed, iteration, g->key, g->stream);
+        init_keys_uniform(n_l, dev_gid_l, idx_l, o->seed, iteration, g->key, g->stream);
       } else {
         da.iteration = iteration;
         LearnedArgs la{nullptr};
@@ -584,7 +606,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
           mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
           la.delay = g->delay;
         }
-        init_keys_exp(n_l, gid_l, da, la, g->key, g->stream);
+        init_keys_exp(n_l, dev_gid_l, idx_l, da, la, g->key, g->stream);
       }
     }
     uint64_t sweeps = 0;
@@ -612,6 +634,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       // iteration draws new deviates on the same graph.
       LevelRec rec;
       rec.stall = true;
+      rec.idx = idx_l;
       rec.n = n_l;
       rec.m2 = m2_l;
       rec.gid = gid_l;
@@ -626,6 +649,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       continue;
     }
     LevelRec rec;
+    rec.idx = idx_l;
     rec.n = n_l;
     rec.m2 = m2_l;
     rec.gid = gid_l;
@@ -640,7 +664,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     fill_u32(g->small + 4, 1, 0, g->stream);
     {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
-      labels_radius(n_l, g->key, rec.lab, g->small + 4, !local, g->stream);
+      labels_radius(n_l, g->key, pos_l, rec.lab, g->small + 4, !local, g->stream);
     }
     if (g->collect) {
       // trace of the plain method: features and the log2 size of each vertex's cluster
@@ -722,6 +746,9 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     off_l = off_next;
     col_l = col_next;
     gid_l = gid_next;
+    dev_gid_l = gid_next;
+    idx_l = nullptr;
+    pos_l = nullptr;
     pp ^= 1;
     bins_valid = false;
     ++iteration;
@@ -743,15 +770,15 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   }
   // canonical labels: minimum vertex id per component
   const uint64_t n0 = g->n0;
-  const uint64_t gid_bytes = g->gid0 ? 4 * n0 : 0;
+  const uint64_t gid_bytes = canon_gid ? 4 * n0 : 0;
   fill_u32(g->minid, g->n, kNone, g->stream);
   {
     TimedScope ts(g->timer, KC_K_CANON, 8 * n0 + gid_bytes);
-    canon_min(n0, rep_up, g->gid0, g->minid, g->stream);
+    canon_min(n0, rep_up, canon_gid, g->minid, g->stream);
   }
   {
     TimedScope ts(g->timer, KC_K_CANON, 12 * n0 + gid_bytes);
-    canon_label(n0, rep_up, g->gid0, g->minid, g->labels, g->small + 5, reinterpret_cast<uint32_t *>(g->scan_scratch),
+    canon_label(n0, rep_up, canon_gid, g->minid, g->labels, g->small + 5, reinterpret_cast<uint32_t *>(g->scan_scratch),
                 g->stream);
   }
 }
@@ -882,6 +909,17 @@ void init_work(kc_graph *g) {
     if (g->n0 != g->non_isolated)
       fail(KC_ERR_INVALID, "internal: %llu vertices of degree > 0, expected %llu", (unsigned long long)g->n0,
            (unsigned long long)g->non_isolated);
+  }
+  if (g->n0 >= kOrderMinVertices) {
+    const uint64_t n0 = g->n0;
+    g->idx0 = g->dalloc<uint32_t>(n0);
+    g->pos0 = g->dalloc<uint32_t>(n0);
+    g->gidp0 = g->dalloc<uint32_t>(n0);
+    g->off0p = g->dalloc<uint64_t>(n0 + 1);
+    g->col0p = g->dalloc<uint32_t>(m2);
+    level0_order(n0, g->off0c, g->col0c, g->gid0, g->idx0, g->pos0, g->gidp0, g->off0p, g->col0p, g->sk, g->sv, g->sk2,
+                 g->sv2, g->sort_scratch, g->scan_scratch, g->stream);
+    sync(g);
   }
   KC_HIP(hipDeviceSynchronize());
 }
@@ -1209,6 +1247,12 @@ int kc_trace_level_keys(const kc_graph *g, uint32_t level, uint64_t *keys_out, u
     if (keys_out) {
       if (!r.keys_copy) fail(KC_ERR_INVALID, "run without record_trace");
       KC_HIP(hipMemcpy(keys_out, r.keys_copy, r.n * 8, hipMemcpyDeviceToHost));
+      if (r.idx) {  // stored in another order: back to level-index order
+        std::vector<uint64_t> byp(keys_out, keys_out + r.n);
+        std::vector<uint32_t> idx(r.n);
+        KC_HIP(hipMemcpy(idx.data(), r.idx, r.n * 4, hipMemcpyDeviceToHost));
+        for (uint64_t p = 0; p < r.n; ++p) keys_out[idx[p]] = byp[p];
+      }
     }
     if (gid_out) {
       if (r.gid) {

@@ -60,10 +60,11 @@ std::vector<double> build_deviate_table(double beta);
 void fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
 
-void init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, LearnedArgs la, uint64_t *key,
+void init_keys_exp(uint64_t n, const uint32_t *gid, const uint32_t *idx, DeviateArgs d, LearnedArgs la, uint64_t *key,
                    hipStream_t s);
 void init_keys_index(uint64_t n, uint64_t *key, hipStream_t s);
-void init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t iteration, uint64_t *key,
+void init_keys_uniform(uint64_t n, const uint32_t *gid, const uint32_t *idx, int32_t seed, uint32_t iteration,
+                       uint64_t *key,
                        hipStream_t s);
 void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, hipStream_t s);
 
@@ -94,7 +95,8 @@ void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uin
 void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
 
 // lab[v] = low half of key[v]; max cluster radius (atomicMax into *radius).
-void labels_radius(uint64_t n, const uint64_t *key, uint32_t *lab, uint32_t *radius, bool with_radius,
+void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
+                   bool with_radius,
                    hipStream_t s);
 // (label, vertex) of every vertex with a cluster edge; *count (device) = N.
 void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *sk, uint32_t *sv,
@@ -180,6 +182,13 @@ void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid, const uin
 void drop_isolated(uint64_t n, const uint64_t *off, const uint32_t *col, uint64_t m2, uint32_t *newid, uint32_t *gid0,
                    uint64_t *off_c, uint32_t *col_c, uint64_t *n_out, uint64_t *scratch, hipStream_t s);
 void identity_u32(uint64_t n, uint32_t *p, hipStream_t s);
+// Level 0 in degree-descending storage order (ties in index order): idx (position
+// -> level index), pos (its inverse), gidp (position -> input id; gid null =
+// identity), offp/colp (the CSR in storage order, neighbours as positions).
+// k0/v0/k1/v1 are n-word sort buffers.
+void level0_order(uint64_t n, const uint64_t *off, const uint32_t *col, const uint32_t *gid, uint32_t *idx,
+                  uint32_t *pos, uint32_t *gidp, uint64_t *offp, uint32_t *colp, uint32_t *k0, uint32_t *v0,
+                  uint32_t *k1, uint32_t *v1, uint64_t *sort_scratch, uint64_t *scan_scratch, hipStream_t s);
 // (size, number of components of that size) pairs of a canonical labelling in
 // increasing size; count and hist are n and n + 1 words of scratch, pairs
 // holds 2 * (distinct sizes) words, *num_pairs the number of pairs.

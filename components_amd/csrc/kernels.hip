@@ -61,14 +61,16 @@ __global__ void k_fill(T *p, uint64_t n, T v) {
 
 // ---------------------------------------------------------------------------
 // Keys.
-__global__ void k_init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, LearnedArgs la,
+// Key of the vertex stored at v: (deviate << 32) | level index, where the
+// level index is idx[v] when the level is stored in another order (null: v).
+__global__ void k_init_keys_exp(uint64_t n, const uint32_t *gid, const uint32_t *idx, DeviateArgs d, LearnedArgs la,
                                 uint64_t *key) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (v >= n) return;
   uint32_t g = gid ? gid[v] : static_cast<uint32_t>(v);
   uint32_t dev = exp_deviate(g, d.seed, d.iteration, d.rng_offset, d.fused != 0, d.stream, d.table, d.k_max);
   if (la.delay) dev += la.delay[v];
-  key[v] = (static_cast<uint64_t>(dev) << 32) | static_cast<uint32_t>(v);
+  key[v] = (static_cast<uint64_t>(dev) << 32) | (idx ? idx[v] : static_cast<uint32_t>(v));
 }
 
 __global__ void k_init_keys_index(uint64_t n, uint64_t *key) {
@@ -76,11 +78,12 @@ __global__ void k_init_keys_index(uint64_t n, uint64_t *key) {
   if (v < n) key[v] = static_cast<uint32_t>(v);
 }
 
-__global__ void k_init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t it, uint64_t *key) {
+__global__ void k_init_keys_uniform(uint64_t n, const uint32_t *gid, const uint32_t *idx, int32_t seed, uint32_t it,
+                                    uint64_t *key) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (v >= n) return;
   uint32_t g = gid ? gid[v] : static_cast<uint32_t>(v);
-  key[v] = (static_cast<uint64_t>(uniform_deviate_100(g, seed, it)) << 32) | static_cast<uint32_t>(v);
+  key[v] = (static_cast<uint64_t>(uniform_deviate_100(g, seed, it)) << 32) | (idx ? idx[v] : static_cast<uint32_t>(v));
 }
 
 __global__ void k_deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out) {
@@ -436,14 +439,17 @@ __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
 
 // ---------------------------------------------------------------------------
 // Centre labels and the maximum cluster radius (hops from the centre).
-__global__ void k_labels_radius(uint64_t n, const uint64_t *key, uint32_t *lab, uint32_t *radius, int with_radius) {
+// lab[v] = the centre's level index; the centre's key is at pos[l] when the
+// level is stored in another order (null: l).
+__global__ void k_labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
+                                int with_radius) {
   uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   uint32_t r = 0;
   if (v < n) {
     const uint64_t k = key[v];
     const uint32_t l = static_cast<uint32_t>(k);
     lab[v] = l;
-    if (with_radius) r = static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[l] >> 32);
+    if (with_radius) r = static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[pos ? pos[l] : l] >> 32);
   }
   if (with_radius) {
     for (int o = 32; o > 0; o >>= 1) r = max(r, static_cast<uint32_t>(__shfl_xor(r, o)));
@@ -1065,9 +1071,10 @@ void fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s) {
   KC_LAUNCH_CHECK();
 }
 
-void init_keys_exp(uint64_t n, const uint32_t *gid, DeviateArgs d, LearnedArgs la, uint64_t *key, hipStream_t s) {
+void init_keys_exp(uint64_t n, const uint32_t *gid, const uint32_t *idx, DeviateArgs d, LearnedArgs la, uint64_t *key,
+                   hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_init_keys_exp, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gid, d, la, key);
+  hipLaunchKernelGGL(k_init_keys_exp, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gid, idx, d, la, key);
   KC_LAUNCH_CHECK();
 }
 void init_keys_index(uint64_t n, uint64_t *key, hipStream_t s) {
@@ -1075,9 +1082,10 @@ void init_keys_index(uint64_t n, uint64_t *key, hipStream_t s) {
   hipLaunchKernelGGL(k_init_keys_index, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key);
   KC_LAUNCH_CHECK();
 }
-void init_keys_uniform(uint64_t n, const uint32_t *gid, int32_t seed, uint32_t it, uint64_t *key, hipStream_t s) {
+void init_keys_uniform(uint64_t n, const uint32_t *gid, const uint32_t *idx, int32_t seed, uint32_t it, uint64_t *key,
+                       hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_init_keys_uniform, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gid, seed, it, key);
+  hipLaunchKernelGGL(k_init_keys_uniform, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gid, idx, seed, it, key);
   KC_LAUNCH_CHECK();
 }
 void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, hipStream_t s) {
@@ -1168,9 +1176,11 @@ void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, 
   KC_LAUNCH_CHECK();
 }
 
-void labels_radius(uint64_t n, const uint64_t *key, uint32_t *lab, uint32_t *radius, bool with_radius, hipStream_t s) {
+void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
+                   bool with_radius, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_labels_radius, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, lab, radius, with_radius ? 1 : 0);
+  hipLaunchKernelGGL(k_labels_radius, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, pos, lab, radius,
+                     with_radius ? 1 : 0);
   KC_LAUNCH_CHECK();
 }
 
@@ -1362,6 +1372,59 @@ void component_histogram(uint64_t n, const uint32_t *labels, uint32_t *count, ui
   hipLaunchKernelGGL(k_size_hist, dim3(grid), dim3(kBlock), 0, s, n, count, hist);
   KC_LAUNCH_CHECK();
   scan_exclusive<uint64_t>(n + 1, HistLoad{hist}, HistStore{hist, pairs}, num_pairs, scratch, s);
+}
+
+// Degree-ordered storage of level 0: sort keys ~deg (stable, so ties keep
+// index order), then the maps and the CSR in storage order.
+__global__ void k_perm_sortkeys(uint64_t n, const uint64_t *off, uint32_t *key, uint32_t *val) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    key[i] = ~static_cast<uint32_t>(off[i + 1] - off[i]);
+    val[i] = static_cast<uint32_t>(i);
+  }
+}
+__global__ void k_perm_maps(uint64_t n, const uint32_t *idx, const uint32_t *gid, const uint64_t *off, uint32_t *pos,
+                            uint32_t *gidp, uint32_t *degp) {
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < n;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t i = idx[p];
+    pos[i] = static_cast<uint32_t>(p);
+    gidp[p] = gid ? gid[i] : i;
+    degp[p] = static_cast<uint32_t>(off[i + 1] - off[i]);
+  }
+}
+// One 64-lane block per row: row idx[p] of (off, col) becomes row p, its
+// neighbours mapped to storage positions.
+__global__ __launch_bounds__(64) void k_perm_rows(uint64_t n, const uint32_t *idx, const uint64_t *off,
+                                                  const uint32_t *col, const uint32_t *pos, const uint64_t *offp,
+                                                  uint32_t *colp) {
+  for (uint64_t p = blockIdx.x; p < n; p += gridDim.x) {
+    const uint32_t i = idx[p];
+    const uint64_t s0 = off[i], len = off[i + 1] - s0, d0 = offp[p];
+    for (uint64_t k = threadIdx.x; k < len; k += 64) colp[d0 + k] = pos[col[s0 + k]];
+  }
+}
+
+void level0_order(uint64_t n, const uint64_t *off, const uint32_t *col, const uint32_t *gid, uint32_t *idx,
+                  uint32_t *pos, uint32_t *gidp, uint64_t *offp, uint32_t *colp, uint32_t *k0, uint32_t *v0,
+                  uint32_t *k1, uint32_t *v1, uint64_t *sort_scratch, uint64_t *scan_scratch, hipStream_t s) {
+  if (!n) return;
+  const unsigned grid = grid_for(n) > 65536 ? 65536 : grid_for(n);
+  hipLaunchKernelGGL(k_perm_sortkeys, dim3(grid), dim3(kBlock), 0, s, n, off, k0, v0);
+  KC_LAUNCH_CHECK();
+  const int which = radix_sort_pairs(n, 32, k0, v0, k1, v1, sort_scratch, s);
+  const uint32_t *sorted = which ? v1 : v0;
+  KC_HIP(hipMemcpyAsync(idx, sorted, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  uint32_t *degp = which ? k0 : k1;  // the other key buffer is free
+  hipLaunchKernelGGL(k_perm_maps, dim3(grid), dim3(kBlock), 0, s, n, idx, gid, off, pos, gidp, degp);
+  KC_LAUNCH_CHECK();
+  uint64_t *total = scan_scratch;
+  scan_exclusive_u32_to_u64(n, degp, offp, total, scan_scratch + 4, s);
+  hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, total, offp + n);
+  KC_LAUNCH_CHECK();
+  const unsigned rows = n > (1u << 20) ? (1u << 20) : static_cast<unsigned>(n);
+  hipLaunchKernelGGL(k_perm_rows, dim3(rows), dim3(64), 0, s, n, idx, off, col, pos, offp, colp);
+  KC_LAUNCH_CHECK();
 }
 
 void identity_u32(uint64_t n, uint32_t *p, hipStream_t s) {

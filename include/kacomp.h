@@ -115,7 +115,9 @@ typedef struct kc_options {
   int32_t sweep_group_large;
   int32_t deviate_stream; /* KC_STREAM_REFERENCE (default) or KC_STREAM_HASHED */
   float learned_alpha;    /* learned: delay = round(learned_alpha * max(score, 0)) rounds (default 1) */
-  int32_t reserved[5];
+  int32_t input_order;    /* 1: level 0 keeps the input vertex order instead of the degree-descending
+                             storage order (same results; for comparisons) */
+  int32_t reserved[4];
 } kc_options;
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit

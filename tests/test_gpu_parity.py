@@ -303,3 +303,32 @@ def test_app_prints_component_summary(tmp_path):
         assert [(int(a), int(b)) for a, b in pairs] == [(int(a), int(b)) for a, b in want]
         num = [l for l in r.stdout.splitlines() if l.startswith("NUM COMPONENTS")]
         assert num and int(num[0].split()[-1]) == int(want[:, 1].sum())
+
+
+@pytest.mark.parametrize("stream", [ca.STREAM_HASHED, ca.STREAM_REFERENCE])
+@pytest.mark.parametrize("bfs", [False, True])
+def test_degree_order_matches_input_order(stream, bfs):
+    """Level 0 stored in degree order (large graphs) gives the same levels, keys
+    and labels as the input order, and both equal the oracle's trace."""
+    g = ca.generate("rmat", n=1 << 17, m=1 << 20, seed=5)
+    assert int((np.diff(g.offsets.astype(np.int64)) > 0).sum()) >= 1 << 16
+    dg = ca.DeviceGraph(g)
+    try:
+        runs = {}
+        for inorder in (False, True):
+            lab, st = dg.find_components(seed=9, stream=stream, use_bfs=bfs, seq_limit=4096, record_trace=True,
+                                         input_order=inorder)
+            runs[inorder] = (lab, dg.trace())
+    finally:
+        dg.close()
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    run = oracle.Run(g.offsets, g.targets, variant="exp", seed=9, seq_limit=4096, stream=stream, keep_keys=True)
+    want = run.levels()
+    for inorder, (lab, tr) in runs.items():
+        assert np.array_equal(lab, ref), inorder
+        assert len(tr) == len(want), inorder
+        for lvl, ((info, keys, gid), winfo) in enumerate(zip(tr, want)):
+            assert info[0] == winfo[0] and info[1] == winfo[1] and info[3] == winfo[3], (inorder, lvl)
+            wkeys, wgid = run.level_keys(lvl)
+            assert np.array_equal(keys, wkeys), (inorder, lvl)
+            assert np.array_equal(gid, wgid), (inorder, lvl)
