@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--wmax", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other plain workloads")
     ap.add_argument("--learned-rhg-log-n", type=int, default=24)
     ap.add_argument("--learned-rmat-scale", type=int, default=25)
     ap.add_argument("--batch-graphs", type=int, default=4096)
@@ -175,11 +176,64 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"], out["labels_bit_exact"] = cpu_leg(a, dg, run_kw, stream)
     dg.close()
+    if not a.no_configs:
+        out["plain_configs"] = configs_leg(a, stream, run_kw)
     if not a.no_batched:
         out["batched"] = batched_leg(a, stream)
     if not a.no_learned:
         out["learned"] = learned_leg(a, stream, run_kw)
     print(json.dumps(out), flush=True)
+
+
+def configs_leg(a, stream, run_kw):
+    """The plain form on the other workloads: G(n,m) n = 2^20, m = 2^22, seed 1;
+    3D grid 256^3 with every bond kept with probability 0.3, seed 2; 2D random
+    geometric graph n = 2^24 at average degree 8, seed 2.  Labels are checked
+    against the oracle's union-find."""
+    import numpy as np
+
+    import components_amd as ca
+    import oracle
+
+    out = {}
+    cases = [("gnm-n2^20-m2^22-seed1", "device", "gnm_undirected", dict(n=1 << 20, m=1 << 22, seed=1)),
+             ("grid3d-256^3-p0.3-seed2", "device", "grid_3d", dict(nx=256, ny=256, nz=256, p=0.3, seed=2)),
+             ("rgg2d-n2^24-deg8-seed2", "host", "rgg_2d",
+              dict(n=1 << 24, r=float(np.sqrt(8 / (np.pi * (1 << 24)))), seed=2))]
+    for label, where, name, kw in cases:
+        t0 = time.time()
+        if where == "device":
+            dg = ca.DeviceGraph.generate(name, **kw)
+            host = None
+        else:
+            host = ca.generate(name, **kw)
+            dg = ca.DeviceGraph(host)
+        m = dg.m
+        seq = max(m // 256, 4096)
+        res = {"n": dg.n, "m": m}
+        for sched, bfs in (("rounds", True), ("sweeps", False)):
+            kw2 = dict(run_kw, seq_limit=seq, use_bfs=bfs)
+            for i in range(a.warmup):
+                dg.find_components(copy_labels=False, seed=1 + i + 1000, **kw2)
+            sts = []
+            t1 = time.perf_counter()
+            for i in range(a.steps):
+                sts.append(dg.find_components(copy_labels=False, seed=1 + i + 1000, **kw2)[1])
+            ms = (time.perf_counter() - t1) * 1e3 / a.steps
+            res[sched] = {"value": round(m / (ms / 1e3) / 1e9, 4), "unit": "GTEPS", "ms_per_step": round(ms, 4),
+                          "levels": float(np.mean([s["levels"] for s in sts])),
+                          "sweeps": float(np.mean([s["sweeps"] for s in sts])),
+                          "radius_sum": float(np.mean([s["radius_sum"] for s in sts]))}
+        lab = dg.find_components(seed=1001, **dict(run_kw, seq_limit=seq))[0]
+        if host is None:
+            host = dg.download()
+        res["labels_bit_exact"] = bool(np.array_equal(lab, oracle.canonical_labels(host.offsets, host.targets)))
+        dg.close()
+        del host
+        out[label] = res
+        print(f"[bench] {label}: rounds {res['rounds']['ms_per_step']:.2f} ms, sweeps {res['sweeps']['ms_per_step']:.2f} ms,"
+              f" exact {res['labels_bit_exact']} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    return out
 
 
 def batched_leg(a, stream):
