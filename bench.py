@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other plain workloads")
+    ap.add_argument("--untimed-kernels", action="store_true",
+                    help="time the steps without per-launch HIP events (no kernel breakdown or roofline)")
     ap.add_argument("--learned-rhg-log-n", type=int, default=24)
     ap.add_argument("--learned-rmat-scale", type=int, default=25)
     ap.add_argument("--batch-graphs", type=int, default=4096)
@@ -120,7 +122,7 @@ def main():
     stats = []
     t0 = time.perf_counter()
     for i in range(a.steps):
-        _, st = dg.find_components(copy_labels=False, seed=1 + i + 1000, time_kernels=True, **run_kw)
+        _, st = dg.find_components(copy_labels=False, seed=1 + i + 1000, time_kernels=not a.untimed_kernels, **run_kw)
         stats.append(st)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / a.steps

@@ -475,11 +475,14 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
                             uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base) {
   uint64_t done = 0;
   uint32_t t = 0;
-  const uint32_t batch = 4;
   Bins b = level_bins;
-  uint32_t next_rebin = batch;
+  uint32_t next_rebin = 4;
   int first_slot = -1;
   while (true) {
+    // the first rounds carry most of the pushes; later ones (over the
+    // re-binned unsettled vertices) are cheap, so they go in longer
+    // speculative batches with one synchronisation each
+    const uint32_t batch = t < 8 ? 4u : 16u;
     if (t >= next_rebin) {
       // drop the settled vertices from the rounds' bins (distances below t)
       b = make_bins(g, n, off, cfg, key, t);
@@ -489,7 +492,7 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
       flag_base = 0;
     }
-    int slots[8];
+    int slots[16];
     for (uint32_t i = 0; i < batch; ++i) {
       const uint32_t f = flag_base + i;
       const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
