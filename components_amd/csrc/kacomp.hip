@@ -227,6 +227,8 @@ struct kc_graph {
   uint64_t *dl_huge_tab = nullptr;
   uint32_t *huge_tables = nullptr;
   uint64_t huge_cap = 0;
+  uint32_t *huge_bits = nullptr;  // bitmaps of the huge buckets over the label space
+  uint64_t huge_bits_cap = 0;
   HugeChunk *d_chunks = nullptr;
   uint64_t chunks_cap = 0;
   uint64_t *d_huge_info = nullptr;
@@ -285,6 +287,7 @@ namespace {
 
 constexpr uint32_t kMaxFlags = 1u << 16;
 constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep the input order
+constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
 constexpr uint32_t kMinWaveBin = 256;
 
 
@@ -399,7 +402,7 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
 // processed in fixed-size chunks planned on the host.  Returns the timer slot
 // of the first dedup launch (its algorithmic bytes are booked by the caller).
 int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
-                   uint32_t *medium, uint32_t *huge, uint64_t *huge_tab) {
+                   uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space) {
   hipStream_t s = g->stream;
   fill_u64(g->sizes + 3, 3, 0, s);
   DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
@@ -444,6 +447,24 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *b
       c.hi = std::min(tsz, lo + kHugeChunk);
       slot.push_back(c);
     }
+  }
+  // Bitmaps over the label space when they fit the cap: one pass, in place.
+  const uint64_t words = (label_space + 31) / 32;
+  if (n_huge * words * 4 <= kHugeBitmapBytes) {
+    if (elem.size() > g->chunks_cap) {
+      g->chunks_cap = elem.size() + 1024;
+      g->d_chunks = g->dalloc<HugeChunk>(g->chunks_cap);
+    }
+    KC_HIP(hipMemcpyAsync(g->d_chunks, elem.data(), elem.size() * sizeof(HugeChunk), hipMemcpyHostToDevice, s));
+    if (n_huge * words > g->huge_bits_cap) {
+      g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
+      g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
+    }
+    TimedScope ts(g->timer, KC_K_DEDUP, 0);
+    dedup_huge_bits(g->d_chunks, static_cast<uint32_t>(elem.size()), static_cast<uint32_t>(n_huge), huge, g->huge_bits,
+                    words, bucket, uniq, s);
+    sync(g);  // the chunk vector dies here
+    return first;
   }
   if (elem.size() + slot.size() > g->chunks_cap) {
     g->chunks_cap = elem.size() + slot.size() + 1024;
@@ -726,7 +747,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, g->bucket, g->stream);
     }
     const int dedup_slot =
-        dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab);
+        dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab, n_l);
     // next-level CSR
     uint64_t *off_next = g->off_pp[pp];
     uint32_t *col_next = g->col_pp[pp];
@@ -961,7 +982,7 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   uint32_t *medium = lists.as<uint32_t>();
   uint32_t *huge = medium + n + 2;
   uint64_t *huge_tab = reinterpret_cast<uint64_t *>(huge + n + 2) + 1;
-  dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), medium, huge, huge_tab);
+  dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), medium, huge, huge_tab, n);
   g->off0 = g->dalloc<uint64_t>(n + 1);
   csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));

@@ -146,6 +146,10 @@ void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg
                   uint32_t *uniq, hipStream_t s);
 // out[2h] = bucket offset, out[2h+1] = size of huge bucket h.
 void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out, hipStream_t s);
+// Huge buckets against bitmaps of `words` u32 per bucket over the label space
+// (duplicates become kNone in place, uniq holds the kept counts).
+void dedup_huge_bits(const HugeChunk *elem_chunks, uint32_t n_elem, uint32_t n_huge, const uint32_t *huge,
+                     uint32_t *bitmaps, uint64_t words, uint32_t *bucket, uint32_t *uniq, hipStream_t s);
 void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,
                 uint32_t *tables, uint32_t *bucket, uint32_t *uniq, uint32_t *hcount, hipStream_t s);
 

@@ -886,6 +886,50 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(const HugeChunk *chu
   }
 }
 
+// Huge buckets against per-bucket bitmaps over the label space: the first
+// lane to set a value's bit keeps it, every other copy becomes kNone in place
+// (read before the atomic: a set bit is never stale, bits only get set), and
+// each block adds its kept count to uniq of the bucket.
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_bits(const HugeChunk *chunks, uint32_t n_chunks,
+                                                           uint32_t *bitmaps, uint64_t words, uint32_t *bucket,
+                                                           uint32_t *uniq) {
+  __shared__ uint32_t wave_kept[kBlock / kWave];
+  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const HugeChunk h = chunks[c];
+    uint32_t *bm = bitmaps + static_cast<uint64_t>(h.huge) * words;
+    uint32_t kept = 0;
+    for (uint64_t j = h.lo + threadIdx.x; j < h.hi; j += kBlock) {
+      uint32_t *slot = &bucket[h.base + j];
+      const uint32_t v = *slot;
+      if ((v >> 5) >= words) {  // outside the label space: kept unchecked (never the case)
+        ++kept;
+        continue;
+      }
+      const uint32_t bit = 1u << (v & 31u);
+      uint32_t *w = &bm[v >> 5];
+      bool dup = (*w & bit) != 0;
+      if (!dup) dup = (atomicOr(w, bit) & bit) != 0;
+      if (dup)
+        *slot = kNone;
+      else
+        ++kept;
+    }
+    for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
+    if (lane_id() == 0) wave_kept[threadIdx.x / kWave] = kept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t t = wave_kept[0] + wave_kept[1] + wave_kept[2] + wave_kept[3];
+      if (t) atomicAdd(&uniq[h.a2], t);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_huge_uniq(uint32_t n_huge, const uint32_t *huge, uint32_t *uniq) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < n_huge) uniq[huge[h]] = 0;
+}
+
 // (offset, size) of every huge bucket, for the host to plan the chunks.
 __global__ void k_huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out) {
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1250,6 +1294,18 @@ void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *
   hipLaunchKernelGGL(k_dedup_huge_emit, dim3(gs), dim3(kBlock), 0, s, slot_chunks, n_slot, tables, bucket, hcount);
   KC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_dedup_huge_tail, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, bucket, uniq, hcount);
+  KC_LAUNCH_CHECK();
+}
+
+void dedup_huge_bits(const HugeChunk *elem_chunks, uint32_t n_elem, uint32_t n_huge, const uint32_t *huge,
+                     uint32_t *bitmaps, uint64_t words, uint32_t *bucket, uint32_t *uniq, hipStream_t s) {
+  if (!n_elem) return;
+  fill_u32(bitmaps, static_cast<uint64_t>(n_huge) * words, 0u, s);
+  hipLaunchKernelGGL(k_zero_huge_uniq, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s, n_huge,
+                     huge, uniq);
+  KC_LAUNCH_CHECK();
+  const unsigned ge = n_elem < 8192 ? n_elem : 8192;
+  hipLaunchKernelGGL(k_dedup_huge_bits, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, bitmaps, words, bucket, uniq);
   KC_LAUNCH_CHECK();
 }
 
