@@ -287,6 +287,7 @@ namespace {
 
 constexpr uint32_t kMaxFlags = 1u << 16;
 constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep the input order
+constexpr int kTailShift = 5;  // -bfs: pull sweeps finish a level once at most n/32 rows are pending
 constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
 constexpr uint32_t kMinWaveBin = 256;
 
@@ -359,10 +360,10 @@ uint64_t sweep_bytes(uint64_t n, uint64_t m2, bool count) {
 // (the last of them found no change).
 uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                            uint64_t *key, uint64_t hop, const Bins &b, bool count, bool with_jump,
-                           uint32_t &flag_base) {
+                           uint32_t &flag_base, bool book_bytes = true) {
   uint64_t done = 0;
   const uint32_t batch = n < (1u << 16) ? 8u : 3u;
-  const uint64_t bytes = sweep_bytes(n, m2, count);
+  const uint64_t bytes = book_bytes ? sweep_bytes(n, m2, count) : 0;
   while (true) {
     if (flag_base + batch + 2 >= kMaxFlags) {
       fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
@@ -493,7 +494,8 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *b
 // one counting pass for the cluster-edge counts.  Returns the rounds that did
 // work (the reported "sweeps" of this schedule).
 uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
-                            uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base) {
+                            uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base,
+                            int tail_shift) {
   uint64_t done = 0;
   uint32_t t = 0;
   Bins b = level_bins;
@@ -508,6 +510,17 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       // drop the settled vertices from the rounds' bins (distances below t)
       b = make_bins(g, n, off, cfg, key, t);
       next_rebin *= 2;
+      const uint64_t pending = uint64_t(b.nT) + b.nS + b.nW + b.nH;
+      if (tail_shift >= 0 && (pending << tail_shift) <= n) {
+        // Few vertices left: finish with in-place pull sweeps over them.  The
+        // settled vertices have pushed their final keys already, so the keys
+        // still to change follow from the pending rows alone, and the sweeps
+        // reach the same fixed point in as many sweeps as the longest chain
+        // of improvements among them, instead of one round per distance left.
+        // (The rounds' booked bytes already cover one read of every row.)
+        done += sweep_to_fixpoint(g, n, m2, off, col, key, kOneHop, b, false, false, flag_base, false);
+        break;
+      }
     }
     if (flag_base + batch + 2 >= kMaxFlags) {
       fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
@@ -647,7 +660,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       }
       sweeps = 2;
     } else if (o->use_bfs) {
-      sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base);
+      sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base,
+                                  o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps);
     } else {
       sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
     }

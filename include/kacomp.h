@@ -117,7 +117,9 @@ typedef struct kc_options {
   float learned_alpha;    /* learned: delay = round(learned_alpha * max(score, 0)) rounds (default 1) */
   int32_t input_order;    /* 1: level 0 keeps the input vertex order instead of the degree-descending
                              storage order (same results; for comparisons) */
-  int32_t reserved[4];
+  int32_t tail_sweeps;    /* -bfs: finish a level with pull sweeps once at most n >> k rows are pending:
+                             0 = default k, k > 0 = that k, -1 = rounds to the end (same results) */
+  int32_t reserved[3];
 } kc_options;
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit

@@ -205,15 +205,17 @@ def test_rmat_scale22_schedules_agree():
     g = dg.download()
     ref = oracle.canonical_labels(g.offsets, g.targets)
     traces = []
-    for bfs in (False, True):
+    # sweeps; push rounds finished by pull sweeps (default); push rounds only; early switch to sweeps
+    for bfs, tail in ((False, 0), (True, 0), (True, -1), (True, 1)):
         lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
-                                     use_bfs=bfs, record_trace=True)
-        assert np.array_equal(lab, ref)
+                                     use_bfs=bfs, tail_sweeps=tail, record_trace=True)
+        assert np.array_equal(lab, ref), (bfs, tail)
         traces.append(dg.trace())
-    assert len(traces[0]) == len(traces[1])
-    for (i0, k0, g0), (i1, k1, g1) in zip(*traces):
-        assert i0[0] == i1[0] and i0[1] == i1[1] and i0[3] == i1[3]
-        assert np.array_equal(k0, k1) and np.array_equal(g0, g1)
+    for other in traces[1:]:
+        assert len(traces[0]) == len(other)
+        for (i0, k0, g0), (i1, k1, g1) in zip(traces[0], other):
+            assert i0[0] == i1[0] and i0[1] == i1[1] and i0[3] == i1[3]
+            assert np.array_equal(k0, k1) and np.array_equal(g0, g1)
     dg.close()
 
 
