@@ -206,6 +206,9 @@ struct SweepArgs {
   const int *prev_flag;
 };
 
+// Edges whose gathers one lane keeps in flight together in the row loops.
+constexpr int kGather = 4;
+
 template <int MODE, bool COUNT, int W>
 __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uint32_t v, uint64_t e0,
                                             uint64_t e1, int gl, bool hub, uint32_t chunk) {
@@ -214,7 +217,23 @@ __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uin
   uint32_t cnt = 0;
   if (active) {
     const uint32_t lv = static_cast<uint32_t>(kv);
-    for (uint64_t e = e0 + gl; e < e1; e += W) {
+    uint64_t e = e0 + gl;
+    // kGather edges per lane per step: their neighbour ids, then their keys,
+    // are all in flight together
+    for (; e + (kGather - 1) * W < e1; e += kGather * W) {
+      uint32_t c[kGather];
+      uint64_t k[kGather];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) c[j] = a.col[e + j * W];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) k[j] = a.kin[c[j]];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) {
+        best = min_u64(best, k[j] + a.hop);
+        if (COUNT) cnt += static_cast<uint32_t>(k[j]) != lv;
+      }
+    }
+    for (; e < e1; e += W) {
       const uint64_t kw = a.kin[a.col[e]];
       best = min_u64(best, kw + a.hop);
       if (COUNT) cnt += static_cast<uint32_t>(kw) != lv;
@@ -302,7 +321,20 @@ __device__ __forceinline__ bool round_group(const SweepArgs &a, bool active, uin
       e1 = a.off[v + 1];
     }
     const uint64_t cand = kv + a.hop;
-    for (uint64_t e = e0 + gl; e < e1; e += W) {
+    uint64_t e = e0 + gl;
+    for (; e + (kGather - 1) * W < e1; e += kGather * W) {
+      uint32_t c[kGather];
+      uint64_t k[kGather];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) c[j] = a.col[e + j * W];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) k[j] = a.kout[c[j]];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j)
+        if (cand < k[j])
+          atomicMin(reinterpret_cast<unsigned long long *>(a.kout + c[j]), static_cast<unsigned long long>(cand));
+    }
+    for (; e < e1; e += W) {
       uint64_t *kw = a.kout + a.col[e];
       if (cand < *kw) atomicMin(reinterpret_cast<unsigned long long *>(kw), static_cast<unsigned long long>(cand));
     }
@@ -355,8 +387,7 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t
   }
   if (active) {
     const uint32_t lv = lab[v];
-    for (uint64_t e = e0 + gl; e < e1; e += W) {
-      const uint32_t lw = lab[a.col[e]];
+    auto one = [&](uint64_t e, uint32_t lw) {
       bool inter = lw != lv;
       if (W > 1 && inter) {
         const uint32_t slot = hash32(lw) & fmask;
@@ -367,7 +398,18 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t
       }
       relab[e] = inter ? lw : kNone;
       cnt += inter;
+    };
+    uint64_t e = e0 + gl;
+    for (; e + (kGather - 1) * W < e1; e += kGather * W) {
+      uint32_t c[kGather], l[kGather];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) c[j] = a.col[e + j * W];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) l[j] = lab[c[j]];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) one(e + j * W, l[j]);
     }
+    for (; e < e1; e += W) one(e, lab[a.col[e]]);
   }
   if (W > 1) cnt = group_sum_u32(cnt, W);
   if (active && gl == 0) {
