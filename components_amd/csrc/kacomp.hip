@@ -216,7 +216,8 @@ struct kc_graph {
   uint32_t *flists = nullptr;  // bins of the unsettled vertices (time-stepped schedule)
   HubChunk *fhubs = nullptr;
   uint64_t hubs_cap = 0;
-  uint32_t *bucket = nullptr, *old_of = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
+  uint32_t *slots = nullptr, *seg_of_cid = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
+  uint32_t *crank = nullptr, *cid = nullptr;  // dense cluster ids: rank by label, id per vertex
   uint32_t *sk = nullptr, *sv = nullptr, *sk2 = nullptr, *sv2 = nullptr;  // (label, vertex) sort buffers
   uint64_t *seg_off = nullptr, *sort_scratch = nullptr;
   uint32_t *rep_a = nullptr, *rep_b = nullptr, *minid = nullptr, *labels = nullptr;
@@ -235,7 +236,7 @@ struct kc_graph {
   uint64_t huge_info_cap = 0;
   int *flags = nullptr;
   uint32_t *small = nullptr;  // [0..3] bin counters, [4] radius, [5] roots
-  uint64_t *sizes = nullptr;  // [0] n_next [1] P [2] m2_next [3..5] dedup counters
+  uint64_t *sizes = nullptr;  // [0] n_next [1] P [2] m2_next [3..5] dedup counters [8] N [9] clusters
   // pinned host mirrors
   uint64_t *h_sizes = nullptr;
   uint32_t *h_small = nullptr;
@@ -402,8 +403,8 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
 // in wave tables, medium ones in block tables, huge ones in global tables
 // processed in fixed-size chunks planned on the host.  Returns the timer slot
 // of the first dedup launch (its algorithmic bytes are booked by the caller).
-int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
-                   uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space) {
+int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
+                   uint32_t *uniq, uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space) {
   hipStream_t s = g->stream;
   fill_u64(g->sizes + 3, 3, 0, s);
   DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
@@ -411,19 +412,32 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *b
   {
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
     first = ts.slot;
-    dedup_small(nb, seg_off, bucket, uniq, dl, s);
+    dedup_small(nb, seg_off, bucket, out, uniq, dl, s);
   }
   d2h(g, g->h_sizes + 3, g->sizes + 3, 3 * sizeof(uint64_t));
   sync(g);
   const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
   if (n_medium) {
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
-    dedup_medium(static_cast<uint32_t>(n_medium), medium, seg_off, bucket, uniq, s);
+    dedup_medium(static_cast<uint32_t>(n_medium), medium, seg_off, bucket, out, uniq, s);
   }
   if (!n_huge) return first;
   if (2 * n_huge > g->huge_info_cap) {
     g->huge_info_cap = 2 * n_huge + 64;
     g->d_huge_info = g->dalloc<uint64_t>(g->huge_info_cap);
+  }
+  // Bitmaps over the label space when they fit the cap: one pass, in place,
+  // with the chunks planned on the device.
+  const uint64_t words = (label_space + 31) / 32;
+  if (n_huge * words * 4 <= kHugeBitmapBytes) {
+    if (n_huge * words > g->huge_bits_cap) {
+      g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
+      g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
+    }
+    TimedScope ts(g->timer, KC_K_DEDUP, 0);
+    dedup_huge_bits(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, g->d_huge_info,
+                    g->huge_bits, words, bucket, out, uniq, g->scan_scratch, s);
+    return first;
   }
   huge_sizes(static_cast<uint32_t>(n_huge), huge, seg_off, g->d_huge_info, s);
   std::vector<uint64_t> info(2 * n_huge), tab(n_huge);
@@ -449,24 +463,6 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *b
       slot.push_back(c);
     }
   }
-  // Bitmaps over the label space when they fit the cap: one pass, in place.
-  const uint64_t words = (label_space + 31) / 32;
-  if (n_huge * words * 4 <= kHugeBitmapBytes) {
-    if (elem.size() > g->chunks_cap) {
-      g->chunks_cap = elem.size() + 1024;
-      g->d_chunks = g->dalloc<HugeChunk>(g->chunks_cap);
-    }
-    KC_HIP(hipMemcpyAsync(g->d_chunks, elem.data(), elem.size() * sizeof(HugeChunk), hipMemcpyHostToDevice, s));
-    if (n_huge * words > g->huge_bits_cap) {
-      g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
-      g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
-    }
-    TimedScope ts(g->timer, KC_K_DEDUP, 0);
-    dedup_huge_bits(g->d_chunks, static_cast<uint32_t>(elem.size()), static_cast<uint32_t>(n_huge), huge, g->huge_bits,
-                    words, bucket, uniq, s);
-    sync(g);  // the chunk vector dies here
-    return first;
-  }
   if (elem.size() + slot.size() > g->chunks_cap) {
     g->chunks_cap = elem.size() + slot.size() + 1024;
     g->d_chunks = g->dalloc<HugeChunk>(g->chunks_cap);
@@ -484,7 +480,8 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, uint32_t *b
   fill_u32(hcount, n_huge, 0, s);
   TimedScope ts(g->timer, KC_K_DEDUP, 0);
   dedup_huge(g->d_chunks, static_cast<uint32_t>(elem.size()), g->d_chunks + elem.size(),
-             static_cast<uint32_t>(slot.size()), g->huge_tables, bucket, uniq, hcount, s);
+             static_cast<uint32_t>(slot.size()), static_cast<uint32_t>(n_huge), huge, g->huge_tables, bucket, out, uniq,
+             hcount, s);
   // the chunk vectors die here: wait until the uploads have been consumed
   sync(g);
   return first;
@@ -711,12 +708,27 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       cluster_targets(n_l, rec.lab, g->csize, g->target, g->stream);
       mlp_append(g->collect, n_l, g->feat, g->target, g->collect_stride, g->stream);
     }
-    // relabel: every edge slot gets the label of its far end (kNone inside a
-    // cluster); the slots live in the next level's column buffer until it is built
-    uint32_t *relab = g->col_pp[pp];
+    // dense cluster ids in label order: the buckets below hold them, so the
+    // dedup works over the C clusters instead of the n_l labels
+    fill_u32(g->cid, n_l, 0, g->stream);  // the used-label flags, then the ids
+    {
+      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l);
+      mark_labels(n_l, rec.lab, g->cid, g->stream);
+    }
+    {
+      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l);
+      cluster_ranks(n_l, g->cid, g->crank, g->sizes + 9, g->scan_scratch, g->stream);
+    }
+    {
+      TimedScope ts(g->timer, KC_K_COUNT, 12 * n_l);
+      cluster_ids(n_l, rec.lab, g->crank, g->cid, g->stream);
+    }
+    // relabel: every edge slot gets the cluster id of its far end (kNone inside
+    // a cluster); the slot buffer later takes the unique runs of the dedup
+    uint32_t *relab = g->slots;
     {
       TimedScope ts(g->timer, KC_K_RELABEL, 8 * (n_l + 1) + 4 * m2_l + 4 * m2_l + 4 * n_l + 4 * m2_l + 4 * n_l);
-      relabel(off_l, col_l, rec.lab, b, relab, g->deg_inter, g->hub_inter, g->stream);
+      relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, g->stream);
     }
     if (b.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
@@ -727,9 +739,10 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       TimedScope ts(g->timer, KC_K_SCAN, 2 * 4 * n_l + 8 * n_l + 8 * n_l);
       select_inter(n_l, g->key, g->deg_inter, g->sk, g->sv, g->sizes + 8, g->scan_scratch, g->stream);
     }
-    d2h(g, g->h_sizes + 8, g->sizes + 8, sizeof(uint64_t));
+    d2h(g, g->h_sizes + 8, g->sizes + 8, 2 * sizeof(uint64_t));
     sync(g);
     const uint64_t N = g->h_sizes[8];
+    const uint64_t clusters = g->h_sizes[9];
     int bits = 1;
     while (bits < 32 && (1ull << bits) < n_l) ++bits;
     int which = 0;
@@ -741,8 +754,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     fill_u32(rec.newidx, n_l, kNone, g->stream);
     {
       TimedScope ts(g->timer, KC_K_SCAN, 2 * (8 * N + 4 * N) + 4 * N + 20 * n_l / 8);
-      segments(N, sk, sv, g->deg_inter, gid_l, g->pos_of, rec.newidx, g->old_of, gid_next, g->seg_off, g->sizes,
-               g->scan_scratch, g->stream);
+      segments(N, sk, sv, g->deg_inter, gid_l, g->crank, g->pos_of, rec.newidx, g->seg_of_cid, gid_next, g->seg_off,
+               g->sizes, g->scan_scratch, g->stream);
     }
     d2h(g, g->h_sizes, g->sizes, 2 * sizeof(uint64_t));
     d2h(g, g->h_small + 4, g->small + 4, sizeof(uint32_t));
@@ -755,16 +768,19 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     g->levels.push_back(rec);
     if (n_next == 0) break;  // every cluster is a whole component
 
-    // gather: every row copies its cluster edges to its position in its cluster's bucket
-    {
-      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l + 4 * P);
-      scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, g->bucket, g->stream);
-    }
-    const int dedup_slot =
-        dedup_buckets(g, n_next, g->seg_off, g->bucket, g->uniq, g->dl_medium, g->dl_huge, g->dl_huge_tab, n_l);
-    // next-level CSR
+    // gather: every row copies its cluster edges to its position in its
+    // cluster's bucket, which lives in the next level's column buffer until
+    // the columns are built
     uint64_t *off_next = g->off_pp[pp];
     uint32_t *col_next = g->col_pp[pp];
+    uint32_t *bucket = col_next;
+    {
+      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l + 4 * P);
+      scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, bucket, g->stream);
+    }
+    const int dedup_slot = dedup_buckets(g, n_next, g->seg_off, bucket, g->slots, g->uniq, g->dl_medium, g->dl_huge,
+                                         g->dl_huge_tab, clusters);
+    // next-level CSR
     {
       TimedScope ts(g->timer, KC_K_SCAN, 4 * n_next + 8 * (n_next + 1));
       csr_offsets(n_next, g->uniq, off_next, g->sizes, g->scan_scratch, g->stream);
@@ -776,8 +792,9 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     // cluster its bucket bounds and its unique count
     g->timer.retag(dedup_slot, KC_K_DEDUP, 4 * P + 4 * m2_next + 8 * (n_next + 1) + 4 * n_next);
     {
-      TimedScope ts(g->timer, KC_K_CSR, 4 * P + 4 * m2_next);
-      compact(P, g->bucket, col_next, rec.newidx, g->scan_scratch, g->stream);
+      // unique runs read, mapped to next-level indices, written
+      TimedScope ts(g->timer, KC_K_CSR, 4 * m2_next + 4 * m2_next + 4 * m2_next);
+      gather_uniques(m2_next, n_next, off_next, g->seg_off, g->slots, g->seg_of_cid, col_next, g->stream);
     }
     n_l = n_next;
     m2_l = m2_next;
@@ -904,8 +921,10 @@ void init_work(kc_graph *g) {
   g->flists = g->dalloc<uint32_t>(3 * n);
   g->fhubs = g->dalloc<HubChunk>(g->hubs_cap);
   g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
-  g->bucket = g->dalloc<uint32_t>(m2);
-  g->old_of = g->dalloc<uint32_t>(n);
+  g->slots = g->dalloc<uint32_t>(m2);
+  g->seg_of_cid = g->dalloc<uint32_t>(n);
+  g->crank = g->dalloc<uint32_t>(n);
+  g->cid = g->dalloc<uint32_t>(n);
   g->uniq = g->dalloc<uint32_t>(n);
   g->pos_of = g->dalloc<uint32_t>(n);
   g->hub_pos = g->dalloc<uint32_t>(g->hubs_cap);
@@ -996,14 +1015,16 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   uint32_t *medium = lists.as<uint32_t>();
   uint32_t *huge = medium + n + 2;
   uint64_t *huge_tab = reinterpret_cast<uint64_t *>(huge + n + 2) + 1;
-  dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), uniq.as<uint32_t>(), medium, huge, huge_tab, n);
+  DevTemp runs(P * 4 + 64);
+  dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), runs.as<uint32_t>(), uniq.as<uint32_t>(), medium,
+                huge, huge_tab, n);
   g->off0 = g->dalloc<uint64_t>(n + 1);
   csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));
   KC_HIP(hipStreamSynchronize(s));
   g->m2 = g->h_sizes[2];
   g->col0 = g->dalloc<uint32_t>(g->m2);
-  compact(P, bucket.as<uint32_t>(), g->col0, nullptr, scratch.as<uint64_t>(), s);
+  gather_uniques(g->m2, n, g->off0, off_tmp.as<uint64_t>(), runs.as<uint32_t>(), nullptr, g->col0, s);
   unsigned long long *cnt = reinterpret_cast<unsigned long long *>(g->sizes + 6);
   fill_u64(g->sizes + 6, 1, 0, s);
   nonzero_rows(n, g->off0, cnt, s);

@@ -98,16 +98,24 @@ void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
 void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
                    bool with_radius,
                    hipStream_t s);
+// Dense cluster ids in label order: used[lab[v]] = 1 (used pre-zeroed);
+// rank[a] = number of used labels below a, written for the used ones, and
+// *count (device) = number of clusters C; then cid[v] = rank[lab[v]] in [0, C).
+void mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used, hipStream_t s);
+void cluster_ranks(uint64_t n, const uint32_t *used, uint32_t *rank, uint64_t *count, uint64_t *scratch,
+                   hipStream_t s);
+void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid, hipStream_t s);
 // (label, vertex) of every vertex with a cluster edge; *count (device) = N.
 void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *sk, uint32_t *sv,
                   uint64_t *count, uint64_t *scratch, hipStream_t s);
 // Over the label-sorted list: pos_of[v] = write position of v's cluster edges,
-// newidx[a] = next-level index of label a (caller pre-fills kNone), old_of and
-// gid_next by next-level index, seg_off[0..n_next] bucket offsets;
-// sizes[0] = n_next, sizes[1] = P (cluster edges).
+// newidx[a] = next-level index of label a (caller pre-fills kNone),
+// seg_of_cid[rank[a]] = the same index by dense cluster id, gid_next by
+// next-level index, seg_off[0..n_next] bucket offsets; sizes[0] = n_next,
+// sizes[1] = P (cluster edges).
 void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t *deg_inter, const uint32_t *gid,
-              uint32_t *pos_of, uint32_t *newidx, uint32_t *old_of, uint32_t *gid_next, uint64_t *seg_off,
-              uint64_t *sizes, uint64_t *scratch, hipStream_t s);
+              const uint32_t *rank, uint32_t *pos_of, uint32_t *newidx, uint32_t *seg_of_cid, uint32_t *gid_next,
+              uint64_t *seg_off, uint64_t *sizes, uint64_t *scratch, hipStream_t s);
 
 // relab[e] = lab[col[e]] when it differs from lab[v], else kNone;
 // deg_inter[v] (hub chunks: hub_inter) = number of such edges.
@@ -118,9 +126,10 @@ void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const ui
              const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *pos_of, uint32_t *bucket,
              hipStream_t s);
 
-// Dedup of every bucket a2 (elements [seg_off[a2], seg_off[a2+1])): small ones
-// in wave LDS tables; medium and huge ones are listed (counters zeroed by the
-// caller: [0] medium count, [1] huge count, [2] huge table slots).
+// Dedup of every bucket a2 (elements [seg_off[a2], seg_off[a2+1]) of bucket):
+// its uniq[a2] distinct values go to out[seg_off[a2] ...] in any order.  Small
+// buckets in wave LDS tables; medium and huge ones are listed (counters zeroed
+// by the caller: [0] medium count, [1] huge count, [2] huge table slots).
 struct DedupLists {
   uint32_t *medium;    // a2 of buckets with wave_cap < size <= block_cap
   uint32_t *huge;      // a2 of buckets above block_cap
@@ -129,7 +138,8 @@ struct DedupLists {
 };
 constexpr uint32_t kDedupWaveCap = 512;
 constexpr uint32_t kDedupBlockCap = 8192;
-constexpr uint64_t kHugeChunk = 16384;  // elements (or table slots) per huge-bucket chunk
+constexpr uint64_t kHugeChunk = 16384;  // elements (or table slots) per huge-bucket chunk (tables)
+constexpr uint32_t kBitsChunk = 16384;   // elements per huge-bucket chunk (bitmaps)
 // A fixed-size slice of a huge bucket: elements or table slots [lo, hi).
 struct HugeChunk {
   uint64_t base;      // bucket offset of the huge bucket
@@ -140,26 +150,30 @@ struct HugeChunk {
   uint32_t a2;        // bucket (next-level vertex)
   uint32_t pad;
 };
-void dedup_small(uint64_t n_next, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq, DedupLists l,
-                 hipStream_t s);
-void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, uint32_t *bucket,
-                  uint32_t *uniq, hipStream_t s);
+void dedup_small(uint64_t n_next, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
+                 DedupLists l, hipStream_t s);
+void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, const uint32_t *bucket,
+                  uint32_t *out, uint32_t *uniq, hipStream_t s);
 // out[2h] = bucket offset, out[2h+1] = size of huge bucket h.
 void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out, hipStream_t s);
 // Huge buckets against bitmaps of `words` u32 per bucket over the label space
-// (duplicates become kNone in place, uniq holds the kept counts).
-void dedup_huge_bits(const HugeChunk *elem_chunks, uint32_t n_elem, uint32_t n_huge, const uint32_t *huge,
-                     uint32_t *bitmaps, uint64_t words, uint32_t *bucket, uint32_t *uniq, hipStream_t s);
+// (cstart: n_huge + 1 words of scratch for the chunk plan).
+void dedup_huge_bits(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
+                     uint64_t *cstart, uint32_t *bitmaps, uint64_t words, const uint32_t *bucket, uint32_t *out,
+                     uint32_t *uniq, uint64_t *scratch, hipStream_t s);
+// Huge buckets in global open-addressing tables (pre-filled with kNone by the
+// caller, hcount zeroed), for label spaces whose bitmaps would not fit.
 void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,
-                uint32_t *tables, uint32_t *bucket, uint32_t *uniq, uint32_t *hcount, hipStream_t s);
+                uint32_t n_huge, const uint32_t *huge, uint32_t *tables, const uint32_t *bucket, uint32_t *out,
+                uint32_t *uniq, uint32_t *hcount, hipStream_t s);
 
 // off_next = exclusive scan of uniq (n_next+1 entries); sizes[2] = total.
 void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint64_t *sizes,
                  uint64_t *scratch, hipStream_t s);
-// Stream compaction of bucket[0..total) dropping kNone into col_next, each
-// value mapped through map (identity when null).
-void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, const uint32_t *map, uint64_t *scratch,
-             hipStream_t s);
+// col_next[off_next[a2] + i] = map[vals[seg_off[a2] + i]] for i < uniq[a2]
+// (map null: the value itself): the unique runs of the dedup, concatenated.
+void gather_uniques(uint64_t m2_next, uint64_t n_next, const uint64_t *off_next, const uint64_t *seg_off,
+                    const uint32_t *vals, const uint32_t *map, uint32_t *col_next, hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
 void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, uint64_t *total, uint64_t *scratch,
                                hipStream_t s);

@@ -651,8 +651,8 @@ struct SegLoad {
   }
 };
 struct SegStore {
-  const uint32_t *sk, *sv, *gid;
-  uint32_t *pos_of, *newidx, *old_of, *gid_next;
+  const uint32_t *sk, *sv, *gid, *rank;
+  uint32_t *pos_of, *newidx, *seg_of_cid, *gid_next;
   uint64_t *seg_off;
   __device__ void operator()(uint64_t i, Pair64 prefix, Pair64 v) const {
     pos_of[sv[i]] = static_cast<uint32_t>(prefix.b);
@@ -660,7 +660,7 @@ struct SegStore {
       const uint32_t a = sk[i];
       const uint32_t seg = static_cast<uint32_t>(prefix.a);
       newidx[a] = seg;
-      old_of[seg] = a;
+      seg_of_cid[rank[a]] = seg;
       gid_next[seg] = gid ? gid[a] : a;
       seg_off[seg] = prefix.b;
     }
@@ -779,8 +779,9 @@ __device__ __forceinline__ bool lds_insert(uint32_t *tab, uint32_t mask, uint32_
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint64_t *seg_off, uint32_t *bucket,
-                                                       uint32_t *uniq, DedupLists l) {
+__global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint64_t *seg_off,
+                                                       const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
+                                                       DedupLists l) {
   __shared__ uint32_t tabs[kBlock / kWave][kWaveTab];
   const int w = threadIdx.x / kWave;
   const int ln = threadIdx.x & 63;
@@ -807,7 +808,10 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
       continue;
     }
     if (s <= 1) {
-      if (ln == 0) uniq[a2] = s;
+      if (ln == 0) {
+        uniq[a2] = s;
+        if (s) out[base] = bucket[base];
+      }
       continue;
     }
     uint32_t tsz = 2;
@@ -824,17 +828,17 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
       const uint32_t val = j < tsz ? tab[j] : kNone;
       const bool keep = val != kNone;
       const uint64_t m = __ballot(keep);
-      if (keep) bucket[base + written + __popcll(m & lanes_below())] = val;
+      if (keep) out[base + written + __popcll(m & lanes_below())] = val;
       written += __popcll(m);
     }
-    for (uint32_t j = written + ln; j < s; j += kWave) bucket[base + j] = kNone;
     if (ln == 0) uniq[a2] = written;
     wave_fence();
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, const uint32_t *medium,
-                                                        const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq) {
+                                                        const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
+                                                        uint32_t *uniq) {
   __shared__ uint32_t tab[kBlockTab];
   __shared__ uint32_t psum[kBlock];
   const int tid = threadIdx.x;
@@ -857,9 +861,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, cons
     uint32_t tot;
     uint32_t p = block_exclusive_scan_shift<uint32_t>(c, psum, &tot);
     for (uint32_t j = j0; j < j0 + per && j < tsz; ++j)
-      if (tab[j] != kNone) bucket[base + p++] = tab[j];
-    __syncthreads();
-    for (uint32_t j = tot + tid; j < s; j += kBlock) bucket[base + j] = kNone;
+      if (tab[j] != kNone) out[base + p++] = tab[j];
     if (tid == 0) uniq[a2] = tot;
     __syncthreads();
   }
@@ -894,7 +896,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_insert(const HugeChunk *c
 // coalesced 64-slot rows twice: first to count the kept values (one atomic per
 // block reserves the output range), then to write them with ballot prefixes.
 __global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(const HugeChunk *chunks, uint32_t n_chunks,
-                                                           const uint32_t *tables, uint32_t *bucket, uint32_t *hcount) {
+                                                           const uint32_t *tables, uint32_t *dst, uint32_t *hcount) {
   __shared__ uint32_t wave_tot[kBlock / kWave];
   __shared__ uint32_t at;
   const int w = threadIdx.x / kWave;
@@ -921,7 +923,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(const HugeChunk *chu
       const uint32_t val = j < h.hi ? tab[j] : kNone;
       const bool keep = val != kNone;
       const uint64_t m = __ballot(keep);
-      if (keep) bucket[h.base + out + __popcll(m & lanes_below())] = val;
+      if (keep) dst[h.base + out + __popcll(m & lanes_below())] = val;
       out += __popcll(m);
     }
     __syncthreads();
@@ -929,43 +931,85 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(const HugeChunk *chu
 }
 
 // Huge buckets against per-bucket bitmaps over the label space: the first
-// lane to set a value's bit keeps it, every other copy becomes kNone in place
-// (read before the atomic: a set bit is never stale, bits only get set), and
-// each block adds its kept count to uniq of the bucket.
-__global__ __launch_bounds__(kBlock) void k_dedup_huge_bits(const HugeChunk *chunks, uint32_t n_chunks,
-                                                           uint32_t *bitmaps, uint64_t words, uint32_t *bucket,
-                                                           uint32_t *uniq) {
-  __shared__ uint32_t wave_kept[kBlock / kWave];
-  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-    const HugeChunk h = chunks[c];
-    uint32_t *bm = bitmaps + static_cast<uint64_t>(h.huge) * words;
-    uint32_t kept = 0;
-    for (uint64_t j = h.lo + threadIdx.x; j < h.hi; j += kBlock) {
-      uint32_t *slot = &bucket[h.base + j];
-      const uint32_t v = *slot;
-      if ((v >> 5) >= words) {  // outside the label space: kept unchecked (never the case)
-        ++kept;
-        continue;
-      }
-      const uint32_t bit = 1u << (v & 31u);
-      uint32_t *w = &bm[v >> 5];
-      bool dup = (*w & bit) != 0;
-      if (!dup) dup = (atomicOr(w, bit) & bit) != 0;
-      if (dup)
-        *slot = kNone;
+// lane to set a value's bit keeps it (read before the atomic: a set bit is
+// never stale, bits only get set).  The chunks are numbered bucket after
+// bucket (cstart[h] = first chunk of huge bucket h, *total = all chunks) and
+// found by binary search; they are spread over the whole grid, so few blocks
+// work on one bitmap word at a time (the device performs these atomics beyond
+// the per-XCD caches).  A block first drops the repeats inside its chunk
+// through a direct-mapped LDS filter -- a value is dropped there only when a
+// lane of the block holding the same value went on to the bitmap, so exactly
+// one copy of every value is kept per bucket -- then stages the kept values in
+// LDS and appends them to the bucket's run in out, reserving the range with
+// one atomic on uniq (zeroed for the huge buckets).
+constexpr int kHugeFilter = 4096;
+constexpr uint32_t kStagePiece = 4096;  // entries of a chunk staged between two appends
+
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_bits(uint32_t n_huge, const uint32_t *huge,
+                                                           const uint64_t *seg_off, const uint64_t *cstart,
+                                                           const uint64_t *total, uint32_t *bitmaps, uint64_t words,
+                                                           const uint32_t *bucket, uint32_t *out, uint32_t *uniq) {
+  __shared__ uint32_t filt[kHugeFilter];
+  __shared__ uint32_t stage[kStagePiece];
+  __shared__ uint32_t n_kept, at;
+  const uint64_t T = *total;
+  for (uint64_t c = blockIdx.x; c < T; c += gridDim.x) {
+    uint32_t lo = 0, hi = n_huge;  // cstart[lo] <= c < cstart[hi] (cstart[n_huge] = T)
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (cstart[mid] <= c)
+        lo = mid;
       else
-        ++kept;
+        hi = mid;
     }
-    for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
-    if (lane_id() == 0) wave_kept[threadIdx.x / kWave] = kept;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t t = wave_kept[0] + wave_kept[1] + wave_kept[2] + wave_kept[3];
-      if (t) atomicAdd(&uniq[h.a2], t);
+    const uint32_t a2 = huge[lo];
+    const uint64_t base = seg_off[a2], size = seg_off[a2 + 1] - base;
+    const uint64_t e_lo = (c - cstart[lo]) * kBitsChunk, e_hi = min(size, e_lo + kBitsChunk);
+    uint32_t *bm = bitmaps + static_cast<uint64_t>(lo) * words;
+    for (int j = threadIdx.x; j < kHugeFilter; j += kBlock) filt[j] = kNone;
+    for (uint64_t p0 = e_lo; p0 < e_hi; p0 += kStagePiece) {
+      const uint64_t p1 = min(e_hi, p0 + kStagePiece);
+      if (threadIdx.x == 0) n_kept = 0;
+      __syncthreads();
+      for (uint64_t j = p0 + threadIdx.x; j < p1; j += kBlock) {
+        const uint32_t v = bucket[base + j];
+        bool keep;
+        if ((v >> 5) >= words) {  // outside the label space: kept unchecked (never the case)
+          keep = true;
+        } else {
+          const uint32_t f = hash32(v) & (kHugeFilter - 1);
+          keep = filt[f] != v;
+          if (keep) {
+            filt[f] = v;
+            const uint32_t bit = 1u << (v & 31u);
+            uint32_t *w = &bm[v >> 5];
+            keep = (*w & bit) == 0 && (atomicOr(w, bit) & bit) == 0;
+          }
+        }
+        const uint64_t m = __ballot(keep);
+        uint32_t wbase = 0;
+        if (lane_id() == 0 && m) wbase = atomicAdd(&n_kept, static_cast<uint32_t>(__popcll(m)));
+        wbase = __shfl(wbase, 0);
+        if (keep) stage[wbase + __popcll(m & lanes_below())] = v;
+      }
+      __syncthreads();
+      const uint32_t k = n_kept;
+      if (threadIdx.x == 0) at = k ? atomicAdd(&uniq[a2], k) : 0u;
+      __syncthreads();
+      for (uint32_t j = threadIdx.x; j < k; j += kBlock) out[base + at + j] = stage[j];
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
+
+struct HugeChunkLoad {
+  const uint32_t *huge;
+  const uint64_t *seg_off;
+  __device__ uint64_t operator()(uint64_t h) const {
+    const uint32_t a2 = huge[h];
+    return (seg_off[a2 + 1] - seg_off[a2] + kBitsChunk - 1) / kBitsChunk;
+  }
+};
 
 __global__ void k_zero_huge_uniq(uint32_t n_huge, const uint32_t *huge, uint32_t *uniq) {
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -981,17 +1025,9 @@ __global__ void k_huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64
   out[2 * h + 1] = seg_off[a2 + 1] - seg_off[a2];
 }
 
-// Element chunks again: positions at or after the unique count become kNone;
-// the first chunk of a bucket stores the count.
-__global__ __launch_bounds__(kBlock) void k_dedup_huge_tail(const HugeChunk *chunks, uint32_t n_chunks,
-                                                           uint32_t *bucket, uint32_t *uniq, const uint32_t *hcount) {
-  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-    const HugeChunk h = chunks[c];
-    const uint32_t u = hcount[h.huge];
-    if (h.lo == 0 && threadIdx.x == 0) uniq[h.a2] = u;
-    const uint64_t from = h.lo > u ? h.lo : u;
-    for (uint64_t j = from + threadIdx.x; j < h.hi; j += kBlock) bucket[h.base + j] = kNone;
-  }
+__global__ void k_huge_counts(uint32_t n_huge, const uint32_t *huge, const uint32_t *hcount, uint32_t *uniq) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < n_huge) uniq[huge[h]] = hcount[h];
 }
 
 // ---------------------------------------------------------------------------
@@ -1008,33 +1044,33 @@ struct OffStore {
     if (i + 1 == n) off[n] = prefix + v;
   }
 };
-// Compaction: count non-kNone per tile, scan, write.
-__global__ __launch_bounds__(kBlock) void k_compact_count(uint64_t total, const uint32_t *src, uint64_t *counts) {
-  __shared__ uint64_t smem[kBlock];
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
-  uint64_t c = 0;
-  for (int j = 0; j < kScanPerThread; ++j) {
-    const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
-    if (i < total) c += src[i] != kNone;
-  }
-  uint64_t tot;
-  block_exclusive_scan_shift<uint64_t>(c, smem, &tot);
-  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
-}
+// Next-level columns: entry k of the CSR copies the (k - off[a2])-th unique
+// value of bucket a2 (the row holding k), mapped to its next-level index.
+// Each wave covers kGatherSteps * 64 consecutive entries, coalesced: a lane
+// finds the row of its first entry by binary search over the offsets, then
+// walks forward to the rows of its later entries (64 apart).
+constexpr int kGatherSteps = 8;
 
-__global__ __launch_bounds__(kBlock) void k_compact_write(uint64_t total, const uint32_t *src, uint32_t *dst,
-                                                         const uint64_t *prefix, const uint32_t *map) {
-  __shared__ uint64_t smem[kBlock];
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
-  uint64_t out = prefix[blockIdx.x];
-  for (int j = 0; j < kScanPerThread; ++j) {
-    const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
-    const uint32_t val = i < total ? src[i] : kNone;
-    const uint64_t keep = val != kNone;
-    uint64_t tot;
-    uint64_t p = block_exclusive_scan_shift<uint64_t>(keep, smem, &tot);
-    if (keep) dst[out + p] = map ? map[val] : val;
-    out += tot;
+__global__ __launch_bounds__(kBlock) void k_gather_uniques(uint64_t m2_next, uint64_t n_next, const uint64_t *off_next,
+                                                          const uint64_t *seg_off, const uint32_t *uniq_vals,
+                                                          const uint32_t *map, uint32_t *col_next) {
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x) / kWave;
+  const uint64_t k0 = wave * (kGatherSteps * kWave) + lane_id();
+  if (k0 >= m2_next) return;
+  uint64_t lo = 0, hi = n_next;  // off_next[lo] <= k0 < off_next[hi]
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (off_next[mid] <= k0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  for (int i = 0; i < kGatherSteps; ++i) {
+    const uint64_t k = k0 + static_cast<uint64_t>(i) * kWave;
+    if (k >= m2_next) break;
+    while (off_next[lo + 1] <= k) ++lo;
+    const uint32_t v = uniq_vals[seg_off[lo] + (k - off_next[lo])];
+    col_next[k] = map ? map[v] : v;
   }
 }
 
@@ -1272,17 +1308,52 @@ void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_
 
 uint64_t scan_scratch_words(uint64_t n) { return scan_words_for(n, 2) + 64; }
 
+__global__ void k_mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used) {
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) used[lab[v]] = 1;
+}
+
+void mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_mark_labels, dim3(grid_for(n)), dim3(kBlock), 0, s, n, lab, used);
+  KC_LAUNCH_CHECK();
+}
+
+struct RankStore {
+  uint32_t *rank;
+  __device__ void operator()(uint64_t i, uint64_t prefix, uint64_t v) const {
+    if (v) rank[i] = static_cast<uint32_t>(prefix);
+  }
+};
+
+void cluster_ranks(uint64_t n, const uint32_t *used, uint32_t *rank, uint64_t *count, uint64_t *scratch,
+                   hipStream_t s) {
+  scan_exclusive<uint64_t>(n, ArrLoadU32{used}, RankStore{rank}, count, scratch, s);
+}
+
+__global__ void k_cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid) {
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) cid[v] = rank[lab[v]];
+}
+
+void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cluster_ids, dim3(grid_for(n)), dim3(kBlock), 0, s, n, lab, rank, cid);
+  KC_LAUNCH_CHECK();
+}
+
 void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *sk, uint32_t *sv,
                   uint64_t *count, uint64_t *scratch, hipStream_t s) {
   scan_exclusive<uint64_t>(n, SelectLoad{deg_inter}, SelectStore{key, sk, sv}, count, scratch, s);
 }
 
 void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t *deg_inter, const uint32_t *gid,
-              uint32_t *pos_of, uint32_t *newidx, uint32_t *old_of, uint32_t *gid_next, uint64_t *seg_off,
-              uint64_t *sizes, uint64_t *scratch, hipStream_t s) {
+              const uint32_t *rank, uint32_t *pos_of, uint32_t *newidx, uint32_t *seg_of_cid, uint32_t *gid_next,
+              uint64_t *seg_off, uint64_t *sizes, uint64_t *scratch, hipStream_t s) {
   Pair64 *total = reinterpret_cast<Pair64 *>(scratch);
-  scan_exclusive<Pair64>(N, SegLoad{sk, sv, deg_inter}, SegStore{sk, sv, gid, pos_of, newidx, old_of, gid_next, seg_off},
-                         total, scratch + 4, s);
+  scan_exclusive<Pair64>(N, SegLoad{sk, sv, deg_inter},
+                         SegStore{sk, sv, gid, rank, pos_of, newidx, seg_of_cid, gid_next, seg_off}, total, scratch + 4,
+                         s);
   hipLaunchKernelGGL(k_seg_finish, dim3(1), dim3(1), 0, s, total, seg_off, sizes);
   KC_LAUNCH_CHECK();
 }
@@ -1309,45 +1380,51 @@ void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const ui
   KC_LAUNCH_CHECK();
 }
 
-void dedup_small(uint64_t n_next, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq, DedupLists l,
-                 hipStream_t s) {
+void dedup_small(uint64_t n_next, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
+                 DedupLists l, hipStream_t s) {
   if (!n_next) return;
   uint64_t blocks = ceil_div(n_next, kBlock / kWave);
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_dedup_small, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n_next, seg_off, bucket,
-                     uniq, l);
+                     out, uniq, l);
   KC_LAUNCH_CHECK();
 }
 
-void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, uint32_t *bucket, uint32_t *uniq,
-                  hipStream_t s) {
+void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, const uint32_t *bucket,
+                  uint32_t *out, uint32_t *uniq, hipStream_t s) {
   if (!n_medium) return;
   unsigned blocks = n_medium > 2048 ? 2048 : n_medium;
-  hipLaunchKernelGGL(k_dedup_medium, dim3(blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off, bucket, uniq);
+  hipLaunchKernelGGL(k_dedup_medium, dim3(blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off, bucket, out, uniq);
   KC_LAUNCH_CHECK();
 }
 
 void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,
-                uint32_t *tables, uint32_t *bucket, uint32_t *uniq, uint32_t *hcount, hipStream_t s) {
+                uint32_t n_huge, const uint32_t *huge, uint32_t *tables, const uint32_t *bucket, uint32_t *out,
+                uint32_t *uniq, uint32_t *hcount, hipStream_t s) {
   if (!n_elem) return;
   const unsigned ge = n_elem < 8192 ? n_elem : 8192, gs = n_slot < 8192 ? n_slot : 8192;
   hipLaunchKernelGGL(k_dedup_huge_insert, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, tables, bucket);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dedup_huge_emit, dim3(gs), dim3(kBlock), 0, s, slot_chunks, n_slot, tables, bucket, hcount);
+  hipLaunchKernelGGL(k_dedup_huge_emit, dim3(gs), dim3(kBlock), 0, s, slot_chunks, n_slot, tables, out, hcount);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dedup_huge_tail, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, bucket, uniq, hcount);
+  hipLaunchKernelGGL(k_huge_counts, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s, n_huge,
+                     huge, hcount, uniq);
   KC_LAUNCH_CHECK();
 }
 
-void dedup_huge_bits(const HugeChunk *elem_chunks, uint32_t n_elem, uint32_t n_huge, const uint32_t *huge,
-                     uint32_t *bitmaps, uint64_t words, uint32_t *bucket, uint32_t *uniq, hipStream_t s) {
-  if (!n_elem) return;
+void dedup_huge_bits(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
+                     uint64_t *cstart, uint32_t *bitmaps, uint64_t words, const uint32_t *bucket, uint32_t *out,
+                     uint32_t *uniq, uint64_t *scratch, hipStream_t s) {
+  if (!n_huge) return;
+  uint64_t *total = cstart + n_huge;
+  scan_exclusive<uint64_t>(n_huge, HugeChunkLoad{huge, seg_off}, ArrayStore<uint64_t>{cstart}, total, scratch, s);
   fill_u32(bitmaps, static_cast<uint64_t>(n_huge) * words, 0u, s);
   hipLaunchKernelGGL(k_zero_huge_uniq, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s, n_huge,
                      huge, uniq);
   KC_LAUNCH_CHECK();
-  const unsigned ge = n_elem < 8192 ? n_elem : 8192;
-  hipLaunchKernelGGL(k_dedup_huge_bits, dim3(ge), dim3(kBlock), 0, s, elem_chunks, n_elem, bitmaps, words, bucket, uniq);
+  const uint64_t g = std::min<uint64_t>(8192, max_chunks);
+  hipLaunchKernelGGL(k_dedup_huge_bits, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, n_huge, huge, seg_off,
+                     cstart, total, bitmaps, words, bucket, out, uniq);
   KC_LAUNCH_CHECK();
 }
 
@@ -1364,19 +1441,12 @@ void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint
   KC_LAUNCH_CHECK();
 }
 
-void compact(uint64_t total, const uint32_t *bucket, uint32_t *col_next, const uint32_t *map, uint64_t *scratch,
-             hipStream_t s) {
-  if (!total) return;
-  const uint64_t blocks = ceil_div(total, kScanTile);
-  uint64_t *counts = scratch;
-  uint64_t *prefix = scratch + blocks + 1;
-  uint64_t *tot = prefix + blocks + 1;
-  uint64_t *rest = tot + 4;
-  hipLaunchKernelGGL(k_compact_count, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, total, bucket, counts);
-  KC_LAUNCH_CHECK();
-  scan_exclusive<uint64_t>(blocks, ArrayLoad<uint64_t>{counts}, ArrayStore<uint64_t>{prefix}, tot, rest, s);
-  hipLaunchKernelGGL(k_compact_write, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, total, bucket, col_next,
-                     prefix, map);
+void gather_uniques(uint64_t m2_next, uint64_t n_next, const uint64_t *off_next, const uint64_t *seg_off,
+                    const uint32_t *vals, const uint32_t *map, uint32_t *col_next, hipStream_t s) {
+  if (!m2_next) return;
+  const uint64_t blocks = ceil_div(m2_next, static_cast<uint64_t>(kBlock) * kGatherSteps);
+  hipLaunchKernelGGL(k_gather_uniques, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, m2_next, n_next,
+                     off_next, seg_off, vals, map, col_next);
   KC_LAUNCH_CHECK();
 }
 
