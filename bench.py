@@ -258,7 +258,8 @@ def batched_leg(a, stream):
     print(f"[bench] batch: {b.graphs} graphs, n={n} m={m} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
     db = ca.DeviceBatch(b)
     tune = db.autotune(seed=1001, stream=stream)
-    db.set_tuning(block=tune["block"], groups_per_cu=tune["groups_per_cu"], use_graph=True)
+    db.set_tuning(block=tune["block"], groups_per_cu=tune["groups_per_cu"], use_graph=True,
+                  split_vertices=tune["split_vertices"])
     print(f"[bench] batch: tuned {tune}", file=sys.stderr, flush=True)
     for i in range(a.warmup):
         db.run(copy_labels=False, seed=1 + i + 1000, stream=stream)

@@ -90,7 +90,8 @@ class BatchGenParams(ctypes.Structure):
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("block", ctypes.c_int32), ("groups_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+    _fields_ = [("block", ctypes.c_int32), ("groups_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("split_vertices", ctypes.c_uint32)]
 
 
 class BatchStats(ctypes.Structure):
@@ -518,7 +519,8 @@ def batch_generate(graphs=4096, log_n_min=8.0, log_n_max=18.0, deg_min=2.0, deg_
 def batch_stats_dict(st):
     d = {k: getattr(st, k) for k, _ in BatchStats._fields_ if k != "tuning"}
     t = st.tuning
-    d["tuning"] = {"block": t.block, "groups_per_cu": t.groups_per_cu, "use_graph": t.use_graph}
+    d["tuning"] = {"block": t.block, "groups_per_cu": t.groups_per_cu, "use_graph": t.use_graph,
+                   "split_vertices": t.split_vertices}
     return d
 
 
@@ -548,14 +550,16 @@ class DeviceBatch:
         except Exception:
             pass
 
-    def set_tuning(self, block=256, groups_per_cu=4, use_graph=True):
-        t = BatchTuning(block, groups_per_cu, 1 if use_graph else 0)
+    def set_tuning(self, block=256, groups_per_cu=4, use_graph=True, split_vertices=0):
+        """split_vertices: graphs with more vertices go through the device-wide path as one union."""
+        t = BatchTuning(block, groups_per_cu, 1 if use_graph else 0, split_vertices)
         _check(lib().kc_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, **kw):
         t = BatchTuning()
         _check(lib().kc_batch_autotune(self.h, ctypes.byref(options(**kw)), ctypes.byref(t)))
-        return {"block": t.block, "groups_per_cu": t.groups_per_cu, "use_graph": t.use_graph}
+        return {"block": t.block, "groups_per_cu": t.groups_per_cu, "use_graph": t.use_graph,
+                "split_vertices": t.split_vertices}
 
     def run(self, copy_labels=True, **kw):
         st = BatchStats()

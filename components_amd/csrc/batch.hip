@@ -44,7 +44,8 @@ struct BatchParams {
 
 struct BatchDev {
   uint32_t graphs;
-  const uint32_t *order;  // graphs by decreasing edge count
+  uint32_t queued;        // graphs the persistent kernel takes (the first entries of order)
+  const uint32_t *order;  // those graphs by decreasing edge count
   const uint64_t *vtx_off, *edge_off;
   const uint64_t *off;  // global row offsets into col (sum n + 1)
   const uint32_t *col;  // local neighbour ids
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(B) void k_batch(BatchDev d) {
     __syncthreads();
     const uint32_t qi = s.graph;
     __syncthreads();
-    if (qi >= d.graphs) break;
+    if (qi >= d.queued) break;
     contract_graph<B>(d, p, d.order[qi], s);
   }
 }
@@ -419,6 +420,87 @@ __global__ void k_table_clear(uint64_t n, uint64_t *tab) {
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
     tab[i] = kEmpty;
+}
+
+// Union of the large graphs: union vertex u (or edge j) lies in large graph k,
+// the largest k with uoff[k] <= u (ueoff[k] <= j); its rows and columns are
+// those of batch graph big[k], shifted to the union's numbering.
+__device__ __forceinline__ uint32_t union_part(const uint64_t *start, uint32_t parts, uint64_t x) {
+  uint32_t lo = 0, hi = parts;  // start[lo] <= x < start[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (start[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_union_rows(uint64_t n_union, uint32_t parts, const uint64_t *uoff, const uint64_t *ueoff,
+                             const uint32_t *big, const uint64_t *vtx_off, const uint64_t *edge_off,
+                             const uint64_t *off, uint64_t *uoff_rows) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u <= n_union;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (u == n_union) {
+      uoff_rows[u] = ueoff[parts];
+      continue;
+    }
+    const uint32_t k = union_part(uoff, parts, u);
+    const uint32_t g = big[k];
+    uoff_rows[u] = off[vtx_off[g] + (u - uoff[k])] - edge_off[g] + ueoff[k];
+  }
+}
+
+__global__ void k_union_cols(uint64_t m2_union, uint32_t parts, const uint64_t *uoff, const uint64_t *ueoff,
+                             const uint32_t *big, const uint64_t *edge_off, const uint32_t *col, uint32_t *ucol) {
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < m2_union;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t k = union_part(ueoff, parts, j);
+    ucol[j] = col[edge_off[big[k]] + (j - ueoff[k])] + static_cast<uint32_t>(uoff[k]);
+  }
+}
+
+// Statistics of the large graphs: the union run's levels, sweeps and radius
+// sum, status 0, and the component count zeroed for k_union_labels.
+__global__ void k_union_stats(uint32_t parts, const uint32_t *big, uint32_t levels, uint32_t sweeps, uint32_t radius,
+                              uint32_t *glevels, uint32_t *gsweeps, uint32_t *gradius, uint32_t *gstatus,
+                              uint32_t *gcomp) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= parts) return;
+  const uint32_t g = big[k];
+  glevels[g] = levels;
+  gsweeps[g] = sweeps;
+  gradius[g] = radius;
+  gstatus[g] = 0;
+  gcomp[g] = 0;
+}
+
+// Local canonical labels of the large graphs from the union's (a component
+// never leaves its graph, so its minimum union id lies in the same graph),
+// and each graph's component count (vertices that are their own label).
+__global__ void k_union_labels(uint64_t n_union, uint32_t parts, const uint64_t *uoff, const uint32_t *big,
+                               const uint64_t *vtx_off, const uint32_t *ulab, uint32_t *labels, uint32_t *gcomp) {
+  const uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const bool act = u < n_union;
+  uint32_t g = 0;
+  bool root = false;
+  if (act) {
+    const uint32_t k = union_part(uoff, parts, u);
+    g = big[k];
+    const uint32_t l = ulab[u];
+    labels[vtx_off[g] + (u - uoff[k])] = l - static_cast<uint32_t>(uoff[k]);
+    root = l == static_cast<uint32_t>(u);
+  }
+  // one atomic per wave when the wave's vertices share a graph
+  const uint32_t g0 = __shfl(g, 0);
+  const bool same = __ballot(act && g != g0) == 0;
+  if (same) {
+    const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(root)));
+    if (lane_id() == 0 && c) atomicAdd(&gcomp[g0], c);
+  } else if (root) {
+    atomicAdd(&gcomp[g], 1u);
+  }
 }
 
 }  // namespace
@@ -459,8 +541,17 @@ struct kc_batch {
   uint32_t graphs = 0;
   uint64_t n = 0, m2 = 0;
   int cus = 0;
-  std::vector<uint64_t> h_vtx_off;
+  std::vector<uint64_t> h_vtx_off, h_edge_off;
   std::vector<void *> owned;
+  uint32_t *order_d = nullptr;
+  // graphs above the split size go through the device-wide path as one union
+  bool split_valid = false;
+  uint32_t split_built = 0;
+  kc_graph *big = nullptr;
+  uint32_t n_big = 0;
+  uint64_t n_union = 0, m2_union = 0;
+  uint32_t *d_big = nullptr;
+  uint64_t *d_uoff = nullptr, *d_ueoff = nullptr;
   BatchDev dev{};
   BatchParams *d_params = nullptr, *h_params = nullptr;
   uint64_t *table = nullptr;
@@ -470,9 +561,9 @@ struct kc_batch {
   uint64_t table_cap = 0;
   double table_beta = -1.0;
   uint32_t table_k = 0;
-  kc_batch_tuning tuning{256, 4, 1};
+  kc_batch_tuning tuning{256, 4, 1, 0};
   hipGraphExec_t exec = nullptr;
-  kc_batch_tuning exec_tuning{0, 0, 0};
+  kc_batch_tuning exec_tuning{0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool ran = false;
 
@@ -486,6 +577,7 @@ struct kc_batch {
   }
   ~kc_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (big) (void)kc_graph_destroy(big);
     if (exec) (void)hipGraphExecDestroy(exec);
     for (void *p : owned) (void)hipFree(p);
     if (h_params) (void)hipHostFree(h_params);
@@ -530,6 +622,61 @@ void batch_params(kc_batch *b, const kc_options &o) {
   KC_HIP(hipMemcpyAsync(b->d_params, b->h_params, sizeof(BatchParams), hipMemcpyHostToDevice, b->stream));
 }
 
+// Routes the graphs with more than `split` vertices (none when 0) to the
+// device-wide path: the persistent kernel's queue gets the others, largest
+// first, and the large ones are copied into one union handle.
+void prepare_split(kc_batch *b, uint32_t split) {
+  if (b->split_valid && b->split_built == split) return;
+  b->split_valid = false;
+  std::vector<uint32_t> small, bigs;
+  for (uint32_t g = 0; g < b->graphs; ++g) {
+    const uint64_t ng = b->h_vtx_off[g + 1] - b->h_vtx_off[g];
+    (split && ng > split ? bigs : small).push_back(g);
+  }
+  std::stable_sort(small.begin(), small.end(), [&](uint32_t x, uint32_t y) {
+    return b->h_edge_off[x + 1] - b->h_edge_off[x] > b->h_edge_off[y + 1] - b->h_edge_off[y];
+  });
+  KC_HIP(hipStreamSynchronize(b->stream));  // the old queue order may still be read
+  if (!small.empty()) KC_HIP(hipMemcpy(b->order_d, small.data(), 4ull * small.size(), hipMemcpyHostToDevice));
+  b->dev.queued = static_cast<uint32_t>(small.size());
+  if (b->big) {
+    (void)kc_graph_destroy(b->big);
+    b->big = nullptr;
+  }
+  b->n_big = static_cast<uint32_t>(bigs.size());
+  b->n_union = b->m2_union = 0;
+  if (b->n_big) {
+    std::vector<uint64_t> uoff(b->n_big + 1, 0), ueoff(b->n_big + 1, 0);
+    for (uint32_t k = 0; k < b->n_big; ++k) {
+      const uint32_t g = bigs[k];
+      uoff[k + 1] = uoff[k] + (b->h_vtx_off[g + 1] - b->h_vtx_off[g]);
+      ueoff[k + 1] = ueoff[k] + (b->h_edge_off[g + 1] - b->h_edge_off[g]);
+    }
+    b->n_union = uoff[b->n_big];
+    b->m2_union = ueoff[b->n_big];
+    KC_HIP(hipMemcpy(b->d_big, bigs.data(), 4ull * b->n_big, hipMemcpyHostToDevice));
+    KC_HIP(hipMemcpy(b->d_uoff, uoff.data(), 8ull * (b->n_big + 1), hipMemcpyHostToDevice));
+    KC_HIP(hipMemcpy(b->d_ueoff, ueoff.data(), 8ull * (b->n_big + 1), hipMemcpyHostToDevice));
+    uint64_t *off = nullptr;
+    uint32_t *col = nullptr;
+    hipStream_t gs = nullptr;
+    b->big = graph_device_begin(b->n_union, b->m2_union, &off, &col, &gs);
+    const uint32_t gr = static_cast<uint32_t>(std::min<uint64_t>(ceil_div(b->n_union + 1, kBlock), 65536));
+    hipLaunchKernelGGL(k_union_rows, dim3(gr), dim3(kBlock), 0, gs, b->n_union, b->n_big, b->d_uoff, b->d_ueoff,
+                       b->d_big, b->dev.vtx_off, b->dev.edge_off, b->dev.off, off);
+    KC_LAUNCH_CHECK();
+    if (b->m2_union) {
+      const uint32_t gc = static_cast<uint32_t>(std::min<uint64_t>(ceil_div(b->m2_union, kBlock), 65536));
+      hipLaunchKernelGGL(k_union_cols, dim3(gc), dim3(kBlock), 0, gs, b->m2_union, b->n_big, b->d_uoff, b->d_ueoff,
+                         b->d_big, b->dev.edge_off, b->dev.col, col);
+      KC_LAUNCH_CHECK();
+    }
+    graph_device_finish(b->big);
+  }
+  b->split_built = split;
+  b->split_valid = true;
+}
+
 void enqueue(kc_batch *b, const kc_batch_tuning &t) {
   fill_u32(b->queue, 1, 0, b->stream);
   const uint32_t grid = static_cast<uint32_t>(b->cus) * static_cast<uint32_t>(t.groups_per_cu);
@@ -567,10 +714,35 @@ void launch(kc_batch *b) {
   KC_HIP(hipGraphLaunch(b->exec, b->stream));
 }
 
-// Runs once and checks every graph's status; returns the device time.
-float run_once(kc_batch *b) {
+// Runs once and checks every graph's status; returns the device time.  The
+// persistent kernel takes the small graphs while the device-wide path runs the
+// union of the large ones on the union handle's stream.
+float run_once(kc_batch *b, const kc_options &o) {
+  prepare_split(b, b->tuning.split_vertices);
   KC_HIP(hipEventRecord(b->ev0, b->stream));
   launch(b);
+  if (b->big) {
+    kc_options ob = o;
+    ob.use_bfs = 1;
+    ob.record_trace = 0;
+    ob.time_kernels = 0;
+    ob.learned = 0;
+    ob.seq_limit = std::max<uint64_t>(o.seq_limit, b->m2_union / 2 / 256);
+    kc_stats st;
+    if (kc_find_components(b->big, &ob, nullptr, &st) != KC_OK)
+      fail(KC_ERR_INVALID, "batch: large graphs: %s", kc_last_error());
+    hipLaunchKernelGGL(k_union_stats, dim3(static_cast<unsigned>(ceil_div(b->n_big, kBlock))), dim3(kBlock), 0,
+                       b->stream, b->n_big, b->d_big, st.levels, static_cast<uint32_t>(st.sweeps),
+                       static_cast<uint32_t>(st.radius_sum), b->dev.glevels, b->dev.gsweeps, b->dev.gradius,
+                       b->dev.gstatus, b->dev.gcomp);
+    KC_LAUNCH_CHECK();
+    if (b->n_union) {
+      hipLaunchKernelGGL(k_union_labels, dim3(static_cast<unsigned>(ceil_div(b->n_union, kBlock))), dim3(kBlock), 0,
+                         b->stream, b->n_union, b->n_big, b->d_uoff, b->d_big, b->dev.vtx_off,
+                         graph_device_labels(b->big), b->dev.labels, b->dev.gcomp);
+      KC_LAUNCH_CHECK();
+    }
+  }
   KC_HIP(hipEventRecord(b->ev1, b->stream));
   KC_HIP(hipStreamSynchronize(b->stream));
   float ms = 0.f;
@@ -630,11 +802,6 @@ int kc_batch_create(const kc_batch_host *h, kc_batch **out) {
       while (tw < 2 * slots) tw <<= 1;
       toff[g + 1] = toff[g] + tw;
     }
-    std::vector<uint32_t> order(G);
-    for (uint32_t g = 0; g < G; ++g) order[g] = g;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-      return h->edge_off[x + 1] - h->edge_off[x] > h->edge_off[y + 1] - h->edge_off[y];
-    });
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(KC_ERR_HIP, "no HIP device available");
     std::unique_ptr<kc_batch> b(new kc_batch());
@@ -650,16 +817,20 @@ int kc_batch_create(const kc_batch_host *h, kc_batch **out) {
     b->n = n;
     b->m2 = m2;
     b->h_vtx_off.assign(h->vtx_off, h->vtx_off + G + 1);
+    b->h_edge_off.assign(h->edge_off, h->edge_off + G + 1);
     BatchDev &d = b->dev;
     d.graphs = G;
     uint32_t *order_d = b->dalloc<uint32_t>(G);
+    b->order_d = order_d;
+    b->d_big = b->dalloc<uint32_t>(G);
+    b->d_uoff = b->dalloc<uint64_t>(G + 1);
+    b->d_ueoff = b->dalloc<uint64_t>(G + 1);
     uint64_t *vtx = b->dalloc<uint64_t>(G + 1), *eoff = b->dalloc<uint64_t>(G + 1);
     uint64_t *off = b->dalloc<uint64_t>(n + 1);
     uint32_t *col = b->dalloc<uint32_t>(m2);
     uint32_t *ni = b->dalloc<uint32_t>(G);
     uint64_t *tabo = b->dalloc<uint64_t>(G + 1);
     if (G) {
-      KC_HIP(hipMemcpy(order_d, order.data(), 4ull * G, hipMemcpyHostToDevice));
       KC_HIP(hipMemcpy(vtx, h->vtx_off, 8ull * (G + 1), hipMemcpyHostToDevice));
       KC_HIP(hipMemcpy(eoff, h->edge_off, 8ull * (G + 1), hipMemcpyHostToDevice));
       KC_HIP(hipMemcpy(off, h->offsets, 8ull * (n + 1), hipMemcpyHostToDevice));
@@ -696,6 +867,7 @@ int kc_batch_create(const kc_batch_host *h, kc_batch **out) {
     d.params = b->d_params;
     batch_table_clear(b->table_words, b->table, b->stream);
     KC_HIP(hipStreamSynchronize(b->stream));
+    prepare_split(b.get(), 0);
     *out = b.release();
   });
 }
@@ -722,7 +894,7 @@ int kc_batch_run(kc_batch *b, const kc_options *opt, uint32_t *labels_out, kc_ba
       kc_options_init(&o);
     const auto t0 = std::chrono::steady_clock::now();
     batch_params(b, o);
-    const float ms = run_once(b);
+    const float ms = run_once(b, o);
     if (labels_out && b->n) KC_HIP(hipMemcpy(labels_out, b->dev.labels, 4ull * b->n, hipMemcpyDeviceToHost));
     const auto t1 = std::chrono::steady_clock::now();
     if (stats) {
@@ -760,23 +932,39 @@ int kc_batch_autotune(kc_batch *b, const kc_options *opt, kc_batch_tuning *chose
     else
       kc_options_init(&o);
     const kc_batch_tuning keep = b->tuning;
+    auto time_of = [&](const kc_batch_tuning &t) {
+      b->tuning = t;
+      batch_params(b, o);
+      run_once(b, o);  // warm
+      const float ms = run_once(b, o);
+      return std::min(ms, run_once(b, o));
+    };
+    // the split size first, at the current launch shape, then the shape
     kc_batch_tuning best = keep;
+    best.use_graph = 0;
     float best_ms = 0.f;
     bool have = false;
+    const uint32_t splits[] = {0u, 1u << 12, 1u << 14, 1u << 16};
+    for (uint32_t sp : splits) {
+      kc_batch_tuning t = best;
+      t.split_vertices = sp;
+      const float ms = time_of(t);
+      if (!have || ms < best_ms) {
+        best_ms = ms;
+        best = t;
+        have = true;
+      }
+    }
     const int blocks[] = {256, 512, 1024};
     const int gpcs[] = {1, 2, 4, 8};
     for (int blk : blocks)
       for (int gpc : gpcs) {
         if (blk * gpc > 2048) continue;  // at most 32 waves per compute unit
-        b->tuning = kc_batch_tuning{blk, gpc, 0};
-        batch_params(b, o);
-        run_once(b);  // warm
-        float ms = run_once(b);
-        ms = std::min(ms, run_once(b));
-        if (!have || ms < best_ms) {
+        kc_batch_tuning t{blk, gpc, 0, best.split_vertices};
+        const float ms = time_of(t);
+        if (ms < best_ms) {
           best_ms = ms;
-          best = b->tuning;
-          have = true;
+          best = t;
         }
       }
     best.use_graph = keep.use_graph;

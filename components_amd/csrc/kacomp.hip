@@ -1123,6 +1123,41 @@ int kc_graph_generate(const char *name, const kc_gen_params *pp, kc_graph **out)
   });
 }
 
+}  // extern "C"
+
+namespace kc {
+
+kc_graph *graph_device_begin(uint64_t n, uint64_t m2, uint64_t **off, uint32_t **col, hipStream_t *stream) {
+  if (n >= 0xFFFFFFFFull) fail(KC_ERR_INVALID, "n=%llu exceeds 32-bit vertex ids", (unsigned long long)n);
+  if (m2 >= (1ull << 32)) fail(KC_ERR_INVALID, "2m=%llu exceeds 2^32 directed edges", (unsigned long long)m2);
+  std::unique_ptr<kc_graph> g = new_handle();
+  g->n = n;
+  g->m2 = m2;
+  g->off0 = g->dalloc<uint64_t>(n + 1);
+  g->col0 = g->dalloc<uint32_t>(m2);
+  *off = g->off0;
+  *col = g->col0;
+  *stream = g->stream;
+  return g.release();
+}
+
+void graph_device_finish(kc_graph *g) {
+  hipStream_t s = g->stream;
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(g->sizes + 6);
+  fill_u64(g->sizes + 6, 1, 0, s);
+  nonzero_rows(g->n, g->off0, cnt, s);
+  KC_HIP(hipMemcpyAsync(g->h_sizes + 6, g->sizes + 6, 8, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  g->non_isolated = g->h_sizes[6];
+  init_work(g);
+}
+
+const uint32_t *graph_device_labels(const kc_graph *g) { return g->labels; }
+
+}  // namespace kc
+
+extern "C" {
+
 int kc_graph_download(const kc_graph *g, kc_csr *out) {
   return guarded([&] {
     if (!g || !out) fail(KC_ERR_INVALID, "null argument");

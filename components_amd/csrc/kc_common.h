@@ -183,4 +183,15 @@ __device__ __forceinline__ void peeled_min(uint32_t *arr, uint32_t key, uint32_t
   if (part && !same && val < arr[key]) atomicMin(&arr[key], val);
 }
 
+// Plain-path handles built from a CSR already in device memory (used by the
+// batched form for the union of its large graphs).  graph_device_begin
+// allocates the handle and its level-0 arrays (n + 1 offsets, m2 columns),
+// which the caller fills on the returned stream; graph_device_finish then
+// counts the non-isolated vertices and allocates the work buffers.  The
+// handle is released with kc_graph_destroy.
+kc_graph *graph_device_begin(uint64_t n, uint64_t m2, uint64_t **off, uint32_t **col, hipStream_t *stream);
+void graph_device_finish(kc_graph *g);
+// Device labels of the handle's last kc_find_components call.
+const uint32_t *graph_device_labels(const kc_graph *g);
+
 }  // namespace kc

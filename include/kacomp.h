@@ -259,6 +259,10 @@ typedef struct kc_batch_tuning {
   int32_t block;
   int32_t groups_per_cu;
   int32_t use_graph;
+  uint32_t split_vertices; /* graphs with more vertices go through the device-wide path of
+                              kc_find_components as one disjoint union, concurrently with the
+                              persistent kernel (0 = every graph in the persistent kernel).
+                              Their per-graph statistics are those of the union run. */
 } kc_batch_tuning;
 
 typedef struct kc_batch_stats {

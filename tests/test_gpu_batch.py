@@ -167,3 +167,35 @@ def test_larger_batch_labels():
         gr = b.graph(g)
         a, z = int(b.vtx_off[g]), int(b.vtx_off[g + 1])
         assert np.array_equal(lab[a:z], oracle.canonical_labels(gr.offsets, gr.targets)), g
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_split_routes_large_graphs(graph):
+    """Graphs above the split size go through the device-wide path as one
+    disjoint union while the persistent kernel takes the rest: labels stay the
+    per-graph canonical labels, the small graphs keep their own level counts,
+    and changing the split between calls rebuilds the union."""
+    b = ca.batch_generate(graphs=160, log_n_min=4, log_n_max=13, seed=31)
+    sizes = np.diff(b.vtx_off.astype(np.int64))
+    db = ca.DeviceBatch(b)
+    try:
+        ref, st0 = db.run(seed=9, stream=ca.STREAM_HASHED)
+        lv0 = db.graph_stats()[0].copy()
+        for split in (1 << 9, 1 << 11, 1 << 9, 0):
+            n_big = int((sizes > split).sum()) if split else 0
+            print(f"split={split}: {n_big} graphs on the device-wide path", flush=True)
+            db.set_tuning(block=256, groups_per_cu=4, use_graph=graph, split_vertices=split)
+            for rep in range(2):
+                lab, st = db.run(seed=9, stream=ca.STREAM_HASHED)
+                assert np.array_equal(lab, ref), (split, rep)
+                assert st["components"] == st0["components"], (split, rep)
+                assert st["tuning"]["split_vertices"] == split
+                lv = db.graph_stats()[0]
+                small = sizes <= split if split else np.ones(len(sizes), bool)
+                assert np.array_equal(lv[small], lv0[small]), (split, rep)
+    finally:
+        db.close()
+    for g in range(b.graphs):
+        gr = b.graph(g)
+        assert np.array_equal(ref[int(b.vtx_off[g]):int(b.vtx_off[g + 1])],
+                              oracle.canonical_labels(gr.offsets, gr.targets)), g
