@@ -625,21 +625,21 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       bins_valid = true;
     }
     // deviates -> keys
+    LearnedArgs la{nullptr};
+    if (!local && o->learned) {
+      // scores of this level's vertices delay their deviates
+      TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
+      level_degrees(n_l, off_l, g->ldeg, g->stream);
+      level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
+      mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
+      la.delay = g->delay;
+    }
     {
-      TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l + (gid_l ? 4 * n_l : 0));
+      TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l + (gid_l ? 4 * n_l : 0) + (la.delay ? n_l : 0));
       if (local) {
         init_keys_uniform(n_l, dev_gid_l, idx_l, o->seed, iteration, g->key, g->stream);
       } else {
         da.iteration = iteration;
-        LearnedArgs la{nullptr};
-        if (o->learned) {
-          // scores of this level's vertices delay their deviates
-          TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
-          level_degrees(n_l, off_l, g->ldeg, g->stream);
-          level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
-          mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
-          la.delay = g->delay;
-        }
         init_keys_exp(n_l, dev_gid_l, idx_l, da, la, g->key, g->stream);
       }
     }

@@ -86,32 +86,72 @@ __global__ void k_degrees(uint64_t n, const uint64_t *off, uint32_t *deg) {
     deg[v] = static_cast<uint32_t>(off[v + 1] - off[v]);
 }
 
+// Loads are issued in four dependent phases (sampled neighbour ids, their
+// degrees, the row starts of the pair heads, the first 8 entries of those
+// rows) with no branch between them, so every phase's loads are in flight
+// together; unused slots repeat sample 0 and are masked out afterwards.
 __global__ void k_features(uint64_t n, const uint64_t *off, const uint32_t *col, const uint32_t *deg, float round,
                            float *X) {
+  const uint64_t m2 = off[n];
   for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n;
        v += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t e0 = off[v];
     const uint32_t d = deg[v];
     const uint32_t S = d < 4 ? d : 4;
-    uint32_t u[4] = {0, 0, 0, 0};
-    float sum = 0.f, mx = 0.f;
-    for (uint32_t i = 0; i < S; ++i) {
-      u[i] = col[e0 + (static_cast<uint64_t>(i) * d) / S];
-      const float du = static_cast<float>(deg[u[i]]);
-      sum += du;
-      mx = du > mx ? du : mx;
+    float sum = 0.f, mx = 0.f, clus = 0.f;
+    const uint32_t pairs = S / 2;
+    if (d > 0) {
+      uint32_t u[4], du[4];
+      // sample i sits at floor(i d / S): i d / 4 when d >= 4, else i (S = d)
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i)
+        u[i] = col[e0 + (d >= 4 ? (static_cast<uint64_t>(i) * d) >> 2 : (i < d ? i : 0u))];
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) du[i] = deg[u[i]];
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i)
+        if (i < S) {
+          const float f = static_cast<float>(du[i]);
+          sum += f;
+          mx = f > mx ? f : mx;
+        }
+      // the first min(deg, 8) entries of a pair head's row lie inside the 12
+      // entries from the row start rounded down to a multiple of 4: three
+      // aligned 16-byte loads (entry-wise near the end of the array)
+      uint64_t r0[2];
+#pragma unroll
+      for (uint32_t p = 0; p < 2; ++p) r0[p] = off[u[2 * p]];
+      uint4 w[2][3];
+      bool vec[2];
+      const uint64_t last = m2 >= 12 ? (m2 - 12) & ~3ull : 0;  // a window start that stays inside col
+#pragma unroll
+      for (uint32_t p = 0; p < 2; ++p) {
+        const uint64_t a0 = r0[p] & ~3ull;
+        vec[p] = m2 >= 12 && a0 + 12 <= m2;
+        const uint4 *src = reinterpret_cast<const uint4 *>(col + (vec[p] ? a0 : last));
+        if (m2 >= 12) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) w[p][q] = src[q];
+        }
+      }
+#pragma unroll
+      for (uint32_t p = 0; p < 2; ++p) {
+        if (p >= pairs) continue;
+        const uint32_t b = u[2 * p + 1], da = du[2 * p] < 8 ? du[2 * p] : 8u;
+        bool hit = false;
+        if (vec[p]) {
+          const uint32_t sh = static_cast<uint32_t>(r0[p] & 3ull), lim = sh + da;
+          const uint32_t c[12] = {w[p][0].x, w[p][0].y, w[p][0].z, w[p][0].w, w[p][1].x, w[p][1].y,
+                                  w[p][1].z, w[p][1].w, w[p][2].x, w[p][2].y, w[p][2].z, w[p][2].w};
+#pragma unroll
+          for (uint32_t j = 0; j < 12; ++j) hit |= j >= sh && j < lim && c[j] == b;
+        } else {
+          for (uint32_t k = 0; k < da; ++k) hit |= col[r0[p] + k] == b;
+        }
+        clus += hit ? 1.f : 0.f;
+      }
     }
     const float mean = S ? sum / static_cast<float>(S) : 0.f;
-    float clus = 0.f;
-    const uint32_t pairs = S / 2;
-    for (uint32_t p = 0; p < pairs; ++p) {
-      const uint32_t a = u[2 * p], b = u[2 * p + 1];
-      const uint64_t r0 = off[a];
-      const uint32_t da = deg[a] < 8 ? deg[a] : 8;
-      bool hit = false;
-      for (uint32_t k = 0; k < da; ++k) hit |= col[r0 + k] == b;
-      clus += hit ? 1.f : 0.f;
-    }
     float *x = X + v * kMlpIn;
     x[0] = log2f(1.f + static_cast<float>(d));
     x[1] = log2f(1.f + mean);
@@ -138,70 +178,76 @@ __global__ void k_cluster_targets(uint64_t n, const uint32_t *lab, const uint32_
 }
 
 // ---------------------------------------------------------------------------
-// Inference: 4 waves per block, one 16-vertex tile per wave per step.
+// Inference: 4 waves per block, one 16-vertex tile per wave per step, every
+// layer computed transposed (weights as the A operand, vertices as columns).
+// For v_mfma_f32_16x16x16_bf16 lane (r, g) holds A[r][4g..4g+3],
+// B[4g..4g+3][r] and D[4g..4g+3][r]; so the transposed layer-1 output a lane
+// holds, H[vertex r][hidden 4g..4g+3 of a block], is exactly its B fragment
+// of that block for layer 2: the hidden layer never leaves the registers.
 __global__ __launch_bounds__(256) void k_mlp_infer(uint64_t n, const float *X, const MlpBf16 *wb, const MlpMaster *wm,
                                                   float *score) {
-  __shared__ uint16_t h1s[4][16 * kRowPad];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
-  v4s b1f[4], b2f[4][4];
-  float b1v[4], b2v[4], w3v[4];
+  v4s a1[4], a2[4][4];  // W1^T rows of hidden block nb; W2^T rows of block nb2 over hidden-1 block kb
+  float b1v[4][4], b2v[4][4], w3v[4][4];
   for (int nb = 0; nb < 4; ++nb) {
-    const int c = nb * 16 + r;
-    b1f[nb] = frag(wb->W1[(4 * g) * kMlpHid + c], wb->W1[(4 * g + 1) * kMlpHid + c], wb->W1[(4 * g + 2) * kMlpHid + c],
-                   wb->W1[(4 * g + 3) * kMlpHid + c]);
+    const int h = nb * 16 + r;
+    a1[nb] = frag(wb->W1[(4 * g) * kMlpHid + h], wb->W1[(4 * g + 1) * kMlpHid + h], wb->W1[(4 * g + 2) * kMlpHid + h],
+                  wb->W1[(4 * g + 3) * kMlpHid + h]);
     for (int kb = 0; kb < 4; ++kb) {
       const int k = kb * 16 + 4 * g;
-      b2f[kb][nb] = frag(wb->W2[k * kMlpHid + c], wb->W2[(k + 1) * kMlpHid + c], wb->W2[(k + 2) * kMlpHid + c],
-                         wb->W2[(k + 3) * kMlpHid + c]);
+      a2[kb][nb] = frag(wb->W2[k * kMlpHid + h], wb->W2[(k + 1) * kMlpHid + h], wb->W2[(k + 2) * kMlpHid + h],
+                        wb->W2[(k + 3) * kMlpHid + h]);
     }
-    b1v[nb] = wm->b1[c];
-    b2v[nb] = wm->b2[c];
-    w3v[nb] = bf16_to_f32(wb->w3[c]);
+    for (int j = 0; j < 4; ++j) {
+      const int c = nb * 16 + 4 * g + j;
+      b1v[nb][j] = wm->b1[c];
+      b2v[nb][j] = wm->b2[c];
+      w3v[nb][j] = bf16_to_f32(wb->w3[c]);
+    }
   }
   const float b3 = wm->b3;
-  uint16_t *h1 = h1s[wave];
+  const v4f zero = {0.f, 0.f, 0.f, 0.f};
   const uint64_t tiles = (n + 15) / 16;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * 4; base < tiles; base += static_cast<uint64_t>(gridDim.x) * 4) {
-    const uint64_t tile = base + wave;
-    uint16_t xa[4] = {0, 0, 0, 0};
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * 4;
+  // the features of a wave's next tile are loaded while it computes this one
+  auto load_x = [&](uint64_t tile, float (&xf)[4]) {
     const uint64_t row = tile * 16 + r;
-    if (row < n)
-      for (int j = 0; j < 4; ++j) {
-        const int k = 4 * g + j;
-        if (k < kMlpIn) xa[j] = f32_to_bf16(X[row * kMlpIn + k]);
-      }
-    const v4s a1 = frag(xa[0], xa[1], xa[2], xa[3]);
-    const v4f zero = {0.f, 0.f, 0.f, 0.f};
-    for (int nb = 0; nb < 4; ++nb) {
-      const v4f z = mfma(a1, b1f[nb], zero);
-      for (int j = 0; j < 4; ++j) {
-        const float h = z[j] + b1v[nb];
-        h1[(4 * g + j) * kRowPad + nb * 16 + r] = f32_to_bf16(h > 0.f ? h : 0.f);
-      }
-    }
-    __syncthreads();
-    v4f z2[4] = {zero, zero, zero, zero};
-    for (int kb = 0; kb < 4; ++kb) {
-      const uint16_t *src = h1 + r * kRowPad + kb * 16 + 4 * g;
-      const v4s a2 = frag(src[0], src[1], src[2], src[3]);
-      for (int nb = 0; nb < 4; ++nb) z2[nb] = mfma(a2, b2f[kb][nb], z2[nb]);
-    }
-    float part[4];
     for (int j = 0; j < 4; ++j) {
-      float acc = 0.f;
-      for (int nb = 0; nb < 4; ++nb) {
-        const float h = z2[nb][j] + b2v[nb];
-        acc += (h > 0.f ? h : 0.f) * w3v[nb];
-      }
-      part[j] = sum16(acc);
+      const int k = 4 * g + j;
+      xf[j] = row < n && k < kMlpIn ? X[row * kMlpIn + k] : 0.f;
     }
-    if (r == 0)
+  };
+  float xnext[4];
+  load_x(static_cast<uint64_t>(blockIdx.x) * 4 + wave, xnext);
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * 4; base < tiles; base += step) {
+    const uint64_t tile = base + wave;
+    const uint64_t row = tile * 16 + r;
+    const float xf[4] = {xnext[0], xnext[1], xnext[2], xnext[3]};
+    load_x(tile + step, xnext);
+    const v4s xb = frag(f32_to_bf16(xf[0]), f32_to_bf16(xf[1]), f32_to_bf16(xf[2]),
+                        f32_to_bf16(xf[3]));  // X^T: inputs 4g.., vertex r
+    v4s hb[4];                                         // H^T fragments, hidden block kb
+    for (int nb = 0; nb < 4; ++nb) {
+      const v4f z = mfma(a1[nb], xb, zero);
+      uint16_t hv[4];
       for (int j = 0; j < 4; ++j) {
-        const uint64_t v = tile * 16 + 4 * g + j;
-        if (v < n) score[v] = part[j] + b3;
+        const float h = z[j] + b1v[nb][j];
+        hv[j] = f32_to_bf16(h > 0.f ? h : 0.f);
       }
-    __syncthreads();
+      hb[nb] = frag(hv[0], hv[1], hv[2], hv[3]);
+    }
+    float acc = 0.f;
+    for (int nb = 0; nb < 4; ++nb) {
+      v4f z2 = zero;
+      for (int kb = 0; kb < 4; ++kb) z2 = mfma(a2[kb][nb], hb[kb], z2);
+      for (int j = 0; j < 4; ++j) {
+        const float h = z2[j] + b2v[nb][j];
+        acc += (h > 0.f ? h : 0.f) * w3v[nb][j];
+      }
+    }
+    acc = sum_groups(acc);  // over the 4 lane groups holding vertex r
+    if (g == 0 && row < n) score[row] = acc + b3;
   }
 }
 
