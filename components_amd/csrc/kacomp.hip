@@ -723,11 +723,15 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       TimedScope ts(g->timer, KC_K_COUNT, 12 * n_l);
       cluster_ids(n_l, rec.lab, g->crank, g->cid, g->stream);
     }
-    // relabel: every edge slot gets the cluster id of its far end (kNone inside
-    // a cluster); the slot buffer later takes the unique runs of the dedup
+    // relabel: every row packs the cluster ids of its far ends outside its
+    // cluster at the start of its slots; the slot buffer later takes the unique
+    // runs of the dedup.  Its bytes (offsets, neighbour ids and their ids, the
+    // row's id and count, the packed writes) are booked once P is known.
     uint32_t *relab = g->slots;
+    int relabel_slot;
     {
-      TimedScope ts(g->timer, KC_K_RELABEL, 8 * (n_l + 1) + 4 * m2_l + 4 * m2_l + 4 * n_l + 4 * m2_l + 4 * n_l);
+      TimedScope ts(g->timer, KC_K_RELABEL, 0);
+      relabel_slot = ts.slot;
       relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, g->stream);
     }
     if (b.nH) {
@@ -775,7 +779,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     uint32_t *col_next = g->col_pp[pp];
     uint32_t *bucket = col_next;
     {
-      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l + 4 * P);
+      g->timer.retag(relabel_slot, KC_K_RELABEL, 8 * (n_l + 1) + 8 * m2_l + 8 * n_l + 4 * P);
+      TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 8 * n_l + 8 * P);
       scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, bucket, g->stream);
     }
     const int dedup_slot = dedup_buckets(g, n_next, g->seg_off, bucket, g->slots, g->uniq, g->dl_medium, g->dl_huge,

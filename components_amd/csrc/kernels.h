@@ -117,11 +117,13 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
               const uint32_t *rank, uint32_t *pos_of, uint32_t *newidx, uint32_t *seg_of_cid, uint32_t *gid_next,
               uint64_t *seg_off, uint64_t *sizes, uint64_t *scratch, hipStream_t s);
 
-// relab[e] = lab[col[e]] when it differs from lab[v], else kNone;
-// deg_inter[v] (hub chunks: hub_inter) = number of such edges.
+// The labels lab[col[e]] of row v that differ from lab[v] (after a best-effort
+// LDS filter of repeats), packed at relab[off[v] ...] (hub chunks: at the
+// chunk start); deg_inter[v] (hub chunks: hub_inter) = their number.
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
              uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s);
-// Copies the non-kNone relab slots of every row to bucket[pos_of[v] + ...].
+// Copies the deg_inter[v] packed relab slots of every row (hub chunks:
+// hub_inter) to bucket[pos_of[v] + ...].
 void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,
              const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *pos_of, uint32_t *bucket,
              hipStream_t s);

@@ -368,9 +368,10 @@ __global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
   if (__ballot(pending) != 0 && lane_id() == 0) *a.flag = 1;
 }
 
-// Relabel pass: relab[e] = label of col[e] when it differs from the row's
-// label, kNone otherwise; writes are coalesced (the CSR layout).  The row's
-// count of cluster edges goes to deg_inter (hub chunks: hub_inter).
+// Relabel pass: the labels of col[e] that differ from the row's label are
+// written packed at the start of the row's slots (a hub chunk's at the start
+// of the chunk), in ballot order; the count goes to deg_inter (hub chunks:
+// hub_inter).  The slots after the count are left as they were.
 // Groups of 16 or 64 lanes also drop repeats of a label inside their row
 // through a small direct-mapped LDS filter (best effort: a label is dropped
 // only when the filter holds it, so every kept set still covers the row's
@@ -380,43 +381,52 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t
                                               uint32_t *relab, bool active, uint32_t v,
                                               uint64_t e0, uint64_t e1, int gl, bool hub, uint32_t chunk,
                                               uint32_t *filt, uint32_t fmask) {
-  uint32_t cnt = 0;
   if (W > 1) {
     for (uint32_t j = gl; j <= fmask; j += W) filt[j] = kNone;
     wave_fence();
   }
-  if (active) {
-    const uint32_t lv = lab[v];
-    auto one = [&](uint64_t e, uint32_t lw) {
-      bool inter = lw != lv;
+  const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
+  const uint64_t gmask = (W >= 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << gbase);
+  const uint32_t lv = active ? lab[v] : 0u;
+  // every lane of the wave takes the same number of W-edge steps (the most any
+  // group needs), so the ballots below see every lane of each group
+  uint64_t rounds = active ? ceil_div(e1 - e0, static_cast<uint64_t>(W)) : 0;
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t t = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(rounds), o, 64));
+    rounds = t > rounds ? t : rounds;
+  }
+  uint32_t base = 0;  // cluster edges of the row (chunk) written so far
+  for (uint64_t r = 0; r < rounds; r += kGather) {
+    uint32_t c[kGather], l[kGather];
+    bool ok[kGather];
+#pragma unroll
+    for (int j = 0; j < kGather; ++j) {
+      const uint64_t e = e0 + (r + j) * W + gl;
+      ok[j] = active && e < e1;
+      c[j] = ok[j] ? a.col[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kGather; ++j) l[j] = ok[j] ? lab[c[j]] : lv;
+#pragma unroll
+    for (int j = 0; j < kGather; ++j) {
+      bool inter = l[j] != lv;
       if (W > 1 && inter) {
-        const uint32_t slot = hash32(lw) & fmask;
-        if (filt[slot] == lw)
+        const uint32_t slot = hash32(l[j]) & fmask;
+        if (filt[slot] == l[j])
           inter = false;
         else
-          filt[slot] = lw;
+          filt[slot] = l[j];
       }
-      relab[e] = inter ? lw : kNone;
-      cnt += inter;
-    };
-    uint64_t e = e0 + gl;
-    for (; e + (kGather - 1) * W < e1; e += kGather * W) {
-      uint32_t c[kGather], l[kGather];
-#pragma unroll
-      for (int j = 0; j < kGather; ++j) c[j] = a.col[e + j * W];
-#pragma unroll
-      for (int j = 0; j < kGather; ++j) l[j] = lab[c[j]];
-#pragma unroll
-      for (int j = 0; j < kGather; ++j) one(e + j * W, l[j]);
+      const uint64_t m = __ballot(inter) & gmask;
+      if (inter) relab[e0 + base + __popcll(m & lanes_below())] = l[j];
+      base += static_cast<uint32_t>(__popcll(m));
     }
-    for (; e < e1; e += W) one(e, lab[a.col[e]]);
   }
-  if (W > 1) cnt = group_sum_u32(cnt, W);
   if (active && gl == 0) {
     if (hub)
-      a.hub_inter[chunk] = cnt;
+      a.hub_inter[chunk] = base;
     else
-      a.deg_inter[v] = cnt;
+      a.deg_inter[v] = base;
   }
 }
 
@@ -693,40 +703,16 @@ struct ScatterArgs {
 template <int W>
 __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active, uint32_t v, uint64_t e0,
                                               uint64_t e1, int gl, uint32_t d, uint32_t extra) {
+  (void)e1;
   // group base lane within the wave
   const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
   uint64_t pos = 0;
   active = active && d > 0;
   if (active && gl == 0) pos = static_cast<uint64_t>(a.pos_of[v]) + extra;
   if (W > 1) pos = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), gbase, 64));
-  if (W == 1) {
-    if (!active) return;
-    for (uint64_t e = e0; e < e1; ++e) {
-      const uint32_t lw = a.relab[e];
-      if (lw != kNone) a.bucket[pos++] = lw;
-    }
-    return;
-  }
-  // all lanes of the wave iterate the same number of rounds (max over groups)
-  uint64_t len = active ? e1 - e0 : 0;
-  uint64_t rounds = ceil_div(len, W);
-  for (int o = 32; o > 0; o >>= 1) {
-    uint64_t t = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(rounds), o, 64));
-    rounds = t > rounds ? t : rounds;
-  }
-  const uint64_t gmask = (W >= 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << gbase);
-  for (uint64_t r = 0; r < rounds; ++r) {
-    const uint64_t e = e0 + r * W + gl;
-    bool pred = false;
-    uint32_t lw = kNone;
-    if (active && e < e1) {
-      lw = a.relab[e];
-      pred = lw != kNone;
-    }
-    const uint64_t m = __ballot(pred) & gmask;
-    if (pred) a.bucket[pos + __popcll(m & lanes_below())] = lw;
-    pos += __popcll(m);
-  }
+  if (!active) return;
+  // the row's (chunk's) d cluster edges sit at its start: a straight copy
+  for (uint32_t i = gl; i < d; i += W) a.bucket[pos + i] = a.relab[e0 + i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(ScatterArgs a) {
