@@ -51,7 +51,8 @@ class Options(ctypes.Structure):
                 ("max_levels", ctypes.c_int32), ("sweep_group_small", ctypes.c_int32),
                 ("sweep_group_mid", ctypes.c_int32), ("sweep_group_large", ctypes.c_int32),
                 ("deviate_stream", ctypes.c_int32), ("learned_alpha", ctypes.c_float),
-                ("input_order", ctypes.c_int32), ("tail_sweeps", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("input_order", ctypes.c_int32), ("tail_sweeps", ctypes.c_int32),
+                ("round1_pull", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class Stats(ctypes.Structure):
@@ -278,7 +279,7 @@ STREAM_REFERENCE, STREAM_HASHED = 0, 1
 
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
             tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
-            alpha=1.0, input_order=False, tail_sweeps=0):
+            alpha=1.0, input_order=False, tail_sweeps=0, round1_pull=0):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -296,6 +297,7 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     o.learned_alpha = alpha
     o.input_order = 1 if input_order else 0
     o.tail_sweeps = int(tail_sweeps)
+    o.round1_pull = int(round1_pull)
     return o
 
 

@@ -119,7 +119,9 @@ typedef struct kc_options {
                              storage order (same results; for comparisons) */
   int32_t tail_sweeps;    /* -bfs: finish a level with pull sweeps once at most n >> k rows are pending:
                              0 = default k, k > 0 = that k, -1 = rounds to the end (same results) */
-  int32_t reserved[3];
+  int32_t round1_pull;    /* -bfs: round 1 of a level pulled by its unsettled rows: 0 = for levels of
+                             at least 2^16 vertices, 1 = always, -1 = never (same results) */
+  int32_t reserved[2];
 } kc_options;
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit

@@ -95,7 +95,9 @@ def test_sampled_graphs_6_7_vertices():
     check_labels(union(graphs), seed=1002, seq_limit=0, stream=ca.STREAM_HASHED)
 
 
-@pytest.mark.parametrize("bfs", [False, True])
+# schedule: pull sweeps; time-stepped rounds (round 1 pulled on levels of 2^16+ vertices, pull
+# sweeps at the tail); rounds with round 1 pulled on every level
+@pytest.mark.parametrize("bfs", [False, True, "pull1"])
 @pytest.mark.parametrize("stream", [ca.STREAM_REFERENCE, ca.STREAM_HASHED])
 @pytest.mark.parametrize("case", ["gnm", "grid2", "grid3", "rmat", "rgg", "path", "star"])
 def test_level_trace_matches_oracle(case, stream, bfs):
@@ -118,7 +120,7 @@ def test_level_trace_matches_oracle(case, stream, bfs):
     run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1005, seq_limit=seq, keep_keys=True, stream=stream)
     dg = ca.DeviceGraph(g)
     lab, st = dg.find_components(variant="exp", seed=1005, seq_limit=seq, record_trace=True, stream=stream,
-                                 use_bfs=bfs)
+                                 use_bfs=bool(bfs), round1_pull=1 if bfs == "pull1" else 0)
     tr = dg.trace()
     dg.close()
     want_levels = run.levels()
@@ -205,11 +207,12 @@ def test_rmat_scale22_schedules_agree():
     g = dg.download()
     ref = oracle.canonical_labels(g.offsets, g.targets)
     traces = []
-    # sweeps; push rounds finished by pull sweeps (default); push rounds only; early switch to sweeps
-    for bfs, tail in ((False, 0), (True, 0), (True, -1), (True, 1)):
+    # sweeps; rounds with round 1 pulled and pull sweeps at the tail (default); push rounds only;
+    # early switch to sweeps; round 1 pulled on every level
+    for bfs, tail, r1 in ((False, 0, 0), (True, 0, 0), (True, -1, -1), (True, 1, 0), (True, 0, 1)):
         lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
-                                     use_bfs=bfs, tail_sweeps=tail, record_trace=True)
-        assert np.array_equal(lab, ref), (bfs, tail)
+                                     use_bfs=bfs, tail_sweeps=tail, round1_pull=r1, record_trace=True)
+        assert np.array_equal(lab, ref), (bfs, tail, r1)
         traces.append(dg.trace())
     for other in traces[1:]:
         assert len(traces[0]) == len(other)
