@@ -404,8 +404,11 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
 // in wave tables, medium ones in block tables, huge ones in global tables
 // processed in fixed-size chunks planned on the host.  Returns the timer slot
 // of the first dedup launch (its algorithmic bytes are booked by the caller).
+// `scratch` serves the scans (the handle's own scan scratch does not exist yet
+// while a generated graph's CSR is built).
 int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
-                   uint32_t *uniq, uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space) {
+                  uint32_t *uniq, uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space,
+                  uint64_t *scratch, bool tables = false) {
   hipStream_t s = g->stream;
   fill_u64(g->sizes + 3, 3, 0, s);
   DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
@@ -430,14 +433,14 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   // Bitmaps over the label space when they fit the cap: one pass, in place,
   // with the chunks planned on the device.
   const uint64_t words = (label_space + 31) / 32;
-  if (n_huge * words * 4 <= kHugeBitmapBytes) {
+  if (!tables && n_huge * words * 4 <= kHugeBitmapBytes) {
     if (n_huge * words > g->huge_bits_cap) {
       g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
       g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
     }
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
     dedup_huge_bits(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, g->d_huge_info,
-                    g->huge_bits, words, bucket, out, uniq, g->scan_scratch, s);
+                    g->huge_bits, words, bucket, out, uniq, scratch, s);
     return first;
   }
   huge_sizes(static_cast<uint32_t>(n_huge), huge, seg_off, g->d_huge_info, s);
@@ -807,7 +810,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, bucket, g->stream);
     }
     const int dedup_slot = dedup_buckets(g, n_next, g->seg_off, bucket, g->slots, g->uniq, g->dl_medium, g->dl_huge,
-                                         g->dl_huge_tab, clusters);
+                                         g->dl_huge_tab, clusters, g->scan_scratch, o->dedup_tables != 0);
     // next-level CSR
     {
       TimedScope ts(g->timer, KC_K_SCAN, 4 * n_next + 8 * (n_next + 1));
@@ -1045,7 +1048,7 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   uint64_t *huge_tab = reinterpret_cast<uint64_t *>(huge + n + 2) + 1;
   DevTemp runs(P * 4 + 64);
   dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), runs.as<uint32_t>(), uniq.as<uint32_t>(), medium,
-                huge, huge_tab, n);
+                huge, huge_tab, n, scratch.as<uint64_t>());
   g->off0 = g->dalloc<uint64_t>(n + 1);
   csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));

@@ -121,7 +121,9 @@ typedef struct kc_options {
                              0 = default k, k > 0 = that k, -1 = rounds to the end (same results) */
   int32_t round1_pull;    /* -bfs: round 1 of a level pulled by its unsettled rows: 0 = for levels of
                              at least 2^16 vertices, 1 = always, -1 = never (same results) */
-  int32_t reserved[2];
+  int32_t dedup_tables;   /* 1: huge cluster-edge buckets deduplicated in global hash tables (the path
+                             used when their bitmaps would exceed 2 GB) instead of bitmaps (same results) */
+  int32_t reserved[1];
 } kc_options;
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit

@@ -208,11 +208,13 @@ def test_rmat_scale22_schedules_agree():
     ref = oracle.canonical_labels(g.offsets, g.targets)
     traces = []
     # sweeps; rounds with round 1 pulled and pull sweeps at the tail (default); push rounds only;
-    # early switch to sweeps; round 1 pulled on every level
-    for bfs, tail, r1 in ((False, 0, 0), (True, 0, 0), (True, -1, -1), (True, 1, 0), (True, 0, 1)):
+    # early switch to sweeps; round 1 pulled on every level; huge buckets in hash tables
+    for bfs, tail, r1, tab in ((False, 0, 0, False), (True, 0, 0, False), (True, -1, -1, False),
+                               (True, 1, 0, False), (True, 0, 1, False), (True, 0, 0, True)):
         lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
-                                     use_bfs=bfs, tail_sweeps=tail, round1_pull=r1, record_trace=True)
-        assert np.array_equal(lab, ref), (bfs, tail, r1)
+                                     use_bfs=bfs, tail_sweeps=tail, round1_pull=r1, dedup_tables=tab,
+                                     record_trace=True)
+        assert np.array_equal(lab, ref), (bfs, tail, r1, tab)
         traces.append(dg.trace())
     for other in traces[1:]:
         assert len(traces[0]) == len(other)
