@@ -758,7 +758,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     {
       TimedScope ts(g->timer, KC_K_RELABEL, 0);
       relabel_slot = ts.slot;
-      relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, g->stream);
+      // the hot-label launch pays off on the degree-ordered level 0
+      relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l ? n_l : 0, g->stream);
     }
     if (b.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);

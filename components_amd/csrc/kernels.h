@@ -120,8 +120,10 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
 // The labels lab[col[e]] of row v that differ from lab[v] (after a best-effort
 // LDS filter of repeats), packed at relab[off[v] ...] (hub chunks: at the
 // chunk start); deg_inter[v] (hub chunks: hub_inter) = their number.
+// One persistent launch; lab[0 .. min(n_hot, 16384)) is served from LDS (for
+// degree-ordered levels, whose first vertices take most endpoints).
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
-             uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s);
+             uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s);
 // Copies the deg_inter[v] packed relab slots of every row (hub chunks:
 // hub_inter) to bucket[pos_of[v] + ...].
 void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,

@@ -376,8 +376,18 @@ __global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
 // through a small direct-mapped LDS filter (best effort: a label is dropped
 // only when the filter holds it, so every kept set still covers the row's
 // distinct labels; the bucket dedup removes the rest).
-template <int W>
-__device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t *__restrict__ lab,
+// Label lookups: the first n_hot entries (the highest-degree vertices of a
+// degree-ordered level, which receive most edge endpoints) from an LDS copy,
+// the others from the array.
+struct LabHot {
+  const uint32_t *__restrict__ lab;
+  const uint32_t *hot;
+  uint32_t n_hot;
+  __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return c < n_hot ? hot[c] : lab[c]; }
+};
+
+template <int W, class Lab>
+__device__ __forceinline__ void relabel_group(const SweepArgs &a, const Lab &lab,
                                               uint32_t *relab, bool active, uint32_t v,
                                               uint64_t e0, uint64_t e1, int gl, bool hub, uint32_t chunk,
                                               uint32_t *filt, uint32_t fmask) {
@@ -387,7 +397,7 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t
   }
   const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
   const uint64_t gmask = (W >= 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << gbase);
-  const uint32_t lv = active ? lab[v] : 0u;
+  const uint32_t lv = active ? lab(v) : 0u;
   // every lane of the wave takes the same number of W-edge steps (the most any
   // group needs), so the ballots below see every lane of each group
   uint64_t rounds = active ? ceil_div(e1 - e0, static_cast<uint64_t>(W)) : 0;
@@ -406,7 +416,7 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t
       c[j] = ok[j] ? a.col[e] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < kGather; ++j) l[j] = ok[j] ? lab[c[j]] : lv;
+    for (int j = 0; j < kGather; ++j) l[j] = ok[j] ? lab(c[j]) : lv;
 #pragma unroll
     for (int j = 0; j < kGather; ++j) {
       bool inter = l[j] != lv;
@@ -432,10 +442,10 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const uint32_t
 
 constexpr int kFilterWords = 4096;  // 16 KB of LDS per block
 
-__global__ __launch_bounds__(kBlock) void k_relabel(SweepArgs a, const uint32_t *lab, uint32_t *relab) {
-  __shared__ uint32_t filt[kFilterWords];
-  const uint32_t blk = blockIdx.x;
-  const int tid = threadIdx.x;
+// One 256-thread block's share of the relabel pass: section by block index.
+template <class Lab>
+__device__ __forceinline__ void relabel_block(const SweepArgs &a, const Lab &lab, uint32_t *relab, uint32_t blk,
+                                              int tid, uint32_t *filt) {
   if (blk < a.b.bS) {
     const uint32_t i = blk * kBlock + tid;
     const bool act = i < a.b.nT;
@@ -460,6 +470,27 @@ __global__ __launch_bounds__(kBlock) void k_relabel(SweepArgs a, const uint32_t 
     HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
     relabel_group<kWave>(a, lab, relab, act, h.v, h.e0, h.e1, tid & 63, true, i, filt + (tid / kWave) * 1024, 1023);
   }
+}
+
+// Persistent launch: each 1024-thread block first copies the labels of the
+// first n_hot vertices into LDS (the highest-degree ones on a degree-ordered
+// level; n_hot = 0 elsewhere), then its four 256-thread quarters walk the
+// virtual 256-thread blocks of the binned layout in a grid stride, each
+// quarter with its own LDS filter (the quarters are whole waves and need no
+// barrier between them).
+constexpr int kRelabelHot = 16384;
+constexpr int kRelabelQuarters = 4;
+
+__global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(SweepArgs a, const uint32_t *lab,
+                                                                        uint32_t *relab, uint32_t n_hot) {
+  __shared__ uint32_t hot[kRelabelHot];
+  __shared__ uint32_t filt[kRelabelQuarters][kFilterWords];
+  for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) hot[i] = lab[i];
+  __syncthreads();
+  const int q = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
+  const LabHot lk{lab, hot, n_hot};
+  for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.blocks; blk += gridDim.x * kRelabelQuarters)
+    relabel_block(a, lk, relab, blk, tid, filt[q]);
 }
 
 __global__ void k_hub_fold(HubChunk *hubs, uint32_t nH, const uint32_t *hub_inter, uint32_t *deg_inter,
@@ -1350,10 +1381,16 @@ void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, ui
 }
 
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
-             uint32_t *deg_inter, uint32_t *hub_inter, hipStream_t s) {
+             uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s) {
   if (b.blocks == 0) return;
   SweepArgs a{off, col, nullptr, nullptr, 0, b, deg_inter, hub_inter, nullptr, nullptr};
-  hipLaunchKernelGGL(k_relabel, dim3(b.blocks), dim3(kBlock), 0, s, a, lab, relab);
+  int dev = 0, cus = 0;
+  KC_HIP(hipGetDevice(&dev));
+  KC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint32_t blocks = static_cast<uint32_t>(ceil_div(b.blocks, kRelabelQuarters));
+  const uint32_t grid = std::min<uint32_t>(blocks, static_cast<uint32_t>(cus > 0 ? cus : 1));
+  hipLaunchKernelGGL(k_relabel_hot, dim3(grid), dim3(kBlock * kRelabelQuarters), 0, s, a, lab, relab,
+                     static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot)));
   KC_LAUNCH_CHECK();
 }
 
