@@ -45,8 +45,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other plain workloads")
-    ap.add_argument("--round1-pull", type=int, default=0,
-                    help="rounds schedule: round 1 pulled by unsettled rows (0 default, 1 always, -1 never)")
+    ap.add_argument("--pull-rounds", type=int, default=0,
+                    help="rounds schedule: push or pull chosen per early round (0 default, 1 every level, -1 never)")
     ap.add_argument("--tail-sweeps", type=int, default=0,
                     help="rounds schedule: pull sweeps once <= n>>k rows are pending (0 default k, -1 never)")
     ap.add_argument("--untimed-kernels", action="store_true",
@@ -118,7 +118,7 @@ def main():
     seq_limit = max(m // 256, 4096)  # ComputeSequentialLimit with one PE per compute unit
     run_kw = dict(variant="exp", beta=0.1, seq_limit=seq_limit, stream=stream, tmax=a.tmax, smax=a.smax, wmax=a.wmax,
                   use_bfs=a.schedule == "rounds", tail_sweeps=a.tail_sweeps,
-                  round1_pull=a.round1_pull)
+                  pull_rounds=a.pull_rounds)
     print(f"[bench] rmat scale {a.scale}: n={n} m={m} non_isolated={dg.non_isolated} built in {gen_s:.1f}s",
           file=sys.stderr, flush=True)
 

@@ -52,7 +52,7 @@ class Options(ctypes.Structure):
                 ("sweep_group_mid", ctypes.c_int32), ("sweep_group_large", ctypes.c_int32),
                 ("deviate_stream", ctypes.c_int32), ("learned_alpha", ctypes.c_float),
                 ("input_order", ctypes.c_int32), ("tail_sweeps", ctypes.c_int32),
-                ("round1_pull", ctypes.c_int32),
+                ("pull_rounds", ctypes.c_int32),
                 ("dedup_tables", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
 
@@ -280,7 +280,7 @@ STREAM_REFERENCE, STREAM_HASHED = 0, 1
 
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
             tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
-            alpha=1.0, input_order=False, tail_sweeps=0, round1_pull=0, dedup_tables=False):
+            alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_tables=False):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -298,7 +298,7 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     o.learned_alpha = alpha
     o.input_order = 1 if input_order else 0
     o.tail_sweeps = int(tail_sweeps)
-    o.round1_pull = int(round1_pull)
+    o.pull_rounds = int(pull_rounds)
     o.dedup_tables = 1 if dedup_tables else 0
     return o
 

@@ -289,7 +289,10 @@ namespace {
 constexpr uint32_t kMaxFlags = 1u << 16;
 constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep the input order
 constexpr int kTailShift = 5;  // -bfs: pull sweeps finish a level once at most n/32 rows are pending
-constexpr uint64_t kRound1PullMin = 1u << 16;  // -bfs: levels at least this large pull round 1
+constexpr uint64_t kPullRoundsMin = 1u << 16;  // -bfs: levels at least this large choose round directions
+constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulled
+constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average degree or more (sparse,
+                                                // high-diameter levels keep batched pushes)
 constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
 constexpr uint32_t kMinWaveBin = 256;
 
@@ -496,38 +499,52 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
 // work (the reported "sweeps" of this schedule).
 uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                             uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base,
-                            int tail_shift, bool pull_round1) {
+                            int tail_shift, bool choose_direction) {
   uint64_t done = 0;
   uint32_t t = 0;
+  bool deciding = choose_direction;
   Bins b = level_bins;
   uint32_t next_rebin = 4;
   int first_slot = -1;
   while (true) {
-    if (pull_round1 && t == 1) {
-      // Round 1 pulled by the unsettled rows instead of pushed by the rows at
-      // distance 1 (most of a large level after round 0): every vertex at
-      // distance >= 2 takes the best neighbour key + 1 in place.  The
-      // distance-1 keys are final after round 0, so every vertex whose final
-      // distance is 2 ends with its final key -- what round 1 guarantees --
-      // and the others keep upper bounds that later rounds lower.
-      b = make_bins(g, n, off, cfg, key, 2);
-      if (flag_base + 3 >= kMaxFlags) {
-        fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
-        flag_base = 0;
+    if (deciding && t >= 1) {
+      // Direction of round t (as in direction-optimising BFS): pushed by the
+      // rows at distance t, or pulled by the rows above it when those hold
+      // fewer edge slots (as in round 1 of R-MAT, where most rows sit at
+      // distance 1 after round 0).  A pull lets every vertex above t take the
+      // best neighbour key + 1 in place: the keys up to distance t are final,
+      // so every vertex whose final distance is t + 1 ends with its final key
+      // -- what round t guarantees -- and the others keep upper bounds that
+      // later rounds lower.
+      round_volumes(n, off, key, t, g->sizes + 12, g->stream);
+      d2h(g, g->h_sizes + 12, g->sizes + 12, 2 * sizeof(uint64_t));
+      sync(g);
+      if (g->h_sizes[13] < g->h_sizes[12]) {
+        b = make_bins(g, n, off, cfg, key, t + 1);
+        if (flag_base + 3 >= kMaxFlags) {
+          fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
+          flag_base = 0;
+        }
+        {
+          TimedScope ts(g->timer, KC_K_SWEEP, 0);
+          sweep(kSweepInPlace, false, off, col, key, key, kOneHop, b, g->deg_inter, g->hub_inter,
+                &g->flags[flag_base], nullptr, g->stream);
+        }
+        ++flag_base;
+        ++done;
+        ++t;
+        deciding = false;  // the rows left above t are few; the rounds go on as pushes
+        while (next_rebin <= t) next_rebin *= 2;
+      } else if (t >= kDirectionRounds) {
+        deciding = false;
       }
-      {
-        TimedScope ts(g->timer, KC_K_SWEEP, 0);
-        sweep(kSweepInPlace, false, off, col, key, key, kOneHop, b, g->deg_inter, g->hub_inter, &g->flags[flag_base],
-              nullptr, g->stream);
-      }
-      ++flag_base;
-      ++done;
-      t = 2;
     }
     // the first rounds carry most of the pushes; later ones (over the
     // re-binned unsettled vertices) are cheap, so they go in longer
     // speculative batches with one synchronisation each
-    const uint32_t batch = (pull_round1 && t == 0) ? 1u : t < 8 ? 4u : 16u;
+    // (one round at a time while the direction is chosen: the choice reads
+    // the previous round's result)
+    const uint32_t batch = deciding ? 1u : t < 8 ? 4u : 16u;
     if (t >= next_rebin) {
       // drop the settled vertices from the rounds' bins (distances below t)
       b = make_bins(g, n, off, cfg, key, t);
@@ -684,7 +701,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     } else if (o->use_bfs) {
       sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base,
                                   o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps,
-                                  o->round1_pull > 0 || (o->round1_pull == 0 && n_l >= kRound1PullMin));
+                                  o->pull_rounds > 0 || (o->pull_rounds == 0 && n_l >= kPullRoundsMin &&
+                                                         m2_l >= kDirectionMinDegree * n_l));
     } else {
       sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
     }

@@ -72,6 +72,8 @@ void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, 
 // receive nT, nS, nW, nH.  deg_inter of degree-0 vertices is cleared.
 // scratch: bin_scratch_words(n) words.
 // With key != null only vertices at key distance >= tmin are binned.
+// out[0] = edge slots of the rows at distance t, out[1] = of the rows above t.
+void round_volumes(uint64_t n, const uint64_t *off, const uint64_t *key, uint32_t t, uint64_t *out, hipStream_t s);
 void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS,
                   uint32_t *listW, HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch,
                   hipStream_t s, const uint64_t *key = nullptr, uint32_t tmin = 0);

@@ -1233,6 +1233,45 @@ void deviates_only(uint32_t gid0, uint32_t count, DeviateArgs d, uint32_t *out, 
   KC_LAUNCH_CHECK();
 }
 
+// Edge slots of the rows at distance t (the frontier of round t) and of the
+// rows above it: out[0] and out[1] (zeroed by the caller); one atomic pair
+// per block.
+__global__ __launch_bounds__(kBlock) void k_round_volumes(uint64_t n, const uint64_t *off, const uint64_t *key,
+                                                         uint32_t t, unsigned long long *out) {
+  __shared__ unsigned long long part[2][kBlock / kWave];
+  uint64_t at = 0, above = 0;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint32_t d = static_cast<uint32_t>(key[v] >> 32);
+    const uint64_t deg = off[v + 1] - off[v];
+    at += d == t ? deg : 0;
+    above += d > t ? deg : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    at += static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(at), o));
+    above += static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(above), o));
+  }
+  if (lane_id() == 0) {
+    part[0][threadIdx.x / kWave] = at;
+    part[1][threadIdx.x / kWave] = above;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned long long s = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) s += part[threadIdx.x][w];
+    if (s) atomicAdd(&out[threadIdx.x], s);
+  }
+}
+
+void round_volumes(uint64_t n, const uint64_t *off, const uint64_t *key, uint32_t t, uint64_t *out, hipStream_t s) {
+  fill_u64(out, 2, 0, s);
+  if (!n) return;
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock), 2048));
+  hipLaunchKernelGGL(k_round_volumes, dim3(g), dim3(kBlock), 0, s, n, off, key, t,
+                     reinterpret_cast<unsigned long long *>(out));
+  KC_LAUNCH_CHECK();
+}
+
 void bin_vertices(uint64_t n, const uint64_t *off, BinCfg cfg, uint32_t *listT, uint32_t *listS, uint32_t *listW,
                   HubChunk *hubs, uint32_t *counters, uint32_t *deg_inter, uint64_t *scratch, hipStream_t s,
                   const uint64_t *key, uint32_t tmin) {

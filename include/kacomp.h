@@ -119,8 +119,10 @@ typedef struct kc_options {
                              storage order (same results; for comparisons) */
   int32_t tail_sweeps;    /* -bfs: finish a level with pull sweeps once at most n >> k rows are pending:
                              0 = default k, k > 0 = that k, -1 = rounds to the end (same results) */
-  int32_t round1_pull;    /* -bfs: round 1 of a level pulled by its unsettled rows: 0 = for levels of
-                             at least 2^16 vertices, 1 = always, -1 = never (same results) */
+  int32_t pull_rounds;    /* -bfs: rounds 1..6 of a level each pushed by the rows at the round's
+                             distance or pulled by the rows above it, whichever hold fewer edge slots:
+                             0 = on levels of >= 2^16 vertices and average degree >= 8, 1 = on every
+                             level, -1 = always pushed (same results) */
   int32_t dedup_tables;   /* 1: huge cluster-edge buckets deduplicated in global hash tables (the path
                              used when their bitmaps would exceed 2 GB) instead of bitmaps (same results) */
   int32_t reserved[1];

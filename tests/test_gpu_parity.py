@@ -95,8 +95,8 @@ def test_sampled_graphs_6_7_vertices():
     check_labels(union(graphs), seed=1002, seq_limit=0, stream=ca.STREAM_HASHED)
 
 
-# schedule: pull sweeps; time-stepped rounds (round 1 pulled on levels of 2^16+ vertices, pull
-# sweeps at the tail); rounds with round 1 pulled on every level
+# schedule: pull sweeps; time-stepped rounds (push or pull chosen per early round on large dense
+# levels, pull sweeps at the tail); rounds with the direction chosen on every level
 @pytest.mark.parametrize("bfs", [False, True, "pull1"])
 @pytest.mark.parametrize("stream", [ca.STREAM_REFERENCE, ca.STREAM_HASHED])
 @pytest.mark.parametrize("case", ["gnm", "grid2", "grid3", "rmat", "rgg", "path", "star"])
@@ -120,7 +120,7 @@ def test_level_trace_matches_oracle(case, stream, bfs):
     run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1005, seq_limit=seq, keep_keys=True, stream=stream)
     dg = ca.DeviceGraph(g)
     lab, st = dg.find_components(variant="exp", seed=1005, seq_limit=seq, record_trace=True, stream=stream,
-                                 use_bfs=bool(bfs), round1_pull=1 if bfs == "pull1" else 0)
+                                 use_bfs=bool(bfs), pull_rounds=1 if bfs == "pull1" else 0)
     tr = dg.trace()
     dg.close()
     want_levels = run.levels()
@@ -207,12 +207,12 @@ def test_rmat_scale22_schedules_agree():
     g = dg.download()
     ref = oracle.canonical_labels(g.offsets, g.targets)
     traces = []
-    # sweeps; rounds with round 1 pulled and pull sweeps at the tail (default); push rounds only;
-    # early switch to sweeps; round 1 pulled on every level; huge buckets in hash tables
+    # sweeps; rounds with directions chosen and pull sweeps at the tail (default); push rounds only;
+    # early switch to sweeps; directions chosen on every level; huge buckets in hash tables
     for bfs, tail, r1, tab in ((False, 0, 0, False), (True, 0, 0, False), (True, -1, -1, False),
                                (True, 1, 0, False), (True, 0, 1, False), (True, 0, 0, True)):
         lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
-                                     use_bfs=bfs, tail_sweeps=tail, round1_pull=r1, dedup_tables=tab,
+                                     use_bfs=bfs, tail_sweeps=tail, pull_rounds=r1, dedup_tables=tab,
                                      record_trace=True)
         assert np.array_equal(lab, ref), (bfs, tail, r1, tab)
         traces.append(dg.trace())
