@@ -516,9 +516,8 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       // so every vertex whose final distance is t + 1 ends with its final key
       // -- what round t guarantees -- and the others keep upper bounds that
       // later rounds lower.
-      round_volumes(n, off, key, t, g->sizes + 12, g->stream);
-      d2h(g, g->h_sizes + 12, g->sizes + 12, 2 * sizeof(uint64_t));
-      sync(g);
+      // (the edge slots at and above t were counted after round t - 1 and
+      // came back with its flag)
       if (g->h_sizes[13] < g->h_sizes[12]) {
         b = make_bins(g, n, off, cfg, key, t + 1);
         if (flag_base + 3 >= kMaxFlags) {
@@ -575,6 +574,10 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       if (first_slot < 0) first_slot = slots[i];
     }
     d2h(g, g->h_flags, &g->flags[flag_base], batch * sizeof(int));
+    if (deciding) {  // edge slots at and above the next round's distance, for its direction
+      round_volumes(n, off, key, t + batch, g->sizes + 12, g->stream);
+      d2h(g, g->h_sizes + 12, g->sizes + 12, 2 * sizeof(uint64_t));
+    }
     sync(g);
     uint32_t i = 0;
     while (i < batch && g->h_flags[i] != 0) ++i;
