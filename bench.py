@@ -49,6 +49,8 @@ def parse():
                     help="rounds schedule: push or pull chosen per early round (0 default, 1 every level, -1 never)")
     ap.add_argument("--tail-sweeps", type=int, default=0,
                     help="rounds schedule: pull sweeps once <= n>>k rows are pending (0 default k, -1 never)")
+    ap.add_argument("--dedup-huge", default="auto", choices=["auto", "tables", "bitmaps", "bytemaps"],
+                    help="dedup of the huge cluster-edge buckets (auto: byte maps, bitmaps, tables by size)")
     ap.add_argument("--untimed-kernels", action="store_true",
                     help="time the steps without per-launch HIP events (no kernel breakdown or roofline)")
     ap.add_argument("--learned-rhg-log-n", type=int, default=24)
@@ -118,7 +120,7 @@ def main():
     seq_limit = max(m // 256, 4096)  # ComputeSequentialLimit with one PE per compute unit
     run_kw = dict(variant="exp", beta=0.1, seq_limit=seq_limit, stream=stream, tmax=a.tmax, smax=a.smax, wmax=a.wmax,
                   use_bfs=a.schedule == "rounds", tail_sweeps=a.tail_sweeps,
-                  pull_rounds=a.pull_rounds)
+                  pull_rounds=a.pull_rounds, dedup_huge=a.dedup_huge)
     print(f"[bench] rmat scale {a.scale}: n={n} m={m} non_isolated={dg.non_isolated} built in {gen_s:.1f}s",
           file=sys.stderr, flush=True)
 

@@ -230,6 +230,8 @@ struct kc_graph {
   uint64_t huge_cap = 0;
   uint32_t *huge_bits = nullptr;  // bitmaps of the huge buckets over the label space
   uint64_t huge_bits_cap = 0;
+  uint8_t *huge_maps = nullptr;  // byte maps of the huge buckets over the label space
+  uint64_t huge_maps_cap = 0;
   HugeChunk *d_chunks = nullptr;
   uint64_t chunks_cap = 0;
   uint64_t *d_huge_info = nullptr;
@@ -294,6 +296,8 @@ constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulle
 constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average degree or more (sparse,
                                                 // high-diameter levels keep batched pushes)
 constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
+constexpr uint64_t kHugeMapBytes = 4ull << 30;     // byte maps of the largest huge buckets
+constexpr uint64_t kMapMinShare = 128;             // byte maps from C / 128 elements on
 constexpr uint32_t kMinWaveBin = 256;
 
 
@@ -404,14 +408,16 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
 }
 
 // Dedup of buckets [seg_off[a2], seg_off[a2+1]) for a2 < nb: small buckets
-// in wave tables, medium ones in block tables, huge ones in global tables
-// processed in fixed-size chunks planned on the host.  Returns the timer slot
-// of the first dedup launch (its algorithmic bytes are booked by the caller).
-// `scratch` serves the scans (the handle's own scan scratch does not exist yet
-// while a generated graph's CSR is built).
+// in wave tables, medium ones in block tables, huge ones against byte maps or
+// bitmaps over the label space (mode KC_DEDUP_AUTO: byte maps while they fit
+// their cap, then bitmaps) or in global tables processed in fixed-size chunks
+// planned on the host.  Returns the timer slot of the first dedup launch (its
+// algorithmic bytes are booked by the caller).  `scratch` serves the scans
+// (the handle's own scan scratch does not exist yet while a generated graph's
+// CSR is built).
 int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
                   uint32_t *uniq, uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space,
-                  uint64_t *scratch, bool tables = false) {
+                  uint64_t *scratch, int mode = KC_DEDUP_AUTO) {
   hipStream_t s = g->stream;
   fill_u64(g->sizes + 3, 3, 0, s);
   DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
@@ -429,21 +435,45 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
     dedup_medium(static_cast<uint32_t>(n_medium), medium, seg_off, bucket, out, uniq, s);
   }
   if (!n_huge) return first;
-  if (2 * n_huge > g->huge_info_cap) {
-    g->huge_info_cap = 2 * n_huge + 64;
+  if (3 * n_huge + 6 > g->huge_info_cap) {
+    g->huge_info_cap = 4 * n_huge + 64;
     g->d_huge_info = g->dalloc<uint64_t>(g->huge_info_cap);
   }
-  // Bitmaps over the label space when they fit the cap: one pass, in place,
-  // with the chunks planned on the device.
+  // Bitmaps over the label space when they fit their cap, the buckets of at
+  // least map_min elements against byte maps instead: those cost two passes
+  // over C bytes (the emit reads the map and clears it) but no atomics, the
+  // bitmaps one device atomic per distinct value (about 0.12 ns each against
+  // 0.4 ps per map byte), so the maps pay from about C/128 distinct values
+  // on.  Their number is at most huge entries / map_min <= huge_slots /
+  // (2 map_min) (every table holds at least twice its bucket), which bounds
+  // the maps' memory before any size is read back; map_min rises until that
+  // bound fits the maps' cap.
   const uint64_t words = (label_space + 31) / 32;
-  if (!tables && n_huge * words * 4 <= kHugeBitmapBytes) {
-    if (n_huge * words > g->huge_bits_cap) {
+  if (mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_BITMAPS ||
+      (mode == KC_DEDUP_AUTO && n_huge * words * 4 <= kHugeBitmapBytes)) {
+    const uint64_t stride = huge_map_stride(label_space);
+    uint64_t map_min = ~0ull, n_maps = 0;
+    if (mode != KC_DEDUP_BITMAPS && label_space) {
+      map_min = mode == KC_DEDUP_BYTEMAPS ? kDedupBlockCap + 1
+                                          : std::max<uint64_t>(kDedupBlockCap + 1, label_space / kMapMinShare);
+      const uint64_t cap_maps = kHugeMapBytes / stride;
+      while (map_min != ~0ull && std::min(n_huge, huge_slots / (2 * map_min)) > cap_maps)
+        map_min = cap_maps ? huge_slots / (2 * cap_maps) + 1 : ~0ull;
+      if (map_min != ~0ull) n_maps = std::min(n_huge, huge_slots / (2 * map_min));
+      if (!n_maps) map_min = ~0ull;
+    }
+    if (map_min > kDedupBlockCap + 1 && n_huge * words > g->huge_bits_cap) {
       g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
       g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
     }
+    if (n_maps * stride > g->huge_maps_cap) {  // maps start zero and every call leaves them zero
+      g->huge_maps_cap = std::min(kHugeMapBytes, n_maps * stride + (n_maps * stride >> 2));
+      g->huge_maps = g->dalloc<uint8_t>(g->huge_maps_cap);
+      fill_u32(reinterpret_cast<uint32_t *>(g->huge_maps), g->huge_maps_cap / 4, 0u, s);
+    }
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
-    dedup_huge_bits(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, g->d_huge_info,
-                    g->huge_bits, words, bucket, out, uniq, scratch, s);
+    dedup_huge_mixed(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, map_min, n_maps,
+                     g->d_huge_info, g->huge_bits, words, g->huge_maps, label_space, bucket, out, uniq, scratch, s);
     return first;
   }
   huge_sizes(static_cast<uint32_t>(n_huge), huge, seg_off, g->d_huge_info, s);
@@ -832,7 +862,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, bucket, g->stream);
     }
     const int dedup_slot = dedup_buckets(g, n_next, g->seg_off, bucket, g->slots, g->uniq, g->dl_medium, g->dl_huge,
-                                         g->dl_huge_tab, clusters, g->scan_scratch, o->dedup_tables != 0);
+                                         g->dl_huge_tab, clusters, g->scan_scratch, o->dedup_huge);
     // next-level CSR
     {
       TimedScope ts(g->timer, KC_K_SCAN, 4 * n_next + 8 * (n_next + 1));
@@ -1247,6 +1277,8 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
     else
       kc_options_init(&o);
     if (o.variant < KC_VARIANT_EXP || o.variant > KC_VARIANT_LABELS) fail(KC_ERR_INVALID, "unknown variant %d", o.variant);
+    if (o.dedup_huge < KC_DEDUP_AUTO || o.dedup_huge > KC_DEDUP_BYTEMAPS)
+      fail(KC_ERR_INVALID, "unknown dedup_huge mode %d", o.dedup_huge);
     if (o.learned) {
       if (o.variant != KC_VARIANT_EXP) fail(KC_ERR_UNSUPPORTED, "learned centre proposals apply to exponential contraction only");
       if (!g->mlp) fail(KC_ERR_INVALID, "learned centre proposals need a model (kc_graph_set_mlp)");

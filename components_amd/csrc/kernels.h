@@ -162,11 +162,16 @@ void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg
                   uint32_t *out, uint32_t *uniq, hipStream_t s);
 // out[2h] = bucket offset, out[2h+1] = size of huge bucket h.
 void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t *out, hipStream_t s);
-// Huge buckets against bitmaps of `words` u32 per bucket over the label space
-// (cstart: n_huge + 1 words of scratch for the chunk plan).
-void dedup_huge_bits(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
-                     uint64_t *cstart, uint32_t *bitmaps, uint64_t words, const uint32_t *bucket, uint32_t *out,
-                     uint32_t *uniq, uint64_t *scratch, hipStream_t s);
+// Huge buckets over the label space [0, space): those of at least map_min
+// elements (at most n_maps of them) against byte maps of huge_map_stride(space)
+// bytes, all zero on entry and left all zero; the others against bitmaps of
+// `words` u32 per bucket.  info: 3 n_huge + 6 words of scratch for the chunk
+// plans (map_min = ~0: bitmaps only).
+uint64_t huge_map_stride(uint64_t space);
+void dedup_huge_mixed(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
+                      uint64_t map_min, uint64_t n_maps, uint64_t *info, uint32_t *bitmaps, uint64_t words,
+                      uint8_t *maps, uint64_t space, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
+                      uint64_t *scratch, hipStream_t s);
 // Huge buckets in global open-addressing tables (pre-filled with kNone by the
 // caller, hcount zeroed), for label spaces whose bitmaps would not fit.
 void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,

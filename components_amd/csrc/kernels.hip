@@ -1019,14 +1019,140 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_bits(uint32_t n_huge, con
   }
 }
 
+// Chunks of huge bucket h when its size lies in [lo, hi), else none.
 struct HugeChunkLoad {
   const uint32_t *huge;
   const uint64_t *seg_off;
+  uint64_t lo, hi;
   __device__ uint64_t operator()(uint64_t h) const {
     const uint32_t a2 = huge[h];
-    return (seg_off[a2 + 1] - seg_off[a2] + kBitsChunk - 1) / kBitsChunk;
+    const uint64_t size = seg_off[a2 + 1] - seg_off[a2];
+    return size >= lo && size < hi ? (size + kBitsChunk - 1) / kBitsChunk : 0;
   }
 };
+
+// (chunks, is a map bucket) of huge bucket h: map buckets are those of at
+// least map_min elements.
+struct MapChunkLoad {
+  const uint32_t *huge;
+  const uint64_t *seg_off;
+  uint64_t map_min;
+  __device__ Pair64 operator()(uint64_t h) const {
+    const uint32_t a2 = huge[h];
+    const uint64_t size = seg_off[a2 + 1] - seg_off[a2];
+    return size >= map_min ? Pair64{(size + kBitsChunk - 1) / kBitsChunk, 1} : Pair64{0, 0};
+  }
+};
+
+// cstart[h] = first map chunk, mslot[h] = map slot (of map buckets; the next
+// slot for the others), mlist[slot] = h.
+struct MapChunkStore {
+  uint64_t *cstart;
+  uint32_t *mslot, *mlist;
+  __device__ void operator()(uint64_t h, Pair64 prefix, Pair64 v) const {
+    cstart[h] = prefix.a;
+    mslot[h] = static_cast<uint32_t>(prefix.b);
+    if (v.b) mlist[prefix.b] = static_cast<uint32_t>(h);
+  }
+};
+
+// Huge buckets of at least map_min elements against byte maps over the label
+// space.  Every element stores a 1 at its value's byte: repeats store the same
+// byte, so no lane needs to learn whether it came first and the device
+// performs no atomics (the bitmap path above needs one per distinct value).
+// The direct-mapped LDS filter only saves the stores of values repeated within
+// a chunk.  Map buckets take map slots in list order (mslot[h], at most
+// n_maps of them); a slot's map is `stride` bytes (the label space rounded up
+// to whole emit pieces) and is all zero on entry.  The chunks of the map
+// buckets are numbered as for the bitmaps (cstart, *total).
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_mark(uint32_t n_huge, const uint32_t *huge,
+                                                           const uint64_t *seg_off, const uint64_t *cstart,
+                                                           const uint64_t *total, const uint32_t *mslot,
+                                                           uint64_t n_maps, uint8_t *maps, uint64_t stride,
+                                                           uint64_t space, const uint32_t *bucket) {
+  __shared__ uint32_t filt[kHugeFilter];
+  const uint64_t T = *total;
+  for (uint64_t c = blockIdx.x; c < T; c += gridDim.x) {
+    uint32_t lo = 0, hi = n_huge;  // cstart[lo] <= c < cstart[hi] (cstart[n_huge] = T)
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (cstart[mid] <= c)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t a2 = huge[lo], slot = mslot[lo];
+    if (slot >= n_maps) continue;  // the caller's bound on the map buckets holds (never the case)
+    const uint64_t base = seg_off[a2], size = seg_off[a2 + 1] - base;
+    const uint64_t e_lo = (c - cstart[lo]) * kBitsChunk, e_hi = min(size, e_lo + kBitsChunk);
+    uint8_t *bm = maps + static_cast<uint64_t>(slot) * stride;
+    __syncthreads();  // the previous chunk's filter probes are done
+    for (int j = threadIdx.x; j < kHugeFilter; j += kBlock) filt[j] = kNone;
+    __syncthreads();
+    for (uint64_t j = e_lo + threadIdx.x; j < e_hi; j += kBlock) {
+      const uint32_t v = bucket[base + j];
+      if (v >= space) continue;  // cluster ids lie below the label space
+      const uint32_t f = hash32(v) & (kHugeFilter - 1);
+      if (filt[f] != v) {
+        filt[f] = v;
+        bm[v] = 1;
+      }
+    }
+  }
+}
+
+// The values whose byte is set, appended to their bucket's run in out: each
+// block step covers one piece of one map (16 bytes per thread), counts the set
+// bytes with wave scans, reserves the piece's range with one atomic on uniq
+// (zeroed for the huge buckets) and writes the values in map order.  Every
+// nonzero 16 bytes read are cleared again, so the maps end all zero.
+// mlist[k] = list index of the bucket in map slot k; *n_used = slots in use.
+constexpr uint32_t kMapPiece = 16 * kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_dedup_huge_emit_map(const uint32_t *huge, const uint64_t *seg_off,
+                                                               const uint32_t *mlist, const uint64_t *n_used,
+                                                               uint64_t n_maps, uint8_t *maps, uint64_t stride,
+                                                               uint32_t *out, uint32_t *uniq) {
+  __shared__ uint32_t wave_sum[kBlock / 64];
+  __shared__ uint32_t at;
+  const uint64_t per = stride / kMapPiece;
+  const uint64_t T = min(*n_used, n_maps) * per;
+  const uint32_t ln = lane_id(), wv = threadIdx.x / 64;
+  for (uint64_t c = blockIdx.x; c < T; c += gridDim.x) {
+    const uint64_t slot = c / per;
+    const uint64_t p = (c - slot * per) * kMapPiece + 16ull * threadIdx.x;
+    uint4 *src = reinterpret_cast<uint4 *>(maps + slot * stride + p);
+    const uint4 w = *src;
+    const uint32_t k = __popc(w.x) + __popc(w.y) + __popc(w.z) + __popc(w.w);  // set bytes hold 1
+    if (k) *src = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t inc = k;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o);
+      if (ln >= static_cast<uint32_t>(o)) inc += t;
+    }
+    if (ln == 63) wave_sum[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+      before += i < static_cast<int>(wv) ? wave_sum[i] : 0u;
+      tot += wave_sum[i];
+    }
+    const uint32_t a2 = huge[mlist[slot]];
+    if (threadIdx.x == 0) at = tot ? atomicAdd(&uniq[a2], tot) : 0u;
+    __syncthreads();
+    if (k) {
+      uint32_t *dst = out + seg_off[a2] + at + before + inc - k;
+      const uint32_t word[4] = {w.x, w.y, w.z, w.w};
+      uint32_t q = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        for (uint32_t x = word[i]; x; x &= x - 1)
+          dst[q++] = static_cast<uint32_t>(p) + 4u * i + (static_cast<uint32_t>(__ffs(x) - 1) >> 3);
+      }
+    }
+    __syncthreads();  // wave_sum and at are rewritten by the next step
+  }
+}
 
 __global__ void k_zero_huge_uniq(uint32_t n_huge, const uint32_t *huge, uint32_t *uniq) {
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1474,20 +1600,42 @@ void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *
   KC_LAUNCH_CHECK();
 }
 
-void dedup_huge_bits(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
-                     uint64_t *cstart, uint32_t *bitmaps, uint64_t words, const uint32_t *bucket, uint32_t *out,
-                     uint32_t *uniq, uint64_t *scratch, hipStream_t s) {
+uint64_t huge_map_stride(uint64_t space) { return (space + kMapPiece - 1) / kMapPiece * kMapPiece; }
+
+void dedup_huge_mixed(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
+                      uint64_t map_min, uint64_t n_maps, uint64_t *info, uint32_t *bitmaps, uint64_t words,
+                      uint8_t *maps, uint64_t space, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
+                      uint64_t *scratch, hipStream_t s) {
   if (!n_huge) return;
-  uint64_t *total = cstart + n_huge;
-  scan_exclusive<uint64_t>(n_huge, HugeChunkLoad{huge, seg_off}, ArrayStore<uint64_t>{cstart}, total, scratch, s);
-  fill_u32(bitmaps, static_cast<uint64_t>(n_huge) * words, 0u, s);
+  if (!n_maps || !space) map_min = ~0ull;
+  uint64_t *cbits = info, *cmap = info + n_huge + 1;
+  Pair64 *mtot = reinterpret_cast<Pair64 *>(info + 2 * n_huge + 2);
+  uint32_t *mslot = reinterpret_cast<uint32_t *>(info + 2 * n_huge + 4);
+  uint32_t *mlist = mslot + n_huge;
   hipLaunchKernelGGL(k_zero_huge_uniq, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s, n_huge,
                      huge, uniq);
   KC_LAUNCH_CHECK();
   const uint64_t g = std::min<uint64_t>(8192, max_chunks);
-  hipLaunchKernelGGL(k_dedup_huge_bits, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, n_huge, huge, seg_off,
-                     cstart, total, bitmaps, words, bucket, out, uniq);
-  KC_LAUNCH_CHECK();
+  if (map_min > kDedupBlockCap + 1) {  // some huge buckets against bitmaps
+    scan_exclusive<uint64_t>(n_huge, HugeChunkLoad{huge, seg_off, 0, map_min}, ArrayStore<uint64_t>{cbits},
+                             cbits + n_huge, scratch, s);
+    fill_u32(bitmaps, static_cast<uint64_t>(n_huge) * words, 0u, s);
+    hipLaunchKernelGGL(k_dedup_huge_bits, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, n_huge, huge, seg_off,
+                       cbits, cbits + n_huge, bitmaps, words, bucket, out, uniq);
+    KC_LAUNCH_CHECK();
+  }
+  if (map_min != ~0ull) {  // the others against byte maps
+    const uint64_t stride = huge_map_stride(space);
+    scan_exclusive<Pair64>(n_huge, MapChunkLoad{huge, seg_off, map_min}, MapChunkStore{cmap, mslot, mlist}, mtot,
+                           scratch, s);
+    hipLaunchKernelGGL(k_dedup_huge_mark, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, n_huge, huge, seg_off,
+                       cmap, &mtot->a, mslot, n_maps, maps, stride, space, bucket);
+    KC_LAUNCH_CHECK();
+    const uint64_t ge = std::min<uint64_t>(8192, n_maps * (stride / kMapPiece));
+    hipLaunchKernelGGL(k_dedup_huge_emit_map, dim3(static_cast<unsigned>(ge)), dim3(kBlock), 0, s, huge, seg_off,
+                       mlist, &mtot->b, n_maps, maps, stride, out, uniq);
+    KC_LAUNCH_CHECK();
+  }
 }
 
 void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint64_t *sizes, uint64_t *scratch,

@@ -123,10 +123,18 @@ typedef struct kc_options {
                              distance or pulled by the rows above it, whichever hold fewer edge slots:
                              0 = on levels of >= 2^16 vertices and average degree >= 8, 1 = on every
                              level, -1 = always pushed (same results) */
-  int32_t dedup_tables;   /* 1: huge cluster-edge buckets deduplicated in global hash tables (the path
-                             used when their bitmaps would exceed 2 GB) instead of bitmaps (same results) */
+  int32_t dedup_huge;     /* dedup of the huge cluster-edge buckets: KC_DEDUP_AUTO (byte maps over the
+                             cluster ids while they take at most 1 GB, then bitmaps up to 2 GB, then
+                             global hash tables), or one of KC_DEDUP_TABLES, KC_DEDUP_BITMAPS,
+                             KC_DEDUP_BYTEMAPS (same results) */
   int32_t reserved[1];
 } kc_options;
+
+/* Dedup of the huge cluster-edge buckets (kc_options.dedup_huge). */
+#define KC_DEDUP_AUTO 0
+#define KC_DEDUP_TABLES 1
+#define KC_DEDUP_BITMAPS 2
+#define KC_DEDUP_BYTEMAPS 3
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit
  * (per-vertex minstd_rand0 seeds; its outputs are linear in the seed, so the

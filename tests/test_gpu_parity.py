@@ -208,11 +208,13 @@ def test_rmat_scale22_schedules_agree():
     ref = oracle.canonical_labels(g.offsets, g.targets)
     traces = []
     # sweeps; rounds with directions chosen and pull sweeps at the tail (default); push rounds only;
-    # early switch to sweeps; directions chosen on every level; huge buckets in hash tables
-    for bfs, tail, r1, tab in ((False, 0, 0, False), (True, 0, 0, False), (True, -1, -1, False),
-                               (True, 1, 0, False), (True, 0, 1, False), (True, 0, 0, True)):
+    # early switch to sweeps; directions chosen on every level; huge buckets in hash tables, bitmaps,
+    # byte maps
+    for bfs, tail, r1, tab in ((False, 0, 0, "auto"), (True, 0, 0, "auto"), (True, -1, -1, "auto"),
+                               (True, 1, 0, "auto"), (True, 0, 1, "auto"), (True, 0, 0, "tables"),
+                               (True, 0, 0, "bitmaps"), (True, 0, 0, "bytemaps")):
         lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
-                                     use_bfs=bfs, tail_sweeps=tail, pull_rounds=r1, dedup_tables=tab,
+                                     use_bfs=bfs, tail_sweeps=tail, pull_rounds=r1, dedup_huge=tab,
                                      record_trace=True)
         assert np.array_equal(lab, ref), (bfs, tail, r1, tab)
         traces.append(dg.trace())
