@@ -524,19 +524,27 @@ __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
 // Centre labels and the maximum cluster radius (hops from the centre).
 // lab[v] = the centre's level index; the centre's key is at pos[l] when the
 // level is stored in another order (null: l).
-__global__ void k_labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
-                                int with_radius) {
-  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+// Grid-stride, with one atomic per workgroup: atomics on one address are
+// serialised by the device, so one per wave cost more than the pass itself.
+__global__ __launch_bounds__(kBlock) void k_labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos,
+                                                         uint32_t *lab, uint32_t *radius, int with_radius) {
+  __shared__ uint32_t wave_max[kBlock / kWave];
   uint32_t r = 0;
-  if (v < n) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
     const uint64_t k = key[v];
     const uint32_t l = static_cast<uint32_t>(k);
     lab[v] = l;
-    if (with_radius) r = static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[pos ? pos[l] : l] >> 32);
+    if (with_radius) r = max(r, static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[pos ? pos[l] : l] >> 32));
   }
   if (with_radius) {
     for (int o = 32; o > 0; o >>= 1) r = max(r, static_cast<uint32_t>(__shfl_xor(r, o)));
-    if (lane_id() == 0 && r > *radius) atomicMax(radius, r);
+    if (lane_id() == 0) wave_max[threadIdx.x / kWave] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < kBlock / kWave; ++w) r = max(r, wave_max[w]);
+      if (r) atomicMax(radius, r);
+    }
   }
 }
 
@@ -961,6 +969,9 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_emit(const HugeChunk *chu
 // one atomic on uniq (zeroed for the huge buckets).
 constexpr int kHugeFilter = 4096;
 constexpr uint32_t kStagePiece = 4096;  // entries of a chunk staged between two appends
+constexpr int kMarkBlock = 1024;       // threads of a mark workgroup
+constexpr int kMarkFilter = 16384;     // LDS filter entries (64 KB)
+constexpr uint64_t kMapChunk = 65536;  // elements per map-bucket chunk
 
 __global__ __launch_bounds__(kBlock) void k_dedup_huge_bits(uint32_t n_huge, const uint32_t *huge,
                                                            const uint64_t *seg_off, const uint64_t *cstart,
@@ -1040,7 +1051,7 @@ struct MapChunkLoad {
   __device__ Pair64 operator()(uint64_t h) const {
     const uint32_t a2 = huge[h];
     const uint64_t size = seg_off[a2 + 1] - seg_off[a2];
-    return size >= map_min ? Pair64{(size + kBitsChunk - 1) / kBitsChunk, 1} : Pair64{0, 0};
+    return size >= map_min ? Pair64{(size + kMapChunk - 1) / kMapChunk, 1} : Pair64{0, 0};
   }
 };
 
@@ -1063,14 +1074,15 @@ struct MapChunkStore {
 // The direct-mapped LDS filter only saves the stores of values repeated within
 // a chunk.  Map buckets take map slots in list order (mslot[h], at most
 // n_maps of them); a slot's map is `stride` bytes (the label space rounded up
-// to whole emit pieces) and is all zero on entry.  The chunks of the map
-// buckets are numbered as for the bitmaps (cstart, *total).
-__global__ __launch_bounds__(kBlock) void k_dedup_huge_mark(uint32_t n_huge, const uint32_t *huge,
-                                                           const uint64_t *seg_off, const uint64_t *cstart,
-                                                           const uint64_t *total, const uint32_t *mslot,
-                                                           uint64_t n_maps, uint8_t *maps, uint64_t stride,
-                                                           uint64_t space, const uint32_t *bucket) {
-  __shared__ uint32_t filt[kHugeFilter];
+// to whole emit pieces) and is all zero on entry.  The map buckets' chunks of
+// kMapChunk elements are numbered as for the bitmaps (cstart, *total); large
+// workgroups and chunks give the filter more repeats to catch.
+__global__ __launch_bounds__(kMarkBlock) void k_dedup_huge_mark(uint32_t n_huge, const uint32_t *huge,
+                                                               const uint64_t *seg_off, const uint64_t *cstart,
+                                                               const uint64_t *total, const uint32_t *mslot,
+                                                               uint64_t n_maps, uint8_t *maps, uint64_t stride,
+                                                               uint64_t space, const uint32_t *bucket) {
+  __shared__ uint32_t filt[kMarkFilter];
   const uint64_t T = *total;
   for (uint64_t c = blockIdx.x; c < T; c += gridDim.x) {
     uint32_t lo = 0, hi = n_huge;  // cstart[lo] <= c < cstart[hi] (cstart[n_huge] = T)
@@ -1084,15 +1096,15 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_mark(uint32_t n_huge, con
     const uint32_t a2 = huge[lo], slot = mslot[lo];
     if (slot >= n_maps) continue;  // the caller's bound on the map buckets holds (never the case)
     const uint64_t base = seg_off[a2], size = seg_off[a2 + 1] - base;
-    const uint64_t e_lo = (c - cstart[lo]) * kBitsChunk, e_hi = min(size, e_lo + kBitsChunk);
+    const uint64_t e_lo = (c - cstart[lo]) * kMapChunk, e_hi = min(size, e_lo + kMapChunk);
     uint8_t *bm = maps + static_cast<uint64_t>(slot) * stride;
     __syncthreads();  // the previous chunk's filter probes are done
-    for (int j = threadIdx.x; j < kHugeFilter; j += kBlock) filt[j] = kNone;
+    for (int j = threadIdx.x; j < kMarkFilter; j += kMarkBlock) filt[j] = kNone;
     __syncthreads();
-    for (uint64_t j = e_lo + threadIdx.x; j < e_hi; j += kBlock) {
+    for (uint64_t j = e_lo + threadIdx.x; j < e_hi; j += kMarkBlock) {
       const uint32_t v = bucket[base + j];
       if (v >= space) continue;  // cluster ids lie below the label space
-      const uint32_t f = hash32(v) & (kHugeFilter - 1);
+      const uint32_t f = hash32(v) & (kMarkFilter - 1);
       if (filt[f] != v) {
         filt[f] = v;
         bm[v] = 1;
@@ -1483,8 +1495,8 @@ void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, 
 void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
                    bool with_radius, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_labels_radius, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, pos, lab, radius,
-                     with_radius ? 1 : 0);
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock), 2048));
+  hipLaunchKernelGGL(k_labels_radius, dim3(g), dim3(kBlock), 0, s, n, key, pos, lab, radius, with_radius ? 1 : 0);
   KC_LAUNCH_CHECK();
 }
 
@@ -1628,8 +1640,8 @@ void dedup_huge_mixed(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg
     const uint64_t stride = huge_map_stride(space);
     scan_exclusive<Pair64>(n_huge, MapChunkLoad{huge, seg_off, map_min}, MapChunkStore{cmap, mslot, mlist}, mtot,
                            scratch, s);
-    hipLaunchKernelGGL(k_dedup_huge_mark, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, n_huge, huge, seg_off,
-                       cmap, &mtot->a, mslot, n_maps, maps, stride, space, bucket);
+    hipLaunchKernelGGL(k_dedup_huge_mark, dim3(static_cast<unsigned>(g)), dim3(kMarkBlock), 0, s, n_huge, huge,
+                       seg_off, cmap, &mtot->a, mslot, n_maps, maps, stride, space, bucket);
     KC_LAUNCH_CHECK();
     const uint64_t ge = std::min<uint64_t>(8192, n_maps * (stride / kMapPiece));
     hipLaunchKernelGGL(k_dedup_huge_emit_map, dim3(static_cast<unsigned>(ge)), dim3(kBlock), 0, s, huge, seg_off,
