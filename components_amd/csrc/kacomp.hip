@@ -217,6 +217,7 @@ struct kc_graph {
   HubChunk *fhubs = nullptr;
   uint64_t hubs_cap = 0;
   uint32_t *slots = nullptr, *seg_of_cid = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
+  uint64_t *hub_prefix = nullptr;  // exclusive prefix of the hub chunks' counts
   uint32_t *crank = nullptr, *cid = nullptr;  // dense cluster ids: rank by label, id per vertex
   uint32_t *sk = nullptr, *sv = nullptr, *sk2 = nullptr, *sv2 = nullptr;  // (label, vertex) sort buffers
   uint64_t *seg_off = nullptr, *sort_scratch = nullptr;
@@ -814,7 +815,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     }
     if (b.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
-      hub_fold(b, g->hub_inter, g->deg_inter, g->hub_pos, g->stream);
+      hub_fold(b, g->hub_inter, g->deg_inter, g->hub_pos, g->hub_prefix, g->scan_scratch, g->stream);
     }
     // clusters as segments of the (label, vertex) list sorted by label
     {
@@ -1011,6 +1012,7 @@ void init_work(kc_graph *g) {
   g->uniq = g->dalloc<uint32_t>(n);
   g->pos_of = g->dalloc<uint32_t>(n);
   g->hub_pos = g->dalloc<uint32_t>(g->hubs_cap);
+  g->hub_prefix = g->dalloc<uint64_t>(g->hubs_cap + 1);
   g->sk = g->dalloc<uint32_t>(n);
   g->sv = g->dalloc<uint32_t>(n);
   g->sk2 = g->dalloc<uint32_t>(n);

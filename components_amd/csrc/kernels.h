@@ -12,7 +12,7 @@ namespace kc {
 // A slice of a hub row (degree above the wave bin) handled by one wave.
 struct HubChunk {
   uint32_t v;
-  uint32_t first;  // 1 for the first chunk of v
+  uint32_t k;  // index of the chunk within v's chunks (0: the first)
   uint64_t e0, e1;
 };
 
@@ -90,8 +90,10 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
 // prev_pending (may be null) reading 0 makes the launch return at once.
 void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
                 int *pending, const int *prev_pending, hipStream_t s);
-// deg_inter of hubs from their chunk counts; hub_pos[c] = offset of chunk c in its hub's run.
-void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, hipStream_t s);
+// deg_inter of hubs from their chunk counts; hub_pos[c] = offset of chunk c in its hub's run
+// (prefix: b.nH + 1 words; scratch: scan scratch).
+void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, uint64_t *prefix,
+              uint64_t *scratch, hipStream_t s);
 
 // Pointer jumping on keys: key[v] <- min(key[v], key[label(v)]).
 void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_write(uint64_t n, const uint64_t
     for (uint32_t c = 0; c < chunks; ++c) {
       HubChunk h;
       h.v = static_cast<uint32_t>(v);
-      h.first = c == 0;
+      h.k = c;
       h.e0 = e0 + static_cast<uint64_t>(c) * cfg.wmax;
       h.e1 = min(e0 + deg, h.e0 + cfg.wmax);
       hubs[pos + c] = h;
@@ -493,17 +493,17 @@ __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(Sweep
     relabel_block(a, lk, relab, blk, tid, filt[q]);
 }
 
-__global__ void k_hub_fold(HubChunk *hubs, uint32_t nH, const uint32_t *hub_inter, uint32_t *deg_inter,
+// From the exclusive prefix of all chunk counts: a chunk's offset in its hub's
+// run is its prefix minus the hub's first chunk's, and the hub's last chunk
+// writes the hub's total (chunks of one hub are consecutive).
+__global__ void k_hub_fold(const HubChunk *hubs, uint32_t nH, const uint64_t *prefix, uint32_t *deg_inter,
                            uint32_t *hub_pos) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nH || !hubs[i].first) return;
-  const uint32_t v = hubs[i].v;
-  uint32_t sum = 0;
-  for (uint32_t c = i; c < nH && hubs[c].v == v; ++c) {
-    hub_pos[c] = sum;
-    sum += hub_inter[c];
-  }
-  deg_inter[v] = sum;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nH) return;
+  const HubChunk h = hubs[c];
+  const uint64_t first = prefix[c - h.k];
+  hub_pos[c] = static_cast<uint32_t>(prefix[c] - first);
+  if (c + 1 == nH || hubs[c + 1].k == 0) deg_inter[h.v] = static_cast<uint32_t>(prefix[c + 1] - first);
 }
 
 __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
@@ -1247,11 +1247,36 @@ __global__ void k_rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t 
   rep[v] = gid ? gid[a] : a;
 }
 
-__global__ void k_canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid, uint32_t *minid) {
-  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  const bool live = v < n;
-  const uint32_t id = live ? (gid ? gid[v] : static_cast<uint32_t>(v)) : 0u;
-  peeled_min(minid, live ? rep[v] : 0u, id, live);
+// minid[rep[v]] = min over v of the input id.  Grid-stride: each wave merges
+// the lanes sharing its first lane's representative and carries that minimum
+// while the next steps keep the same representative (the giant component's),
+// so one atomic per wave serves it instead of one per step; atomics on one
+// address are serialised by the device.  Other lanes take their own atomic
+// (skipped when minid is already smaller).
+__global__ __launch_bounds__(kBlock) void k_canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid,
+                                                     uint32_t *minid) {
+  uint32_t carry_key = kNone, carry_min = kNone;  // wave-uniform
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  const uint64_t v0 = blockIdx.x * static_cast<uint64_t>(kBlock) + (threadIdx.x & ~(kWave - 1));
+  for (uint64_t w = v0; w < n; w += stride) {  // w: the wave's first vertex this step
+    const uint64_t v = w + lane_id();
+    const bool live = v < n;
+    const uint32_t key = live ? rep[v] : 0u;
+    const uint32_t id = live ? (gid ? gid[v] : static_cast<uint32_t>(v)) : 0u;
+    const uint32_t lk = __shfl(key, 0);  // lane 0 is live whenever the wave has a step
+    const bool same = live && key == lk;
+    uint32_t mn = same ? id : kNone;
+    for (int o = 32; o > 0; o >>= 1) mn = min(mn, static_cast<uint32_t>(__shfl_xor(mn, o)));
+    if (lk == carry_key) {
+      carry_min = min(carry_min, mn);
+    } else {
+      if (carry_key != kNone && lane_id() == 0 && carry_min < minid[carry_key]) atomicMin(&minid[carry_key], carry_min);
+      carry_key = lk;
+      carry_min = mn;
+    }
+    if (live && !same && id < minid[key]) atomicMin(&minid[key], id);
+  }
+  if (carry_key != kNone && lane_id() == 0 && carry_min < minid[carry_key]) atomicMin(&minid[carry_key], carry_min);
 }
 
 // Block count of the lanes with pred set, written to partial[blockIdx.x].
@@ -1468,9 +1493,11 @@ void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_
   KC_LAUNCH_CHECK();
 }
 
-void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, hipStream_t s) {
+void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, uint64_t *prefix,
+              uint64_t *scratch, hipStream_t s) {
   if (!b.nH) return;
-  hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, hub_inter, deg_inter, hub_pos);
+  scan_exclusive<uint64_t>(b.nH, ArrLoadU32{hub_inter}, ArrayStore<uint64_t>{prefix}, prefix + b.nH, scratch, s);
+  hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, prefix, deg_inter, hub_pos);
   KC_LAUNCH_CHECK();
 }
 
@@ -1687,7 +1714,8 @@ void rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_
 
 void canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid, uint32_t *minid, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_canon_min, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, gid, minid);
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock), 2048));
+  hipLaunchKernelGGL(k_canon_min, dim3(g), dim3(kBlock), 0, s, n, rep, gid, minid);
   KC_LAUNCH_CHECK();
 }
 
