@@ -53,7 +53,7 @@ class Options(ctypes.Structure):
                 ("deviate_stream", ctypes.c_int32), ("learned_alpha", ctypes.c_float),
                 ("input_order", ctypes.c_int32), ("tail_sweeps", ctypes.c_int32),
                 ("pull_rounds", ctypes.c_int32),
-                ("dedup_huge", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+                ("dedup_huge", ctypes.c_int32), ("time_class", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -282,7 +282,7 @@ DEDUP_MODES = {"auto": 0, "tables": 1, "bitmaps": 2, "bytemaps": 3}
 
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
             tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
-            alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_huge="auto"):
+            alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_huge="auto", time_class=None):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -302,6 +302,8 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     o.tail_sweeps = int(tail_sweeps)
     o.pull_rounds = int(pull_rounds)
     o.dedup_huge = DEDUP_MODES[dedup_huge] if isinstance(dedup_huge, str) else int(dedup_huge)
+    # with time_kernels: events on the launches of one class only (a name of KERNEL_CLASSES)
+    o.time_class = 0 if time_class is None else KERNEL_CLASSES.index(time_class) + 1
     return o
 
 

@@ -91,8 +91,9 @@ KernelTimer::~KernelTimer() {
   for (auto e : pool_) (void)hipEventDestroy(e);
 }
 
-void KernelTimer::begin_run(bool enabled, hipStream_t s) {
+void KernelTimer::begin_run(bool enabled, hipStream_t s, int only_cls) {
   enabled_ = enabled;
+  only_ = only_cls;
   stream_ = s;
   recs_.clear();
   next_ev_ = 0;
@@ -100,7 +101,7 @@ void KernelTimer::begin_run(bool enabled, hipStream_t s) {
 
 int KernelTimer::start(int cls) {
   int ev = -1;
-  if (enabled_) {
+  if (enabled_ && (only_ < 0 || cls == only_)) {
     while (next_ev_ + 2 > pool_.size()) {
       hipEvent_t e;
       KC_HIP(hipEventCreate(&e));
@@ -133,7 +134,7 @@ void KernelTimer::finish(kc_stats *st) {
     st->kernel_bytes[i] = 0;
   }
   if (enabled_) KC_HIP(hipStreamSynchronize(stream_));
-  for (auto &r : recs_) {
+  for (auto &r : recs_) {  // launches without events add no time
     st->kernel_launches[r.cls]++;
     st->kernel_bytes[r.cls] += r.bytes;
     if (r.ev >= 0) {
@@ -1281,6 +1282,8 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
     if (o.variant < KC_VARIANT_EXP || o.variant > KC_VARIANT_LABELS) fail(KC_ERR_INVALID, "unknown variant %d", o.variant);
     if (o.dedup_huge < KC_DEDUP_AUTO || o.dedup_huge > KC_DEDUP_BYTEMAPS)
       fail(KC_ERR_INVALID, "unknown dedup_huge mode %d", o.dedup_huge);
+    if (o.time_class < 0 || o.time_class > KC_KERNEL_CLASSES)
+      fail(KC_ERR_INVALID, "time_class must be in [0, %d], got %d", KC_KERNEL_CLASSES, o.time_class);
     if (o.learned) {
       if (o.variant != KC_VARIANT_EXP) fail(KC_ERR_UNSUPPORTED, "learned centre proposals apply to exponential contraction only");
       if (!g->mlp) fail(KC_ERR_INVALID, "learned centre proposals need a model (kc_graph_set_mlp)");
@@ -1291,7 +1294,7 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
     g->have_labels = false;
     if (o.variant == KC_VARIANT_EXP) ensure_table(g, o.beta);
     g->arena.reset();
-    g->timer.begin_run(o.time_kernels != 0, g->stream);
+    g->timer.begin_run(o.time_kernels != 0, g->stream, o.time_class - 1);
     KC_HIP(hipEventRecord(g->ev_begin, g->stream));
     RunOut out;
     if (o.variant == KC_VARIANT_EXP || o.variant == KC_VARIANT_LOCAL)

@@ -93,7 +93,8 @@ class Arena {
 class KernelTimer {
  public:
   ~KernelTimer();
-  void begin_run(bool enabled, hipStream_t s);
+  // only_cls >= 0: events for the launches of that class alone.
+  void begin_run(bool enabled, hipStream_t s, int only_cls = -1);
   int start(int cls);
   void stop(int slot, uint64_t alg_bytes);
   void retag(int slot, int cls, uint64_t alg_bytes);
@@ -102,6 +103,7 @@ class KernelTimer {
 
  private:
   bool enabled_ = false;
+  int only_ = -1;
   hipStream_t stream_ = nullptr;
   std::vector<hipEvent_t> pool_;
   struct Rec {

@@ -127,7 +127,8 @@ typedef struct kc_options {
                              cluster ids while they take at most 1 GB, then bitmaps up to 2 GB, then
                              global hash tables), or one of KC_DEDUP_TABLES, KC_DEDUP_BITMAPS,
                              KC_DEDUP_BYTEMAPS (same results) */
-  int32_t reserved[1];
+  int32_t time_class;     /* with time_kernels: 0 = every launch bracketed by HIP events, k > 0 = only
+                             the launches of class k - 1 (the others are counted, not timed) */
 } kc_options;
 
 /* Dedup of the huge cluster-edge buckets (kc_options.dedup_huge). */
