@@ -227,7 +227,6 @@ struct kc_graph {
   uint64_t *off_pp[2] = {nullptr, nullptr};
   uint32_t *col_pp[2] = {nullptr, nullptr};
   uint32_t *dl_medium = nullptr, *dl_huge = nullptr, *hcount = nullptr;
-  uint64_t *dl_huge_tab = nullptr;
   uint32_t *huge_tables = nullptr;
   uint64_t huge_cap = 0;
   uint32_t *huge_bits = nullptr;  // bitmaps of the huge buckets over the label space
@@ -413,25 +412,21 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
 // in wave tables, medium ones in block tables, huge ones against byte maps or
 // bitmaps over the label space (mode KC_DEDUP_AUTO: byte maps while they fit
 // their cap, then bitmaps) or in global tables processed in fixed-size chunks
-// planned on the host.  Returns the timer slot of the first dedup launch (its
-// algorithmic bytes are booked by the caller).  `scratch` serves the scans
-// (the handle's own scan scratch does not exist yet while a generated graph's
-// CSR is built).
+// planned on the host.  The medium and huge lists (n_medium, n_huge) come
+// from classify_buckets; total = elements of all buckets.  Returns the timer
+// slot of the first dedup launch (its algorithmic bytes are booked by the
+// caller).  `scratch` serves the scans (the handle's own scan scratch does not
+// exist yet while a generated graph's CSR is built).
 int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
-                  uint32_t *uniq, uint32_t *medium, uint32_t *huge, uint64_t *huge_tab, uint64_t label_space,
-                  uint64_t *scratch, int mode = KC_DEDUP_AUTO) {
+                  uint32_t *uniq, const uint32_t *medium, const uint32_t *huge, uint64_t n_medium, uint64_t n_huge,
+                  uint64_t total, uint64_t label_space, uint64_t *scratch, int mode = KC_DEDUP_AUTO) {
   hipStream_t s = g->stream;
-  fill_u64(g->sizes + 3, 3, 0, s);
-  DedupLists dl{medium, huge, huge_tab, g->sizes + 3};
   int first;
   {
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
     first = ts.slot;
-    dedup_small(nb, seg_off, bucket, out, uniq, dl, s);
+    dedup_small(nb, seg_off, bucket, out, uniq, s);
   }
-  d2h(g, g->h_sizes + 3, g->sizes + 3, 3 * sizeof(uint64_t));
-  sync(g);
-  const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4], huge_slots = g->h_sizes[5];
   if (n_medium) {
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
     dedup_medium(static_cast<uint32_t>(n_medium), medium, seg_off, bucket, out, uniq, s);
@@ -446,10 +441,9 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   // over C bytes (the emit reads the map and clears it) but no atomics, the
   // bitmaps one device atomic per distinct value (about 0.12 ns each against
   // 0.4 ps per map byte), so the maps pay from about C/128 distinct values
-  // on.  Their number is at most huge entries / map_min <= huge_slots /
-  // (2 map_min) (every table holds at least twice its bucket), which bounds
-  // the maps' memory before any size is read back; map_min rises until that
-  // bound fits the maps' cap.
+  // on.  Their number is at most total / map_min, which bounds the maps'
+  // memory before any size is read back; map_min rises until that bound fits
+  // the maps' cap.
   const uint64_t words = (label_space + 31) / 32;
   if (mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_BITMAPS ||
       (mode == KC_DEDUP_AUTO && n_huge * words * 4 <= kHugeBitmapBytes)) {
@@ -459,9 +453,9 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
       map_min = mode == KC_DEDUP_BYTEMAPS ? kDedupBlockCap + 1
                                           : std::max<uint64_t>(kDedupBlockCap + 1, label_space / kMapMinShare);
       const uint64_t cap_maps = kHugeMapBytes / stride;
-      while (map_min != ~0ull && std::min(n_huge, huge_slots / (2 * map_min)) > cap_maps)
-        map_min = cap_maps ? huge_slots / (2 * cap_maps) + 1 : ~0ull;
-      if (map_min != ~0ull) n_maps = std::min(n_huge, huge_slots / (2 * map_min));
+      while (map_min != ~0ull && std::min(n_huge, total / map_min) > cap_maps)
+        map_min = cap_maps ? total / cap_maps + 1 : ~0ull;
+      if (map_min != ~0ull) n_maps = std::min(n_huge, total / map_min);
       if (!n_maps) map_min = ~0ull;
     }
     if (map_min > kDedupBlockCap + 1 && n_huge * words > g->huge_bits_cap) {
@@ -479,18 +473,19 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
     return first;
   }
   huge_sizes(static_cast<uint32_t>(n_huge), huge, seg_off, g->d_huge_info, s);
-  std::vector<uint64_t> info(2 * n_huge), tab(n_huge);
+  std::vector<uint64_t> info(2 * n_huge);
   std::vector<uint32_t> a2s(n_huge);
   KC_HIP(hipMemcpyAsync(info.data(), g->d_huge_info, 16 * n_huge, hipMemcpyDeviceToHost, s));
-  KC_HIP(hipMemcpyAsync(tab.data(), huge_tab, 8 * n_huge, hipMemcpyDeviceToHost, s));
   KC_HIP(hipMemcpyAsync(a2s.data(), huge, 4 * n_huge, hipMemcpyDeviceToHost, s));
   sync(g);
   std::vector<HugeChunk> elem, slot;
+  uint64_t huge_slots = 0;  // tables of at least twice their bucket, one after another
   for (uint64_t h = 0; h < n_huge; ++h) {
     const uint64_t size = info[2 * h + 1];
     uint64_t tsz = 1;
     while (tsz < 2 * size) tsz <<= 1;
-    HugeChunk c{info[2 * h], tab[h], 0, 0, static_cast<uint32_t>(tsz - 1), static_cast<uint32_t>(h), a2s[h], 0};
+    HugeChunk c{info[2 * h], huge_slots, 0, 0, static_cast<uint32_t>(tsz - 1), static_cast<uint32_t>(h), a2s[h], 0};
+    huge_slots += tsz;
     for (uint64_t lo = 0; lo < size; lo += kHugeChunk) {
       c.lo = lo;
       c.hi = std::min(size, lo + kHugeChunk);
@@ -841,11 +836,19 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       segments(N, sk, sv, g->deg_inter, gid_l, g->crank, g->pos_of, rec.newidx, g->seg_of_cid, gid_next, g->seg_off,
                g->sizes, g->scan_scratch, g->stream);
     }
-    d2h(g, g->h_sizes, g->sizes, 2 * sizeof(uint64_t));
+    // medium and huge buckets listed before the sizes come back (the
+    // clusters bound n_next)
+    {
+      TimedScope ts(g->timer, KC_K_SCAN, 16 * clusters + 4 * clusters / 8);
+      classify_buckets(std::min(N, clusters), g->sizes, g->seg_off, g->dl_medium, g->dl_huge, g->sizes + 3,
+                       g->scan_scratch, g->stream);
+    }
+    d2h(g, g->h_sizes, g->sizes, 5 * sizeof(uint64_t));
     d2h(g, g->h_small + 4, g->small + 4, sizeof(uint32_t));
     sync(g);
     const uint64_t n_next = N ? g->h_sizes[0] : 0;
     const uint64_t P = N ? g->h_sizes[1] : 0;
+    const uint64_t n_medium = N ? g->h_sizes[3] : 0, n_huge = N ? g->h_sizes[4] : 0;
     rec.radius = local ? 0 : g->h_small[4];
     out.sweeps += sweeps;
     out.radius_sum += rec.radius;
@@ -864,7 +867,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, bucket, g->stream);
     }
     const int dedup_slot = dedup_buckets(g, n_next, g->seg_off, bucket, g->slots, g->uniq, g->dl_medium, g->dl_huge,
-                                         g->dl_huge_tab, clusters, g->scan_scratch, o->dedup_huge);
+                                         n_medium, n_huge, P, clusters, g->scan_scratch, o->dedup_huge);
     // next-level CSR
     {
       TimedScope ts(g->timer, KC_K_SCAN, 4 * n_next + 8 * (n_next + 1));
@@ -1031,7 +1034,6 @@ void init_work(kc_graph *g) {
   }
   g->dl_medium = g->dalloc<uint32_t>(n);
   g->dl_huge = g->dalloc<uint32_t>(n);
-  g->dl_huge_tab = g->dalloc<uint64_t>(n);
   g->hcount = g->dalloc<uint32_t>(n);
   g->flags = g->dalloc<int>(kMaxFlags);
   // level 0 without the degree-0 vertices (rep_a holds the id map meanwhile)
@@ -1097,13 +1099,15 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   DevTemp bucket(P * 4 + 64);
   fill_u32(fill.as<uint32_t>(), n, 0, s);
   edge_scatter(count, src, dst, off_tmp.as<uint64_t>(), fill.as<uint32_t>(), bucket.as<uint32_t>(), s);
-  DevTemp lists((3 * n + 8) * 8);
+  DevTemp lists((2 * n + 8) * 4);
   uint32_t *medium = lists.as<uint32_t>();
   uint32_t *huge = medium + n + 2;
-  uint64_t *huge_tab = reinterpret_cast<uint64_t *>(huge + n + 2) + 1;
   DevTemp runs(P * 4 + 64);
+  classify_buckets(n, nullptr, off_tmp.as<uint64_t>(), medium, huge, g->sizes + 3, scratch.as<uint64_t>(), s);
+  KC_HIP(hipMemcpyAsync(g->h_sizes + 3, g->sizes + 3, 16, hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
   dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), runs.as<uint32_t>(), uniq.as<uint32_t>(), medium,
-                huge, huge_tab, n, scratch.as<uint64_t>());
+                huge, g->h_sizes[3], g->h_sizes[4], P, n, scratch.as<uint64_t>());
   g->off0 = g->dalloc<uint64_t>(n + 1);
   csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));

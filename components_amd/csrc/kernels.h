@@ -136,14 +136,8 @@ void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const ui
 
 // Dedup of every bucket a2 (elements [seg_off[a2], seg_off[a2+1]) of bucket):
 // its uniq[a2] distinct values go to out[seg_off[a2] ...] in any order.  Small
-// buckets in wave LDS tables; medium and huge ones are listed (counters zeroed
-// by the caller: [0] medium count, [1] huge count, [2] huge table slots).
-struct DedupLists {
-  uint32_t *medium;    // a2 of buckets with wave_cap < size <= block_cap
-  uint32_t *huge;      // a2 of buckets above block_cap
-  uint64_t *huge_tab;  // table offset per huge bucket
-  uint64_t *counters;
-};
+// buckets in wave LDS tables (dedup_small, which skips the others); medium
+// and huge ones are listed first by classify_buckets.
 constexpr uint32_t kDedupWaveCap = 512;
 constexpr uint32_t kDedupBlockCap = 8192;
 constexpr uint64_t kHugeChunk = 16384;  // elements (or table slots) per huge-bucket chunk (tables)
@@ -158,8 +152,13 @@ struct HugeChunk {
   uint32_t a2;        // bucket (next-level vertex)
   uint32_t pad;
 };
+// Medium (wave cap < size <= block cap) and huge (above) buckets among the
+// first *n_dev (or n when n_dev is null; n bounds *n_dev), listed in
+// increasing order: counts[0] = medium, counts[1] = huge.  scratch: scan scratch.
+void classify_buckets(uint64_t n, const uint64_t *n_dev, const uint64_t *seg_off, uint32_t *medium, uint32_t *huge,
+                      uint64_t *counts, uint64_t *scratch, hipStream_t s);
 void dedup_small(uint64_t n_next, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
-                 DedupLists l, hipStream_t s);
+                 hipStream_t s);
 void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, const uint32_t *bucket,
                   uint32_t *out, uint32_t *uniq, hipStream_t s);
 // out[2h] = bucket offset, out[2h+1] = size of huge bucket h.

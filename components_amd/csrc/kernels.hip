@@ -804,73 +804,133 @@ __device__ __forceinline__ bool lds_insert(uint32_t *tab, uint32_t mask, uint32_
   }
 }
 
+// Medium and huge buckets listed by a scan (positions from the prefix, no
+// atomics: appends through one shared counter are serialised by the device).
+struct ClassLoad {
+  const uint64_t *n_dev;
+  uint64_t n;
+  const uint64_t *seg_off;
+  __device__ Pair64 operator()(uint64_t i) const {
+    if (i >= (n_dev ? *n_dev : n)) return Pair64{0, 0};
+    const uint64_t s = seg_off[i + 1] - seg_off[i];
+    return s <= kDedupWaveCap ? Pair64{0, 0} : s <= kDedupBlockCap ? Pair64{1, 0} : Pair64{0, 1};
+  }
+};
+struct ClassStore {
+  uint32_t *medium, *huge;
+  __device__ void operator()(uint64_t i, Pair64 prefix, Pair64 v) const {
+    if (v.a) medium[prefix.a] = static_cast<uint32_t>(i);
+    if (v.b) huge[prefix.b] = static_cast<uint32_t>(i);
+  }
+};
+
+// Small buckets, four consecutive ones per wave step.  When all four hold at
+// most 16 elements (most buckets), each takes 16 lanes, one element per lane,
+// and a lane keeps its element when no lower lane of its group holds the same
+// value: no table, no LDS.  Otherwise the wave takes the four in turn, each
+// against an open-addressing table in LDS (medium and huge ones are skipped).
+constexpr uint32_t kTinyBucket = 16;
+
 __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint64_t *seg_off,
-                                                       const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
-                                                       DedupLists l) {
+                                                       const uint32_t *bucket, uint32_t *out, uint32_t *uniq) {
   __shared__ uint32_t tabs[kBlock / kWave][kWaveTab];
   const int w = threadIdx.x / kWave;
-  const int ln = threadIdx.x & 63;
+  const uint32_t ln = threadIdx.x & 63, grp = ln / kTinyBucket, gl = ln % kTinyBucket;
   uint32_t *tab = tabs[w];
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
-  for (uint64_t a2 = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + w; a2 < n_next; a2 += waves) {
-    const uint64_t base = seg_off[a2];
-    const uint32_t s = static_cast<uint32_t>(seg_off[a2 + 1] - base);
-    if (s > kDedupWaveCap) {
-      if (ln == 0) {
-        if (s <= kDedupBlockCap) {
-          uint64_t p = atomicAdd(reinterpret_cast<unsigned long long *>(&l.counters[0]), 1ull);
-          l.medium[p] = static_cast<uint32_t>(a2);
-        } else {
-          uint64_t p = atomicAdd(reinterpret_cast<unsigned long long *>(&l.counters[1]), 1ull);
-          uint64_t tsz = 1;
-          while (tsz < 2ull * s) tsz <<= 1;
-          uint64_t t = atomicAdd(reinterpret_cast<unsigned long long *>(&l.counters[2]),
-                                 static_cast<unsigned long long>(tsz));
-          l.huge[p] = static_cast<uint32_t>(a2);
-          l.huge_tab[p] = t;
+  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + w; 4 * q < n_next; q += waves) {
+    const uint64_t ag = 4 * q + grp;  // this lane group's bucket
+    uint64_t gbase = 0;
+    uint32_t gs = 0;
+    if (ag < n_next) {
+      gbase = seg_off[ag];
+      gs = static_cast<uint32_t>(seg_off[ag + 1] - gbase);
+    }
+    if (__ballot(gs > kTinyBucket) == 0) {
+      const uint32_t v = gl < gs ? bucket[gbase + gl] : kNone;
+      bool keep = gl < gs;
+      for (uint32_t j = 0; j < kTinyBucket; ++j) {
+        const uint32_t u = static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(grp * kTinyBucket + j)));
+        if (j < gl && u == v) keep = false;
+      }
+      const uint32_t gm = static_cast<uint32_t>((__ballot(keep) >> (grp * kTinyBucket)) & 0xFFFFu);
+      if (keep) out[gbase + __popc(gm & ((1u << gl) - 1u))] = v;
+      if (gl == 0 && ag < n_next) uniq[ag] = __popc(gm);
+      continue;
+    }
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint64_t a2 = 4 * q + k;
+      if (a2 >= n_next) break;
+      const uint64_t base = __shfl(gbase, static_cast<int>(k * kTinyBucket));
+      const uint32_t s = static_cast<uint32_t>(__shfl(static_cast<int>(gs), static_cast<int>(k * kTinyBucket)));
+      if (s > kDedupWaveCap) continue;  // listed as medium or huge
+      if (s <= 1) {
+        if (ln == 0) {
+          uniq[a2] = s;
+          if (s) out[base] = bucket[base];
         }
+        continue;
       }
-      continue;
-    }
-    if (s <= 1) {
-      if (ln == 0) {
-        uniq[a2] = s;
-        if (s) out[base] = bucket[base];
+      uint32_t tsz = 2;
+      while (tsz < 2u * s) tsz <<= 1;
+      const uint32_t mask = tsz - 1;
+      for (uint32_t j = ln; j < tsz; j += kWave) tab[j] = kNone;
+      wave_fence();
+      for (uint32_t j = ln; j < s; j += kWave) lds_insert(tab, mask, bucket[base + j]);
+      wave_fence();
+      // emit table contents in slot order
+      uint32_t written = 0;
+      for (uint32_t j0 = 0; j0 < tsz; j0 += kWave) {
+        const uint32_t j = j0 + ln;
+        const uint32_t val = j < tsz ? tab[j] : kNone;
+        const bool keep = val != kNone;
+        const uint64_t m = __ballot(keep);
+        if (keep) out[base + written + __popcll(m & lanes_below())] = val;
+        written += __popcll(m);
       }
-      continue;
+      if (ln == 0) uniq[a2] = written;
+      wave_fence();
     }
-    uint32_t tsz = 2;
-    while (tsz < 2u * s) tsz <<= 1;
-    const uint32_t mask = tsz - 1;
-    for (uint32_t j = ln; j < tsz; j += kWave) tab[j] = kNone;
-    wave_fence();
-    for (uint32_t j = ln; j < s; j += kWave) lds_insert(tab, mask, bucket[base + j]);
-    wave_fence();
-    // emit table contents in slot order
-    uint32_t written = 0;
-    for (uint32_t j0 = 0; j0 < tsz; j0 += kWave) {
-      const uint32_t j = j0 + ln;
-      const uint32_t val = j < tsz ? tab[j] : kNone;
-      const bool keep = val != kNone;
-      const uint64_t m = __ballot(keep);
-      if (keep) out[base + written + __popcll(m & lanes_below())] = val;
-      written += __popcll(m);
-    }
-    if (ln == 0) uniq[a2] = written;
-    wave_fence();
   }
 }
 
+// Exclusive block-wide scan of v (wave scans, one LDS pass over the wave
+// sums); total = the block's sum.  Every thread of the block calls it.
+__device__ __forceinline__ uint32_t block_scan_waves(uint32_t v, uint32_t *wsum, uint32_t &total) {
+  const uint32_t ln = lane_id(), wv = threadIdx.x / kWave;
+  uint32_t inc = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o);
+    if (ln >= static_cast<uint32_t>(o)) inc += t;
+  }
+  if (ln == kWave - 1) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t before = 0;
+  total = 0;
+  for (uint32_t i = 0; i < kBlock / kWave; ++i) {
+    const uint32_t x = wsum[i];
+    before += i < wv ? x : 0u;
+    total += x;
+  }
+  __syncthreads();  // wsum is free again
+  return before + inc - v;
+}
+
+// Medium buckets with s_lo <= size <= s_hi, one workgroup each, against a
+// table of Tab LDS slots (at least twice the largest size): the smaller class
+// takes a quarter of the LDS, so four times as many workgroups share a CU.
+template <uint32_t Tab>
 __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, const uint32_t *medium,
                                                         const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
-                                                        uint32_t *uniq) {
-  __shared__ uint32_t tab[kBlockTab];
-  __shared__ uint32_t psum[kBlock];
+                                                        uint32_t *uniq, uint32_t s_lo, uint32_t s_hi) {
+  __shared__ uint32_t tab[Tab];
+  __shared__ uint32_t wsum[kBlock / kWave];
   const int tid = threadIdx.x;
   for (uint32_t i = blockIdx.x; i < n_medium; i += gridDim.x) {
     const uint32_t a2 = medium[i];
     const uint64_t base = seg_off[a2];
     const uint32_t s = static_cast<uint32_t>(seg_off[a2 + 1] - base);
+    if (s < s_lo || s > s_hi) continue;  // the other class's launch takes it
     uint32_t tsz = 2;
     while (tsz < 2u * s) tsz <<= 1;
     const uint32_t mask = tsz - 1;
@@ -884,7 +944,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, cons
     uint32_t c = 0;
     for (uint32_t j = j0; j < j0 + per && j < tsz; ++j) c += tab[j] != kNone;
     uint32_t tot;
-    uint32_t p = block_exclusive_scan_shift<uint32_t>(c, psum, &tot);
+    uint32_t p = block_scan_waves(c, wsum, tot);
     for (uint32_t j = j0; j < j0 + per && j < tsz; ++j)
       if (tab[j] != kNone) out[base + p++] = tab[j];
     if (tid == 0) uniq[a2] = tot;
@@ -1607,21 +1667,32 @@ void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const ui
   KC_LAUNCH_CHECK();
 }
 
+void classify_buckets(uint64_t n, const uint64_t *n_dev, const uint64_t *seg_off, uint32_t *medium, uint32_t *huge,
+                      uint64_t *counts, uint64_t *scratch, hipStream_t s) {
+  scan_exclusive<Pair64>(n, ClassLoad{n_dev, n, seg_off}, ClassStore{medium, huge}, reinterpret_cast<Pair64 *>(counts),
+                         scratch, s);
+}
+
 void dedup_small(uint64_t n_next, const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
-                 DedupLists l, hipStream_t s) {
+                 hipStream_t s) {
   if (!n_next) return;
-  uint64_t blocks = ceil_div(n_next, kBlock / kWave);
+  uint64_t blocks = ceil_div(ceil_div(n_next, 4), kBlock / kWave);
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_dedup_small, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n_next, seg_off, bucket,
-                     out, uniq, l);
+                     out, uniq);
   KC_LAUNCH_CHECK();
 }
 
 void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg_off, const uint32_t *bucket,
                   uint32_t *out, uint32_t *uniq, hipStream_t s) {
   if (!n_medium) return;
-  unsigned blocks = n_medium > 2048 ? 2048 : n_medium;
-  hipLaunchKernelGGL(k_dedup_medium, dim3(blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off, bucket, out, uniq);
+  constexpr uint32_t kSplit = kDedupBlockCap / 4;  // sizes up to here take the small tables
+  const unsigned small_blocks = n_medium > 8192 ? 8192 : n_medium, big_blocks = n_medium > 2048 ? 2048 : n_medium;
+  hipLaunchKernelGGL(k_dedup_medium<2 * kSplit>, dim3(small_blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off,
+                     bucket, out, uniq, 0u, kSplit);
+  KC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dedup_medium<kBlockTab>, dim3(big_blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off,
+                     bucket, out, uniq, kSplit + 1, kDedupBlockCap);
   KC_LAUNCH_CHECK();
 }
 
