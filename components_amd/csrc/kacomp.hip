@@ -298,7 +298,7 @@ constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average
                                                 // high-diameter levels keep batched pushes)
 constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
 constexpr uint64_t kHugeMapBytes = 4ull << 30;     // byte maps of the largest huge buckets
-constexpr uint64_t kMapMinShare = 128;             // byte maps from C / 128 elements on
+constexpr uint64_t kMapMinShare = 32;              // byte maps from C / 32 elements on
 constexpr uint32_t kMinWaveBin = 256;
 
 
@@ -441,9 +441,11 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   // over C bytes (the emit reads the map and clears it) but no atomics, the
   // bitmaps one device atomic per distinct value (about 0.12 ns each against
   // 0.4 ps per map byte), so the maps pay from about C/128 distinct values
-  // on.  Their number is at most total / map_min, which bounds the maps'
-  // memory before any size is read back; map_min rises until that bound fits
-  // the maps' cap.
+  // on; a bucket's distinct values are fewer than its elements, and on R-MAT
+  // 24 the step is fastest with the maps from C/32 elements on (flat between
+  // C/16 and C/64).  Their number is at most total / map_min, which bounds the
+  // maps' memory before any size is read back; map_min rises until that bound
+  // fits the maps' cap.
   const uint64_t words = (label_space + 31) / 32;
   if (mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_BITMAPS ||
       (mode == KC_DEDUP_AUTO && n_huge * words * 4 <= kHugeBitmapBytes)) {
