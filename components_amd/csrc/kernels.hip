@@ -894,8 +894,9 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
   }
 }
 
-// Exclusive block-wide scan of v (wave scans, one LDS pass over the wave
-// sums); total = the block's sum.  Every thread of the block calls it.
+// Exclusive block-wide scan of v over B threads (wave scans, one LDS pass
+// over the wave sums); total = the block's sum.  Every thread calls it.
+template <int B>
 __device__ __forceinline__ uint32_t block_scan_waves(uint32_t v, uint32_t *wsum, uint32_t &total) {
   const uint32_t ln = lane_id(), wv = threadIdx.x / kWave;
   uint32_t inc = v;
@@ -907,7 +908,7 @@ __device__ __forceinline__ uint32_t block_scan_waves(uint32_t v, uint32_t *wsum,
   __syncthreads();
   uint32_t before = 0;
   total = 0;
-  for (uint32_t i = 0; i < kBlock / kWave; ++i) {
+  for (uint32_t i = 0; i < B / kWave; ++i) {
     const uint32_t x = wsum[i];
     before += i < wv ? x : 0u;
     total += x;
@@ -916,15 +917,16 @@ __device__ __forceinline__ uint32_t block_scan_waves(uint32_t v, uint32_t *wsum,
   return before + inc - v;
 }
 
-// Medium buckets with s_lo <= size <= s_hi, one workgroup each, against a
-// table of Tab LDS slots (at least twice the largest size): the smaller class
-// takes a quarter of the LDS, so four times as many workgroups share a CU.
-template <uint32_t Tab>
-__global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, const uint32_t *medium,
-                                                        const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
-                                                        uint32_t *uniq, uint32_t s_lo, uint32_t s_hi) {
+// Medium buckets with s_lo <= size <= s_hi, one workgroup of B threads each,
+// against a table of Tab LDS slots (at least twice the largest size): the
+// smaller class takes a quarter of the LDS, so four times as many workgroups
+// share a CU; the larger class, two workgroups to a CU, takes 1024 threads.
+template <uint32_t Tab, int B>
+__global__ __launch_bounds__(B) void k_dedup_medium(uint32_t n_medium, const uint32_t *medium,
+                                                   const uint64_t *seg_off, const uint32_t *bucket, uint32_t *out,
+                                                   uint32_t *uniq, uint32_t s_lo, uint32_t s_hi) {
   __shared__ uint32_t tab[Tab];
-  __shared__ uint32_t wsum[kBlock / kWave];
+  __shared__ uint32_t wsum[B / kWave];
   const int tid = threadIdx.x;
   for (uint32_t i = blockIdx.x; i < n_medium; i += gridDim.x) {
     const uint32_t a2 = medium[i];
@@ -934,17 +936,17 @@ __global__ __launch_bounds__(kBlock) void k_dedup_medium(uint32_t n_medium, cons
     uint32_t tsz = 2;
     while (tsz < 2u * s) tsz <<= 1;
     const uint32_t mask = tsz - 1;
-    for (uint32_t j = tid; j < tsz; j += kBlock) tab[j] = kNone;
+    for (uint32_t j = tid; j < tsz; j += B) tab[j] = kNone;
     __syncthreads();
-    for (uint32_t j = tid; j < s; j += kBlock) lds_insert(tab, mask, bucket[base + j]);
+    for (uint32_t j = tid; j < s; j += B) lds_insert(tab, mask, bucket[base + j]);
     __syncthreads();
     // each thread owns a contiguous range of slots
-    const uint32_t per = tsz / kBlock > 0 ? tsz / kBlock : 1;
+    const uint32_t per = tsz / B > 0 ? tsz / B : 1;
     const uint32_t j0 = tid * per;
     uint32_t c = 0;
     for (uint32_t j = j0; j < j0 + per && j < tsz; ++j) c += tab[j] != kNone;
     uint32_t tot;
-    uint32_t p = block_scan_waves(c, wsum, tot);
+    uint32_t p = block_scan_waves<B>(c, wsum, tot);
     for (uint32_t j = j0; j < j0 + per && j < tsz; ++j)
       if (tab[j] != kNone) out[base + p++] = tab[j];
     if (tid == 0) uniq[a2] = tot;
@@ -1688,10 +1690,10 @@ void dedup_medium(uint32_t n_medium, const uint32_t *medium, const uint64_t *seg
   if (!n_medium) return;
   constexpr uint32_t kSplit = kDedupBlockCap / 4;  // sizes up to here take the small tables
   const unsigned small_blocks = n_medium > 8192 ? 8192 : n_medium, big_blocks = n_medium > 2048 ? 2048 : n_medium;
-  hipLaunchKernelGGL(k_dedup_medium<2 * kSplit>, dim3(small_blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off,
-                     bucket, out, uniq, 0u, kSplit);
+  hipLaunchKernelGGL((k_dedup_medium<2 * kSplit, kBlock>), dim3(small_blocks), dim3(kBlock), 0, s, n_medium, medium,
+                     seg_off, bucket, out, uniq, 0u, kSplit);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dedup_medium<kBlockTab>, dim3(big_blocks), dim3(kBlock), 0, s, n_medium, medium, seg_off,
+  hipLaunchKernelGGL((k_dedup_medium<kBlockTab, 1024>), dim3(big_blocks), dim3(1024), 0, s, n_medium, medium, seg_off,
                      bucket, out, uniq, kSplit + 1, kDedupBlockCap);
   KC_LAUNCH_CHECK();
 }
