@@ -824,11 +824,12 @@ struct ClassStore {
   }
 };
 
-// Small buckets, four consecutive ones per wave step.  When all four hold at
-// most 16 elements (most buckets), each takes 16 lanes, one element per lane,
-// and a lane keeps its element when no lower lane of its group holds the same
-// value: no table, no LDS.  Otherwise the wave takes the four in turn, each
-// against an open-addressing table in LDS (medium and huge ones are skipped).
+// Small buckets, four consecutive ones per wave step.  Those of at most 16
+// elements (most buckets) take 16 lanes each, one element per lane, and a
+// lane keeps its element when no lower lane of its group holds the same
+// value: no table, no LDS.  The wave then takes the others of the four in
+// turn, each against an open-addressing table in LDS (medium and huge ones
+// are skipped).
 constexpr uint32_t kTinyBucket = 16;
 
 __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const uint64_t *seg_off,
@@ -846,21 +847,22 @@ __global__ __launch_bounds__(kBlock) void k_dedup_small(uint64_t n_next, const u
       gbase = seg_off[ag];
       gs = static_cast<uint32_t>(seg_off[ag + 1] - gbase);
     }
-    if (__ballot(gs > kTinyBucket) == 0) {
-      const uint32_t v = gl < gs ? bucket[gbase + gl] : kNone;
-      bool keep = gl < gs;
+    const bool tiny = gs <= kTinyBucket;
+    const uint64_t big = __ballot(!tiny);
+    if (big != ~0ull) {  // the tiny buckets of the four, by their lane groups
+      const uint32_t v = tiny && gl < gs ? bucket[gbase + gl] : kNone;
+      bool keep = tiny && gl < gs;
       for (uint32_t j = 0; j < kTinyBucket; ++j) {
         const uint32_t u = static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(grp * kTinyBucket + j)));
         if (j < gl && u == v) keep = false;
       }
       const uint32_t gm = static_cast<uint32_t>((__ballot(keep) >> (grp * kTinyBucket)) & 0xFFFFu);
       if (keep) out[gbase + __popc(gm & ((1u << gl) - 1u))] = v;
-      if (gl == 0 && ag < n_next) uniq[ag] = __popc(gm);
-      continue;
+      if (tiny && gl == 0 && ag < n_next) uniq[ag] = __popc(gm);
     }
-    for (uint32_t k = 0; k < 4; ++k) {
+    for (uint32_t k = 0; k < 4; ++k) {  // the others in turn, by the whole wave
+      if (!((big >> (k * kTinyBucket)) & 1ull)) continue;
       const uint64_t a2 = 4 * q + k;
-      if (a2 >= n_next) break;
       const uint64_t base = __shfl(gbase, static_cast<int>(k * kTinyBucket));
       const uint32_t s = static_cast<uint32_t>(__shfl(static_cast<int>(gs), static_cast<int>(k * kTinyBucket)));
       if (s > kDedupWaveCap) continue;  // listed as medium or huge
