@@ -754,15 +754,45 @@ __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active,
   for (uint32_t i = gl; i < d; i += W) a.bucket[pos + i] = a.relab[e0 + i];
 }
 
+// Rows of the one-lane bin (16 slots or fewer with the default bins): each
+// lane reads one row's bounds, then the wave copies its 64 rows in 16 steps
+// of four rows, 16 lanes to a row, all loads issued before the stores, so
+// that the copies are contiguous per row instead of one slot per lane.  A
+// wave holding a longer row (larger one-lane bins) copies lane by lane.
+__device__ __forceinline__ void scatter_small_rows(const ScatterArgs &a, bool active, uint32_t v) {
+  constexpr int kSteps = kWave / 4;
+  const uint32_t ln = lane_id(), grp = ln / 16, gl = ln % 16;
+  const uint64_t e0 = active ? a.off[v] : 0;
+  const uint32_t d = active ? a.deg_inter[v] : 0;
+  if (__ballot(d > 16) != 0) {
+    scatter_group<1>(a, active, v, e0, 0, 0, d, 0);
+    return;
+  }
+  const uint64_t pos = d ? static_cast<uint64_t>(a.pos_of[v]) : 0;
+  uint32_t val[kSteps];
+#pragma unroll
+  for (int p = 0; p < kSteps; ++p) {
+    const int src = p * 4 + static_cast<int>(grp);
+    const uint32_t dp = static_cast<uint32_t>(__shfl(static_cast<int>(d), src));
+    const uint64_t ep = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(e0), src));
+    val[p] = gl < dp ? a.relab[ep + gl] : 0u;
+  }
+#pragma unroll
+  for (int p = 0; p < kSteps; ++p) {
+    const int src = p * 4 + static_cast<int>(grp);
+    const uint32_t dp = static_cast<uint32_t>(__shfl(static_cast<int>(d), src));
+    const uint64_t pp = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), src));
+    if (gl < dp) a.bucket[pp + gl] = val[p];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter(ScatterArgs a) {
   const uint32_t blk = blockIdx.x;
   const int tid = threadIdx.x;
   if (blk < a.b.bS) {
     const uint32_t i = blk * kBlock + tid;
     const bool act = i < a.b.nT;
-    uint32_t v = act ? a.b.listT[i] : 0;
-    uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    scatter_group<1>(a, act, v, e0, e1, 0, act ? a.deg_inter[v] : 0, 0);
+    scatter_small_rows(a, act, act ? a.b.listT[i] : 0);
   } else if (blk < a.b.bW) {
     const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
     const bool act = i < a.b.nS;
