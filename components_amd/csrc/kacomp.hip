@@ -299,6 +299,7 @@ constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average
 constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
 constexpr uint64_t kHugeMapBytes = 4ull << 30;     // byte maps of the largest huge buckets
 constexpr uint64_t kMapMinShare = 32;              // byte maps from C / 32 elements on
+constexpr uint64_t kHubPullMin = 65536;            // smallest degree of a hub that pulls round 0
 constexpr uint32_t kMinWaveBin = 256;
 
 
@@ -598,7 +599,12 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       const uint32_t f = flag_base + i;
       const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
       slots[i] = g->timer.start(KC_K_ROUND);
-      push_round(off, col, key, kOneHop, b, t + i, &g->flags[f], prev, g->stream);
+      // Round 0 pushes from about a tenth of the rows; a hub gets about a
+      // tenth of its degree in pushes, serialised on its key.  Once that
+      // exceeds what the round's other pushes take (past about 1/512 of the
+      // level's edge slots), the largest hubs pull first.
+      const uint64_t hub_pull_deg = t + i == 0 ? std::max<uint64_t>(kHubPullMin, m2 / 512) : 0;
+      push_round(off, col, key, kOneHop, b, t + i, &g->flags[f], prev, hub_pull_deg, g->stream);
       g->timer.stop(slots[i], 0);
       if (first_slot < 0) first_slot = slots[i];
     }

@@ -88,8 +88,10 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
 // Round t of the time-stepped schedule: vertices at key distance t push
 // key + hop; *pending = 1 while a vertex is still above distance t;
 // prev_pending (may be null) reading 0 makes the launch return at once.
+// hub_pull_deg > 0: the hubs of at least that degree first pull round t's
+// candidates, so that the pushes find them settled.
 void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
-                int *pending, const int *prev_pending, hipStream_t s);
+                int *pending, const int *prev_pending, uint64_t hub_pull_deg, hipStream_t s);
 // deg_inter of hubs from their chunk counts; hub_pos[c] = offset of chunk c in its hub's run
 // (prefix: b.nH + 1 words; scratch: scan scratch).
 void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, uint64_t *prefix,

@@ -342,6 +342,31 @@ __device__ __forceinline__ bool round_group(const SweepArgs &a, bool active, uin
   return active && dist > t;
 }
 
+// Round t's candidates for the largest hubs (degree >= min_deg), pulled
+// before the round's pushes: one wave per hub chunk takes the best key + hop
+// over the chunk's neighbours at distance t (final in round t) and folds it
+// into the hub's key with one atomic.  The pushes that follow read the
+// settled key and skip the hub; otherwise thousands of them would each take
+// an atomic on that one key, which the device serialises.
+__global__ __launch_bounds__(kBlock) void k_hub_pull(SweepArgs a, uint32_t t, uint64_t min_deg) {
+  if (a.prev_flag && *a.prev_flag == 0) return;
+  const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) / kWave;  // one wave per chunk
+  if (i >= a.b.nH) return;
+  const HubChunk h = a.b.hubs[i];
+  if (a.off[h.v + 1] - a.off[h.v] < min_deg) return;
+  const uint64_t kv = a.kin[h.v];
+  if (static_cast<uint32_t>(kv >> 32) <= t) return;  // final already
+  uint64_t best = ~0ull;
+  for (uint64_t e = h.e0 + lane_id(); e < h.e1; e += kWave) {
+    const uint64_t ku = a.kin[a.col[e]];
+    if (static_cast<uint32_t>(ku >> 32) == t) best = min(best, ku + a.hop);
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    best = min(best, static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(best), o)));
+  if (lane_id() == 0 && best < kv)
+    atomicMin(reinterpret_cast<unsigned long long *>(a.kout + h.v), static_cast<unsigned long long>(best));
+}
+
 __global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
   if (a.prev_flag && *a.prev_flag == 0) return;
   const uint32_t blk = blockIdx.x;
@@ -1580,9 +1605,14 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
 }
 
 void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
-                int *pending, const int *prev_pending, hipStream_t s) {
+                int *pending, const int *prev_pending, uint64_t hub_pull_deg, hipStream_t s) {
   if (b.blocks == 0) return;
   SweepArgs a{off, col, key, key, hop, b, nullptr, nullptr, pending, prev_pending};
+  if (hub_pull_deg && b.nH) {
+    hipLaunchKernelGGL(k_hub_pull, dim3(static_cast<unsigned>(ceil_div(static_cast<uint64_t>(b.nH) * kWave, kBlock))),
+                       dim3(kBlock), 0, s, a, t, hub_pull_deg);
+    KC_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_round, dim3(b.blocks), dim3(kBlock), 0, s, a, t);
   KC_LAUNCH_CHECK();
 }
