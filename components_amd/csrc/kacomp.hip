@@ -219,6 +219,7 @@ struct kc_graph {
   uint64_t hubs_cap = 0;
   uint32_t *slots = nullptr, *seg_of_cid = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
   uint64_t *hub_prefix = nullptr;  // exclusive prefix of the hub chunks' counts
+  uint64_t *row_cstart = nullptr;  // chunk plan of the listed buckets' rows (column gather)
   uint32_t *crank = nullptr, *cid = nullptr;  // dense cluster ids: rank by label, id per vertex
   uint32_t *sk = nullptr, *sv = nullptr, *sk2 = nullptr, *sv2 = nullptr;  // (label, vertex) sort buffers
   uint64_t *seg_off = nullptr, *sort_scratch = nullptr;
@@ -890,7 +891,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     {
       // unique runs read, mapped to next-level indices, written
       TimedScope ts(g->timer, KC_K_CSR, 4 * m2_next + 4 * m2_next + 4 * m2_next);
-      gather_uniques(m2_next, n_next, off_next, g->seg_off, g->slots, g->seg_of_cid, col_next, g->stream);
+      gather_uniques(n_next, off_next, g->seg_off, g->slots, g->seg_of_cid, g->dl_medium, n_medium, g->dl_huge, n_huge,
+                     g->row_cstart, g->scan_scratch, col_next, g->stream);
     }
     n_l = n_next;
     m2_l = m2_next;
@@ -1042,6 +1044,7 @@ void init_work(kc_graph *g) {
   }
   g->dl_medium = g->dalloc<uint32_t>(n);
   g->dl_huge = g->dalloc<uint32_t>(n);
+  g->row_cstart = g->dalloc<uint64_t>(n + 2);
   g->hcount = g->dalloc<uint32_t>(n);
   g->flags = g->dalloc<int>(kMaxFlags);
   // level 0 without the degree-0 vertices (rep_a holds the id map meanwhile)
@@ -1114,15 +1117,18 @@ void build_device_csr(kc_graph *g, uint64_t n, const uint32_t *src, const uint32
   classify_buckets(n, nullptr, off_tmp.as<uint64_t>(), medium, huge, g->sizes + 3, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 3, g->sizes + 3, 16, hipMemcpyDeviceToHost, s));
   KC_HIP(hipStreamSynchronize(s));
+  const uint64_t n_medium = g->h_sizes[3], n_huge = g->h_sizes[4];
   dedup_buckets(g, n, off_tmp.as<uint64_t>(), bucket.as<uint32_t>(), runs.as<uint32_t>(), uniq.as<uint32_t>(), medium,
-                huge, g->h_sizes[3], g->h_sizes[4], P, n, scratch.as<uint64_t>());
+                huge, n_medium, n_huge, P, n, scratch.as<uint64_t>());
   g->off0 = g->dalloc<uint64_t>(n + 1);
   csr_offsets(n, uniq.as<uint32_t>(), g->off0, g->sizes, scratch.as<uint64_t>(), s);
   KC_HIP(hipMemcpyAsync(g->h_sizes + 2, g->sizes + 2, 8, hipMemcpyDeviceToHost, s));
   KC_HIP(hipStreamSynchronize(s));
   g->m2 = g->h_sizes[2];
   g->col0 = g->dalloc<uint32_t>(g->m2);
-  gather_uniques(g->m2, n, g->off0, off_tmp.as<uint64_t>(), runs.as<uint32_t>(), nullptr, g->col0, s);
+  DevTemp cstart((n_medium + n_huge + 2) * 8);
+  gather_uniques(n, g->off0, off_tmp.as<uint64_t>(), runs.as<uint32_t>(), nullptr, medium, n_medium, huge, n_huge,
+                 cstart.as<uint64_t>(), scratch.as<uint64_t>(), g->col0, s);
   unsigned long long *cnt = reinterpret_cast<unsigned long long *>(g->sizes + 6);
   fill_u64(g->sizes + 6, 1, 0, s);
   nonzero_rows(n, g->off0, cnt, s);

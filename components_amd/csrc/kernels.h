@@ -186,8 +186,11 @@ void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint
                  uint64_t *scratch, hipStream_t s);
 // col_next[off_next[a2] + i] = map[vals[seg_off[a2] + i]] for i < uniq[a2]
 // (map null: the value itself): the unique runs of the dedup, concatenated.
-void gather_uniques(uint64_t m2_next, uint64_t n_next, const uint64_t *off_next, const uint64_t *seg_off,
-                    const uint32_t *vals, const uint32_t *map, uint32_t *col_next, hipStream_t s);
+// medium/huge: the listed buckets (classify_buckets); cstart: n_medium +
+// n_huge + 1 words for their chunk plan; scratch: scan scratch.
+void gather_uniques(uint64_t n_next, const uint64_t *off_next, const uint64_t *seg_off, const uint32_t *vals,
+                    const uint32_t *map, const uint32_t *medium, uint64_t n_medium, const uint32_t *huge,
+                    uint64_t n_huge, uint64_t *cstart, uint64_t *scratch, uint32_t *col_next, hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
 void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, uint64_t *total, uint64_t *scratch,
                                hipStream_t s);

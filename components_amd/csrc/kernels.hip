@@ -1318,33 +1318,67 @@ struct OffStore {
     if (i + 1 == n) off[n] = prefix + v;
   }
 };
-// Next-level columns: entry k of the CSR copies the (k - off[a2])-th unique
-// value of bucket a2 (the row holding k), mapped to its next-level index.
-// Each wave covers kGatherSteps * 64 consecutive entries, coalesced: a lane
-// finds the row of its first entry by binary search over the offsets, then
-// walks forward to the rows of its later entries (64 apart).
-constexpr int kGatherSteps = 8;
+// Next-level columns: row a2 of the CSR copies the unique run of bucket a2,
+// mapped to next-level indices.  Rows of small buckets (at most the wave cap
+// of elements, so runs that short): 16 lanes to a row, four rows per wave
+// step, every row's copy contiguous.  The listed (medium and huge) buckets'
+// rows go in chunks of kRowChunk entries over the whole grid.
+constexpr uint64_t kRowChunk = 4096;
 
-__global__ __launch_bounds__(kBlock) void k_gather_uniques(uint64_t m2_next, uint64_t n_next, const uint64_t *off_next,
-                                                          const uint64_t *seg_off, const uint32_t *uniq_vals,
-                                                          const uint32_t *map, uint32_t *col_next) {
-  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x) / kWave;
-  const uint64_t k0 = wave * (kGatherSteps * kWave) + lane_id();
-  if (k0 >= m2_next) return;
-  uint64_t lo = 0, hi = n_next;  // off_next[lo] <= k0 < off_next[hi]
-  while (hi - lo > 1) {
-    const uint64_t mid = (lo + hi) / 2;
-    if (off_next[mid] <= k0)
-      lo = mid;
-    else
-      hi = mid;
+__global__ __launch_bounds__(kBlock) void k_gather_rows(uint64_t n_next, const uint64_t *off_next,
+                                                       const uint64_t *seg_off, const uint32_t *vals,
+                                                       const uint32_t *map, uint32_t *col_next) {
+  const uint32_t ln = lane_id(), grp = ln / 16, gl = ln % 16;
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; 4 * q < n_next;
+       q += waves) {
+    const uint64_t a2 = 4 * q + grp;
+    if (a2 >= n_next) continue;
+    const uint64_t s0 = seg_off[a2];
+    if (seg_off[a2 + 1] - s0 > kDedupWaveCap) continue;  // listed: copied in chunks
+    const uint64_t o0 = off_next[a2], len = off_next[a2 + 1] - o0;
+    for (uint64_t j = gl; j < len; j += 16) {
+      const uint32_t v = vals[s0 + j];
+      col_next[o0 + j] = map ? map[v] : v;
+    }
   }
-  for (int i = 0; i < kGatherSteps; ++i) {
-    const uint64_t k = k0 + static_cast<uint64_t>(i) * kWave;
-    if (k >= m2_next) break;
-    while (off_next[lo + 1] <= k) ++lo;
-    const uint32_t v = uniq_vals[seg_off[lo] + (k - off_next[lo])];
-    col_next[k] = map ? map[v] : v;
+}
+
+struct ListedRow {
+  const uint32_t *medium, *huge;
+  uint64_t n_medium;
+  __device__ uint32_t operator()(uint64_t h) const { return h < n_medium ? medium[h] : huge[h - n_medium]; }
+};
+struct ListedChunks {
+  ListedRow row;
+  const uint64_t *off_next;
+  __device__ uint64_t operator()(uint64_t h) const {
+    const uint32_t a2 = row(h);
+    return (off_next[a2 + 1] - off_next[a2] + kRowChunk - 1) / kRowChunk;
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void k_gather_listed(uint32_t n_list, ListedRow row, const uint64_t *cstart,
+                                                         const uint64_t *total, const uint64_t *off_next,
+                                                         const uint64_t *seg_off, const uint32_t *vals,
+                                                         const uint32_t *map, uint32_t *col_next) {
+  const uint64_t T = *total;
+  for (uint64_t c = blockIdx.x; c < T; c += gridDim.x) {
+    uint32_t lo = 0, hi = n_list;  // cstart[lo] <= c < cstart[hi] (cstart[n_list] = T)
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (cstart[mid] <= c)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t a2 = row(lo);
+    const uint64_t o0 = off_next[a2], len = off_next[a2 + 1] - o0, s0 = seg_off[a2];
+    const uint64_t j0 = (c - cstart[lo]) * kRowChunk, j1 = min(len, j0 + kRowChunk);
+    for (uint64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
+      const uint32_t v = vals[s0 + j];
+      col_next[o0 + j] = map ? map[v] : v;
+    }
   }
 }
 
@@ -1825,12 +1859,22 @@ void csr_offsets(uint64_t n_next, const uint32_t *uniq, uint64_t *off_next, uint
   KC_LAUNCH_CHECK();
 }
 
-void gather_uniques(uint64_t m2_next, uint64_t n_next, const uint64_t *off_next, const uint64_t *seg_off,
-                    const uint32_t *vals, const uint32_t *map, uint32_t *col_next, hipStream_t s) {
-  if (!m2_next) return;
-  const uint64_t blocks = ceil_div(m2_next, static_cast<uint64_t>(kBlock) * kGatherSteps);
-  hipLaunchKernelGGL(k_gather_uniques, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, m2_next, n_next,
-                     off_next, seg_off, vals, map, col_next);
+void gather_uniques(uint64_t n_next, const uint64_t *off_next, const uint64_t *seg_off, const uint32_t *vals,
+                    const uint32_t *map, const uint32_t *medium, uint64_t n_medium, const uint32_t *huge,
+                    uint64_t n_huge, uint64_t *cstart, uint64_t *scratch, uint32_t *col_next, hipStream_t s) {
+  if (!n_next) return;
+  const uint64_t waves = ceil_div(n_next, 4);
+  const uint64_t blocks = std::min<uint64_t>(ceil_div(waves, kBlock / kWave), 8192);
+  hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n_next, off_next, seg_off,
+                     vals, map, col_next);
+  KC_LAUNCH_CHECK();
+  const uint64_t n_list = n_medium + n_huge;
+  if (!n_list) return;
+  const ListedRow row{medium, huge, n_medium};
+  scan_exclusive<uint64_t>(n_list, ListedChunks{row, off_next}, ArrayStore<uint64_t>{cstart}, cstart + n_list, scratch,
+                           s);
+  hipLaunchKernelGGL(k_gather_listed, dim3(8192), dim3(kBlock), 0, s, static_cast<uint32_t>(n_list), row, cstart,
+                     cstart + n_list, off_next, seg_off, vals, map, col_next);
   KC_LAUNCH_CHECK();
 }
 
