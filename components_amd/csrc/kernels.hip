@@ -1220,13 +1220,19 @@ __global__ __launch_bounds__(kMarkBlock) void k_dedup_huge_mark(uint32_t n_huge,
     __syncthreads();  // the previous chunk's filter probes are done
     for (int j = threadIdx.x; j < kMarkFilter; j += kMarkBlock) filt[j] = kNone;
     __syncthreads();
-    for (uint64_t j = e_lo + threadIdx.x; j < e_hi; j += kMarkBlock) {
-      const uint32_t v = bucket[base + j];
-      if (v >= space) continue;  // cluster ids lie below the label space
-      const uint32_t f = hash32(v) & (kMarkFilter - 1);
-      if (filt[f] != v) {
-        filt[f] = v;
-        bm[v] = 1;
+    for (uint64_t jb = e_lo + threadIdx.x; jb < e_hi; jb += 4 * kMarkBlock) {  // four loads in flight per lane
+      uint32_t vs[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) vs[u] = jb + u * kMarkBlock < e_hi ? bucket[base + jb + u * kMarkBlock] : kNone;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t v = vs[u];
+        if (v >= space) continue;  // past the chunk, or (never) outside the label space
+        const uint32_t f = hash32(v) & (kMarkFilter - 1);
+        if (filt[f] != v) {
+          filt[f] = v;
+          bm[v] = 1;
+        }
       }
     }
   }
@@ -1375,9 +1381,13 @@ __global__ __launch_bounds__(kBlock) void k_gather_listed(uint32_t n_list, Liste
     const uint32_t a2 = row(lo);
     const uint64_t o0 = off_next[a2], len = off_next[a2 + 1] - o0, s0 = seg_off[a2];
     const uint64_t j0 = (c - cstart[lo]) * kRowChunk, j1 = min(len, j0 + kRowChunk);
-    for (uint64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
-      const uint32_t v = vals[s0 + j];
-      col_next[o0 + j] = map ? map[v] : v;
+    for (uint64_t jb = j0 + threadIdx.x; jb < j1; jb += 4 * kBlock) {  // four loads in flight per lane
+      uint32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = jb + u * kBlock < j1 ? vals[s0 + jb + u * kBlock] : 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (jb + u * kBlock < j1) col_next[o0 + jb + u * kBlock] = map ? map[v[u]] : v[u];
     }
   }
 }
