@@ -1118,26 +1118,44 @@ __global__ __launch_bounds__(kBlock) void k_dedup_huge_bits(uint32_t n_huge, con
       const uint64_t p1 = min(e_hi, p0 + kStagePiece);
       if (threadIdx.x == 0) n_kept = 0;
       __syncthreads();
-      for (uint64_t j = p0 + threadIdx.x; j < p1; j += kBlock) {
-        const uint32_t v = bucket[base + j];
-        bool keep;
-        if ((v >> 5) >= words) {  // outside the label space: kept unchecked (never the case)
-          keep = true;
-        } else {
+      // four elements per lane at a time: their loads, bit reads and
+      // atomics each in flight together (the filter is probed in order)
+      for (uint64_t jb = p0 + threadIdx.x; jb < p1; jb += 4 * kBlock) {
+        uint32_t vs[4], bit[4];
+        uint32_t *w[4];
+        bool keep[4], probe[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vs[u] = jb + u * kBlock < p1 ? bucket[base + jb + u * kBlock] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t v = vs[u];
+          keep[u] = probe[u] = false;
+          bit[u] = 1u << (v & 31u);
+          w[u] = &bm[(v >> 5) < words ? (v >> 5) : 0];
+          if (jb + u * kBlock >= p1) continue;
+          if ((v >> 5) >= words) {  // outside the label space: kept unchecked (never the case)
+            keep[u] = true;
+            continue;
+          }
           const uint32_t f = hash32(v) & (kHugeFilter - 1);
-          keep = filt[f] != v;
-          if (keep) {
+          if (filt[f] != v) {
             filt[f] = v;
-            const uint32_t bit = 1u << (v & 31u);
-            uint32_t *w = &bm[v >> 5];
-            keep = (*w & bit) == 0 && (atomicOr(w, bit) & bit) == 0;
+            probe[u] = true;
           }
         }
-        const uint64_t m = __ballot(keep);
-        uint32_t wbase = 0;
-        if (lane_id() == 0 && m) wbase = atomicAdd(&n_kept, static_cast<uint32_t>(__popcll(m)));
-        wbase = __shfl(wbase, 0);
-        if (keep) stage[wbase + __popcll(m & lanes_below())] = v;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) probe[u] = probe[u] && (*w[u] & bit[u]) == 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (probe[u]) keep[u] = (atomicOr(w[u], bit[u]) & bit[u]) == 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint64_t m = __ballot(keep[u]);
+          uint32_t wbase = 0;
+          if (lane_id() == 0 && m) wbase = atomicAdd(&n_kept, static_cast<uint32_t>(__popcll(m)));
+          wbase = __shfl(wbase, 0);
+          if (keep[u]) stage[wbase + __popcll(m & lanes_below())] = vs[u];
+        }
       }
       __syncthreads();
       const uint32_t k = n_kept;
