@@ -842,8 +842,9 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     fill_u32(rec.newidx, n_l, kNone, g->stream);
     {
       TimedScope ts(g->timer, KC_K_SCAN, 2 * (8 * N + 4 * N) + 4 * N + 20 * n_l / 8);
+      // the sort's other buffer pair is free: it caches the gathered counts
       segments(N, sk, sv, g->deg_inter, gid_l, g->crank, g->pos_of, rec.newidx, g->seg_of_cid, gid_next, g->seg_off,
-               g->sizes, g->scan_scratch, g->stream);
+               g->sizes, which ? g->sk : g->sk2, g->scan_scratch, g->stream);
     }
     // medium and huge buckets listed before the sizes come back (the
     // clusters bound n_next)

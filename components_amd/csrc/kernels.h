@@ -118,10 +118,10 @@ void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, ui
 // newidx[a] = next-level index of label a (caller pre-fills kNone),
 // seg_of_cid[rank[a]] = the same index by dense cluster id, gid_next by
 // next-level index, seg_off[0..n_next] bucket offsets; sizes[0] = n_next,
-// sizes[1] = P (cluster edges).
+// sizes[1] = P (cluster edges).  cache: N words of scratch.
 void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t *deg_inter, const uint32_t *gid,
               const uint32_t *rank, uint32_t *pos_of, uint32_t *newidx, uint32_t *seg_of_cid, uint32_t *gid_next,
-              uint64_t *seg_off, uint64_t *sizes, uint64_t *scratch, hipStream_t s);
+              uint64_t *seg_off, uint64_t *sizes, uint32_t *cache, uint64_t *scratch, hipStream_t s);
 
 // The labels lab[col[e]] of row v that differ from lab[v] (after a best-effort
 // LDS filter of repeats), packed at relab[off[v] ...] (hub chunks: at the

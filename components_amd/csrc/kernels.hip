@@ -716,12 +716,26 @@ struct SelectStore {
   }
 };
 
+// (segment head, cluster-edge count) of item i.  The first pass of the scan
+// gathers the count by vertex and keeps it in `cache` in item order; the
+// second pass reads it from there instead of gathering again.
 struct SegLoad {
   const uint32_t *sk, *sv, *deg_inter;
+  uint32_t *cache;
   __device__ Pair64 operator()(uint64_t i) const {
     const uint32_t a = sk[i];
     const bool head = i == 0 || sk[i - 1] != a;
-    return {head ? 1ull : 0ull, deg_inter[sv[i]]};
+    const uint32_t d = deg_inter[sv[i]];
+    cache[i] = d;
+    return {head ? 1ull : 0ull, d};
+  }
+};
+struct SegLoadCached {
+  const uint32_t *sk, *cache;
+  __device__ Pair64 operator()(uint64_t i) const {
+    const uint32_t a = sk[i];
+    const bool head = i == 0 || sk[i - 1] != a;
+    return {head ? 1ull : 0ull, cache[i]};
   }
 };
 struct SegStore {
@@ -1756,11 +1770,25 @@ void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, ui
 
 void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t *deg_inter, const uint32_t *gid,
               const uint32_t *rank, uint32_t *pos_of, uint32_t *newidx, uint32_t *seg_of_cid, uint32_t *gid_next,
-              uint64_t *seg_off, uint64_t *sizes, uint64_t *scratch, hipStream_t s) {
+              uint64_t *seg_off, uint64_t *sizes, uint32_t *cache, uint64_t *scratch, hipStream_t s) {
   Pair64 *total = reinterpret_cast<Pair64 *>(scratch);
-  scan_exclusive<Pair64>(N, SegLoad{sk, sv, deg_inter},
-                         SegStore{sk, sv, gid, rank, pos_of, newidx, seg_of_cid, gid_next, seg_off}, total, scratch + 4,
-                         s);
+  const SegStore store{sk, sv, gid, rank, pos_of, newidx, seg_of_cid, gid_next, seg_off};
+  if (ceil_div(N, kScanTile) <= 1) {  // one pass: nothing to cache
+    scan_exclusive<Pair64>(N, SegLoad{sk, sv, deg_inter, cache}, store, total, scratch + 4, s);
+  } else {
+    const uint64_t blocks = ceil_div(N, kScanTile);
+    Pair64 *sums = reinterpret_cast<Pair64 *>(scratch + 4);
+    Pair64 *prefix = sums + blocks;
+    uint64_t *rest = reinterpret_cast<uint64_t *>(prefix + blocks) + 4;
+    hipLaunchKernelGGL((k_scan_reduce<Pair64, SegLoad>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, N,
+                       SegLoad{sk, sv, deg_inter, cache}, sums);
+    KC_LAUNCH_CHECK();
+    scan_exclusive<Pair64>(blocks, ArrayLoad<Pair64>{sums}, ArrayStore<Pair64>{prefix}, total, rest, s);
+    hipLaunchKernelGGL((k_scan_down<Pair64, SegLoadCached, SegStore>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
+                       0, s, N, SegLoadCached{sk, cache}, store, static_cast<const Pair64 *>(prefix),
+                       static_cast<Pair64 *>(nullptr));
+    KC_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_seg_finish, dim3(1), dim3(1), 0, s, total, seg_off, sizes);
   KC_LAUNCH_CHECK();
 }
