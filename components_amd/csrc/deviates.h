@@ -75,17 +75,22 @@ KC_HD inline double canonical_draw(uint32_t engine_seed, bool fused) {
   return u;
 }
 
-// Count of table entries >= w (the table is non-increasing).
+// Count of table entries >= w (the table is non-increasing).  The entries
+// fall geometrically (entry k - 1 is about table[0]^k), so the count is about
+// log(w) / log(table[0]); the search starts there and steps to the exact
+// boundary, which takes a probe or two instead of a binary search's
+// log2(k_max) dependent loads.
 KC_HD inline uint32_t deviate_from_table(double w, const double *table, uint32_t k_max) {
-  uint32_t lo = 0, hi = k_max;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (table[mid] >= w)
-      lo = mid + 1;
-    else
-      hi = mid;
+  if (k_max == 0) return 0;
+  const double t0 = table[0];
+  uint32_t k = 0;
+  if (w <= t0 && t0 < 1.0) {
+    const double g = log(w) / log(t0);
+    k = g >= static_cast<double>(k_max) ? k_max : static_cast<uint32_t>(g);
   }
-  return lo;
+  while (k > 0 && table[k - 1] < w) --k;
+  while (k < k_max && table[k] >= w) ++k;
+  return k;
 }
 
 // Hashed stream: u from a SplitMix64 finaliser over (seed, iteration, gid),

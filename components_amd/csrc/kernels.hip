@@ -1474,26 +1474,30 @@ __global__ __launch_bounds__(kBlock) void k_canon_min(uint64_t n, const uint32_t
   if (carry_key != kNone && lane_id() == 0 && carry_min < minid[carry_key]) atomicMin(&minid[carry_key], carry_min);
 }
 
-// Block count of the lanes with pred set, written to partial[blockIdx.x].
-__device__ __forceinline__ void block_count(bool pred, uint32_t *partial) {
+// Block sum of the threads' counts c, written to partial[blockIdx.x].
+__device__ __forceinline__ void block_sum(uint32_t c, uint32_t *partial) {
   __shared__ uint32_t wc[kBlock / kWave];
-  const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(pred)));
+  for (int o = 32; o > 0; o >>= 1) c += static_cast<uint32_t>(__shfl_xor(static_cast<int>(c), o));
   if (lane_id() == 0) wc[threadIdx.x / kWave] = c;
   __syncthreads();
   if (threadIdx.x == 0) partial[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
+// Grid-stride (a bounded grid keeps the partial counts few for the one-block
+// sum that follows).
+constexpr uint32_t kCountBlocks = 2048;
+
 __global__ __launch_bounds__(kBlock) void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid,
                                                        const uint32_t *minid, uint32_t *labels, uint32_t *partial) {
-  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  bool root = false;
-  if (v < n) {
+  uint32_t roots = 0;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
     const uint32_t id = gid ? gid[v] : static_cast<uint32_t>(v);
     const uint32_t l = minid[rep[v]];
     labels[id] = l;
-    root = l == id;
+    roots += l == id;
   }
-  block_count(root, partial);
+  block_sum(roots, partial);
 }
 
 // Level 0 without isolated vertices: newid[v] (kNone for degree 0), gid of
@@ -1532,14 +1536,14 @@ __global__ void k_identity(uint64_t n, uint32_t *p) {
 
 __global__ __launch_bounds__(kBlock) void k_labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels,
                                                             uint32_t *partial) {
-  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  bool root = false;
-  if (v < n) {
+  uint32_t roots = 0;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
     const uint32_t l = static_cast<uint32_t>(key[v]);
     labels[v] = l;
-    root = l == static_cast<uint32_t>(v);
+    roots += l == static_cast<uint32_t>(v);
   }
-  block_count(root, partial);
+  block_sum(roots, partial);
 }
 
 // out = sum of partial[0..count) (one block).
@@ -1957,9 +1961,10 @@ void canon_min(uint64_t n, const uint32_t *rep, const uint32_t *gid, uint32_t *m
 void canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid, const uint32_t *minid, uint32_t *labels,
                  uint32_t *roots, uint32_t *partial, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_canon_label, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rep, gid, minid, labels, partial);
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(grid_for(n), kCountBlocks));
+  hipLaunchKernelGGL(k_canon_label, dim3(g), dim3(kBlock), 0, s, n, rep, gid, minid, labels, partial);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(grid_for(n)), roots);
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(g), roots);
   KC_LAUNCH_CHECK();
 }
 
@@ -2089,9 +2094,10 @@ void identity_u32(uint64_t n, uint32_t *p, hipStream_t s) {
 void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, uint32_t *partial,
                       hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_labels_from_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, labels, partial);
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(grid_for(n), kCountBlocks));
+  hipLaunchKernelGGL(k_labels_from_keys, dim3(g), dim3(kBlock), 0, s, n, key, labels, partial);
   KC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(grid_for(n)), roots);
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(g), roots);
   KC_LAUNCH_CHECK();
 }
 
