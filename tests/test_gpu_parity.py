@@ -341,3 +341,39 @@ def test_degree_order_matches_input_order(stream, bfs):
             wkeys, wgid = run.level_keys(lvl)
             assert np.array_equal(keys, wkeys), (inorder, lvl)
             assert np.array_equal(gid, wgid), (inorder, lvl)
+
+
+def test_option_checks_and_timed_class():
+    g = ca.generate("rmat", n=1 << 16, m=1 << 20, seed=4)
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    dg = ca.DeviceGraph(g)
+    try:
+        for bad in (dict(dedup_huge=7), dict(dedup_huge=-1)):
+            with pytest.raises(ca.KcError):
+                dg.find_components(variant="exp", seed=1001, **bad)
+        # every launch timed, then only the relabel launches: same launches, time on one class only
+        lab_all, st_all = dg.find_components(variant="exp", seed=1001, use_bfs=True, time_kernels=True)
+        lab_one, st_one = dg.find_components(variant="exp", seed=1001, use_bfs=True, time_kernels=True,
+                                             time_class="relabel")
+        assert np.array_equal(lab_all, ref) and np.array_equal(lab_one, ref)
+        assert st_all["kernel_launches"] == st_one["kernel_launches"]
+        assert st_one["kernel_ms"]["relabel"] > 0
+        assert all(v == 0 for k, v in st_one["kernel_ms"].items() if k != "relabel")
+        assert sum(v > 0 for v in st_all["kernel_ms"].values()) > 1
+    finally:
+        dg.close()
+
+
+@pytest.mark.parametrize("mode", ["bitmaps", "bytemaps", "tables"])
+def test_huge_bucket_modes_on_a_hub_graph(mode):
+    # two hubs sharing most of their neighbours: their clusters' buckets are huge and full of repeats
+    rng = np.random.default_rng(11)
+    n = 1 << 17
+    src = np.concatenate([np.zeros(60000, np.int64), np.ones(60000, np.int64), rng.integers(2, n, 200000)])
+    dst = np.concatenate([rng.integers(2, n, 60000), rng.integers(2, n, 60000), rng.integers(2, n, 200000)])
+    g = ca.Graph.from_edges(n, np.stack([src, dst], axis=1))
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    for bfs in (False, True):
+        lab, _ = ca.find_components(g, variant="exp", seed=1001, seq_limit=0, use_bfs=bfs,
+                                    stream=ca.STREAM_HASHED, dedup_huge=mode)
+        assert np.array_equal(lab, ref), (mode, bfs)
