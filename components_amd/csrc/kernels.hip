@@ -616,30 +616,45 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(uint64_t n, Load load, T
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
+__device__ __forceinline__ uint64_t shfl_up_t(uint64_t x, int o) {
+  return static_cast<uint64_t>(__shfl_up(static_cast<unsigned long long>(x), o));
+}
+__device__ __forceinline__ Pair64 shfl_up_t(Pair64 x, int o) { return {shfl_up_t(x.a, o), shfl_up_t(x.b, o)}; }
+
+// A tile is kScanPerThread rounds of kBlock consecutive items (lane i of a
+// round takes item i, so loads and stores are coalesced); each round is one
+// block-wide scan (wave scans, then the wave sums through LDS), carried into
+// the next.
 template <class T, class Load, class Store>
 __global__ __launch_bounds__(kBlock) void k_scan_down(uint64_t n, Load load, Store store, const T *block_prefix,
                                                      T *total_out) {
-  __shared__ T smem[kBlock];
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + static_cast<uint64_t>(threadIdx.x) * kScanPerThread;
-  T vals[kScanPerThread];
-  T acc{};
-#pragma unroll
+  __shared__ T wsum[kBlock / kWave];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  const uint32_t ln = lane_id(), wv = threadIdx.x / kWave;
+  T run = block_prefix ? block_prefix[blockIdx.x] : T{};
   for (int j = 0; j < kScanPerThread; ++j) {
-    uint64_t i = base + j;
-    vals[j] = i < n ? load(i) : T{};
-    acc = acc + vals[j];
+    const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
+    const T v = i < n ? load(i) : T{};
+    T inc = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const T t = shfl_up_t(inc, o);
+      if (ln >= static_cast<uint32_t>(o)) inc = inc + t;
+    }
+    T ex = shfl_up_t(inc, 1);
+    if (ln == 0) ex = T{};
+    if (ln == kWave - 1) wsum[wv] = inc;
+    __syncthreads();
+    T before{}, round_total{};
+    for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+      const T x = wsum[w];
+      if (w < wv) before = before + x;
+      round_total = round_total + x;
+    }
+    __syncthreads();
+    if (i < n) store(i, run + before + ex, v);
+    run = run + round_total;
   }
-  T tot;
-  T run = block_exclusive_scan_shift<T>(acc, smem, &tot);
-  const T bp = block_prefix ? block_prefix[blockIdx.x] : T{};
-  run = bp + run;
-#pragma unroll
-  for (int j = 0; j < kScanPerThread; ++j) {
-    uint64_t i = base + j;
-    if (i < n) store(i, run, vals[j]);
-    run = run + vals[j];
-  }
-  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total_out = run;
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = run;
 }
 
 template <class T>
