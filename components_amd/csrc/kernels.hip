@@ -465,7 +465,7 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const Lab &lab
   }
 }
 
-constexpr int kFilterWords = 4096;  // 16 KB of LDS per block
+constexpr int kFilterWords = 2048;  // 8 KB of LDS per block
 
 // One 256-thread block's share of the relabel pass: section by block index.
 template <class Lab>
@@ -482,18 +482,18 @@ __device__ __forceinline__ void relabel_block(const SweepArgs &a, const Lab &lab
     const bool act = i < a.b.nS;
     uint32_t v = act ? a.b.listS[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    relabel_group<16>(a, lab, relab, act, v, e0, e1, tid & 15, false, 0, filt + (tid / 16) * 256, 255);
+    relabel_group<16>(a, lab, relab, act, v, e0, e1, tid & 15, false, 0, filt + (tid / 16) * 128, 127);
   } else if (blk < a.b.bH) {
     const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nW;
     uint32_t v = act ? a.b.listW[i] : 0;
     uint64_t e0 = act ? a.off[v] : 0, e1 = act ? a.off[v + 1] : 0;
-    relabel_group<kWave>(a, lab, relab, act, v, e0, e1, tid & 63, false, 0, filt + (tid / kWave) * 1024, 1023);
+    relabel_group<kWave>(a, lab, relab, act, v, e0, e1, tid & 63, false, 0, filt + (tid / kWave) * 512, 511);
   } else {
     const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
     const bool act = i < a.b.nH;
     HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
-    relabel_group<kWave>(a, lab, relab, act, h.v, h.e0, h.e1, tid & 63, true, i, filt + (tid / kWave) * 1024, 1023);
+    relabel_group<kWave>(a, lab, relab, act, h.v, h.e0, h.e1, tid & 63, true, i, filt + (tid / kWave) * 512, 511);
   }
 }
 
@@ -503,7 +503,7 @@ __device__ __forceinline__ void relabel_block(const SweepArgs &a, const Lab &lab
 // virtual 256-thread blocks of the binned layout in a grid stride, each
 // quarter with its own LDS filter (the quarters are whole waves and need no
 // barrier between them).
-constexpr int kRelabelHot = 16384;
+constexpr int kRelabelHot = 28672;
 constexpr int kRelabelQuarters = 4;
 
 __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(SweepArgs a, const uint32_t *lab,
