@@ -216,6 +216,12 @@ struct kc_graph {
   HubChunk *hubs = nullptr;
   uint32_t *flists = nullptr;  // bins of the unsettled vertices (time-stepped schedule)
   HubChunk *fhubs = nullptr;
+  // level 0's bins depend on its degrees alone: built by the first call, kept
+  uint32_t *l0_lists = nullptr;
+  HubChunk *l0_hubs = nullptr;
+  Bins l0_bins;
+  BinCfg l0_cfg{0, 0, 0};
+  const uint64_t *l0_off = nullptr;
   uint64_t hubs_cap = 0;
   uint32_t *slots = nullptr, *seg_of_cid = nullptr, *uniq = nullptr, *pos_of = nullptr, *hub_pos = nullptr;
   uint64_t *hub_prefix = nullptr;  // exclusive prefix of the hub chunks' counts
@@ -337,14 +343,24 @@ BinCfg bin_cfg(const kc_options *o) {
 
 // Bins of the level CSR (one sync to size the binned launches).  With key,
 // only the vertices at key distance >= tmin, into the second set of lists.
+// Level 0's full bins go to their own lists and are reused by later calls
+// with the same thresholds (no launch, no sync).
 Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg, const uint64_t *key = nullptr,
                uint32_t tmin = 0) {
+  const bool level0 = !key && n == g->n0 && (off == g->off0p || off == g->off0c);
+  if (level0 && g->l0_off == off && g->l0_cfg.tmax == cfg.tmax && g->l0_cfg.smax == cfg.smax &&
+      g->l0_cfg.wmax == cfg.wmax)
+    return g->l0_bins;
+  if (level0 && !g->l0_lists) {
+    g->l0_lists = g->dalloc<uint32_t>(3 * g->n);
+    g->l0_hubs = g->dalloc<HubChunk>(g->hubs_cap);
+  }
   Bins b;
-  uint32_t *lists = key ? g->flists : g->lists;
+  uint32_t *lists = key ? g->flists : level0 ? g->l0_lists : g->lists;
   b.listT = lists;
   b.listS = lists + g->n;
   b.listW = lists + 2 * g->n;
-  b.hubs = key ? g->fhubs : g->hubs;
+  b.hubs = key ? g->fhubs : level0 ? g->l0_hubs : g->hubs;
   {
     TimedScope ts(g->timer, KC_K_BIN, (key ? 24 : 16) * n + 8);
     bin_vertices(n, off, cfg, b.listT, b.listS, b.listW, b.hubs, g->small, key ? nullptr : g->deg_inter,
@@ -358,6 +374,11 @@ Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg, 
   b.nH = g->h_small[3];
   if (b.nH > g->hubs_cap) fail(KC_ERR_INVALID, "hub chunk overflow (%u > %llu)", b.nH, (unsigned long long)g->hubs_cap);
   finish_bins(b);
+  if (level0) {
+    g->l0_bins = b;
+    g->l0_cfg = cfg;
+    g->l0_off = off;
+  }
   return b;
 }
 

@@ -352,6 +352,8 @@ def test_option_checks_and_timed_class():
             with pytest.raises(ca.KcError):
                 dg.find_components(variant="exp", seed=1001, **bad)
         # every launch timed, then only the relabel launches: same launches, time on one class only
+        # (after a first call, which also builds the level-0 bins that later calls reuse)
+        dg.find_components(variant="exp", seed=1001, use_bfs=True)
         lab_all, st_all = dg.find_components(variant="exp", seed=1001, use_bfs=True, time_kernels=True)
         lab_one, st_one = dg.find_components(variant="exp", seed=1001, use_bfs=True, time_kernels=True,
                                              time_class="relabel")
