@@ -137,6 +137,10 @@ _SIGNATURES = {
                                    ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_uint32)]),
     "kc_read_metis": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
     "kc_write_metis": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
+    "kc_read_edgefile": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
+    "kc_read_partitioned_edgefile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P(Csr)]),
+    "kc_read_binary": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
+    "kc_write_binary": (ctypes.c_int, [ctypes.c_char_p, _P(Csr)]),
     "kc_gen_params_init": (None, [_P(GenParams)]),
     "kc_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(Csr)]),
     "kc_csr_free": (None, [_P(Csr)]),
@@ -273,6 +277,34 @@ def write_metis(path, g):
     tgt = np.ascontiguousarray(g.targets, np.uint32)
     c = Csr(g.n, g.m, _u64p(off), _u32p(tgt) if len(tgt) else _P(ctypes.c_uint32)())
     _check(lib().kc_write_metis(str(path).encode(), ctypes.byref(c)))
+
+
+def read_edgefile(path):
+    """Sorted edge file ("<type> n m" header, "<type> u v" lines, 1-based)."""
+    c = Csr()
+    _check(lib().kc_read_edgefile(str(path).encode(), ctypes.byref(c)))
+    return _take_csr(c)
+
+
+def read_partitioned_edgefile(path, parts=1):
+    """The edge files "<path>_0" .. "<path>_<parts-1>" as one graph."""
+    c = Csr()
+    _check(lib().kc_read_partitioned_edgefile(str(path).encode(), int(parts), ctypes.byref(c)))
+    return _take_csr(c)
+
+
+def read_binary(path):
+    """Sorted binary file (n, m as u64, then m 1-based (u, v) u64 pairs)."""
+    c = Csr()
+    _check(lib().kc_read_binary(str(path).encode(), ctypes.byref(c)))
+    return _take_csr(c)
+
+
+def write_binary(path, g):
+    off = np.ascontiguousarray(g.offsets, np.uint64)
+    tgt = np.ascontiguousarray(g.targets, np.uint32)
+    c = Csr(g.n, g.m, _u64p(off), _u32p(tgt) if len(tgt) else _P(ctypes.c_uint32)())
+    _check(lib().kc_write_binary(str(path).encode(), ctypes.byref(c)))
 
 
 STREAM_REFERENCE, STREAM_HASHED = 0, 1

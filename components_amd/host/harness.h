@@ -43,6 +43,10 @@ struct Config {
   uint64_t pes = 256;  // P of the relative sequential limit: one PE per compute unit
   bool check = false;  // compare with a host union-find after the last iteration
   bool hashed = false; // -stream hashed: SplitMix64 deviates instead of the reference draw
+  int input_parts = 1;  // -parts: files "<in>_0" .. of a partitioned input (one rank per file in the reference)
+  bool FromFile() const {
+    return input_type == "file" || input_type == "edgefile" || input_type == "partition" || input_type == "binary";
+  }
 };
 
 inline Config ParseParameters(int argc, char **argv) {
@@ -51,6 +55,7 @@ inline Config ParseParameters(int argc, char **argv) {
   c.seed = a.Get<int>("seed", 1);
   c.input_type = a.Get<std::string>("input", "null");
   c.input_file = a.Get<std::string>("in", "null");
+  c.input_parts = a.Get<int>("parts", 1);
   c.output_file = a.Get<std::string>("out", "out");
   c.debug_output_file = a.Get<std::string>("debug_out", "tmp");
   c.print_verbose = a.IsSet("verbose");
@@ -132,10 +137,9 @@ struct Statistics {
 // IOUtility::LoadGraph (lib/io/io_utils.h:33-57).
 inline int LoadGraph(const Config &c, kc_csr *g) {
   if (c.input_type == "file") return kc_read_metis(c.input_file.c_str(), g);
-  if (c.input_type == "edgefile" || c.input_type == "partition" || c.input_type == "binary") {
-    std::fprintf(stderr, "I/O type not supported: %s\n", c.input_type.c_str());
-    return KC_ERR_UNSUPPORTED;
-  }
+  if (c.input_type == "edgefile") return kc_read_edgefile(c.input_file.c_str(), g);
+  if (c.input_type == "partition") return kc_read_partitioned_edgefile(c.input_file.c_str(), c.input_parts, g);
+  if (c.input_type == "binary") return kc_read_binary(c.input_file.c_str(), g);
   if (c.gen != "null") {
     kc_gen_params p;
     kc_gen_params_init(&p);
@@ -200,7 +204,7 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner) {
   for (unsigned i = 0; i < c.iterations; ++i) {
     c.seed = initial_seed + static_cast<int>(i) + 1000;
     kc_csr g{};
-    if (c.input_type == "file" && have_cached) {
+    if (c.FromFile() && have_cached) {
       g = cached;
     } else {
       int rc = LoadGraph(c, &g);
@@ -208,7 +212,7 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner) {
         std::fprintf(stderr, "graph load failed: %s\n", kc_last_error());
         return 1;
       }
-      if (c.input_type == "file") {
+      if (c.FromFile()) {  // a file graph is the same in every iteration
         cached = g;
         have_cached = true;
       }
@@ -293,7 +297,7 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner) {
       std::printf("CHECK labels_bit_exact=%s components=%u\n", ok ? "yes" : "no", st.components);
     }
     kc_graph_destroy(dg);
-    if (c.input_type != "file") kc_csr_free(&g);
+    if (!c.FromFile()) kc_csr_free(&g);
   }
   if (have_cached) kc_csr_free(&cached);
   const double hbm = last_dev_ms > 0 ? last_alg_bytes / (last_dev_ms / 1e3) / 1e9 : 0.0;

@@ -27,6 +27,14 @@
  *       (lib/components/exponential_contraction.h:325-335); always 0 on one GPU.
  *   kc_read_metis / kc_write_metis
  *       GraphIO::ReadMETISFile (lib/io/graph_io.h:281-382, :1207-1248)
+ *   kc_read_edgefile
+ *       GraphIO::ReadSortedEdgeFile (lib/io/graph_io.h:385-669) with the
+ *       line parser ParseEdgeFilestream (:1250-1300)
+ *   kc_read_partitioned_edgefile
+ *       GraphIO::ReadPartitionedSortedEdgeFile (lib/io/graph_io.h:672-907)
+ *   kc_read_binary / kc_write_binary
+ *       GraphIO::ReadSortedBinaryFile (lib/io/graph_io.h:910-1175); the
+ *       writer produces that format (the reference has none)
  *   kc_generate
  *       IOUtility::GenerateSyntheticGraph (lib/io/io_utils.h:59-83); KaGen's
  *       generators are not available, so the generators are the project's own
@@ -379,6 +387,15 @@ int kc_graph_features(kc_graph *g, float *features);
 /* Host I/O and generators. */
 int kc_read_metis(const char *path, kc_csr *out);
 int kc_write_metis(const char *path, const kc_csr *g);
+/* Sorted edge file: '%' comment lines, a header "<type> n m", then lines
+ * "<type> u v" with 1-based ids, of which the first m are read. */
+int kc_read_edgefile(const char *path, kc_csr *out);
+/* The edge files "<path>_0" .. "<path>_<parts-1>" (same header in each,
+ * every edge line of every part read), as one graph. */
+int kc_read_partitioned_edgefile(const char *path, int parts, kc_csr *out);
+/* Sorted binary file: n and m as u64, then m (u, v) u64 pairs, 1-based. */
+int kc_read_binary(const char *path, kc_csr *out);
+int kc_write_binary(const char *path, const kc_csr *g);
 void kc_gen_params_init(kc_gen_params *p);
 int kc_generate(const char *name, const kc_gen_params *p, kc_csr *out);
 void kc_csr_free(kc_csr *g);

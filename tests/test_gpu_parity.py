@@ -247,6 +247,27 @@ def test_app_check_flag():
         assert r.returncode == 0 and "CHECK labels_bit_exact=yes" in r.stdout, app
 
 
+def test_app_reads_edge_and_binary_inputs(tmp_path):
+    g = ca.generate("gnm_undirected", n=3000, m=2500, seed=9)
+    src = np.repeat(np.arange(g.n), np.diff(g.offsets.astype(np.int64)))
+    keep = src < g.targets
+    u, v = src[keep] + 1, g.targets[keep].astype(np.int64) + 1
+    lines = [f"e {a} {b}" for a, b in zip(u.tolist(), v.tolist())]
+    (tmp_path / "g.edges").write_text(f"p {g.n} {len(u)}\n" + "\n".join(lines) + "\n")
+    half = len(lines) // 2
+    (tmp_path / "g.part_0").write_text(f"p {g.n} {len(u)}\n" + "\n".join(lines[:half]) + "\n")
+    (tmp_path / "g.part_1").write_text(f"p {g.n} {len(u)}\n" + "\n".join(lines[half:]) + "\n")
+    ca.write_binary(tmp_path / "g.bin", g)
+    exe = os.path.join(ca.BIN_DIR, "exp_contraction")
+    for args in (["-input", "edgefile", "-in", str(tmp_path / "g.edges")],
+                 ["-input", "partition", "-in", str(tmp_path / "g.part"), "-parts", "2"],
+                 ["-input", "binary", "-in", str(tmp_path / "g.bin")]):
+        r = subprocess.run([exe] + args + ["-i", "2", "-check"], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "CHECK labels_bit_exact=yes" in r.stdout, args
+        assert f"n={g.n} m={g.m}" in r.stdout, args
+
+
 @pytest.mark.parametrize("name,kw", [
     ("gnm_undirected", dict(n=50000, m=200000, seed=1)),
     ("rmat", dict(n=1 << 15, m=1 << 19, seed=3)),

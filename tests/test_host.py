@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import components_amd as ca
+import oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -78,6 +79,105 @@ def test_metis_errors(tmp_path):
     with pytest.raises(ca.KcError) as e:
         ca.read_metis(p)
     assert e.value.code == ca.KC_ERR_INVALID
+
+
+def _edges_once(g):
+    """Every undirected edge of a CSR once, as (u, v) with u < v, by u."""
+    src = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.offsets.astype(np.int64)))
+    tgt = g.targets.astype(np.int64)
+    keep = src < tgt
+    return src[keep], tgt[keep]
+
+
+def _write_edgefile(path, n, u, v, header_m=None, extra=""):
+    lines = [f"p {n} {len(u) if header_m is None else header_m}"]
+    lines += [f"e {a + 1} {b + 1}" for a, b in zip(u.tolist(), v.tolist())]
+    path.write_text("\n".join(lines) + "\n" + extra)
+
+
+def test_edgefile_matches_generated_graph(tmp_path):
+    g = ca.generate("gnm_undirected", n=700, m=1500, seed=6)
+    u, v = _edges_once(g)
+    p = tmp_path / "g.edges"
+    _write_edgefile(p, g.n, u, v)
+    h = ca.read_edgefile(p)
+    assert h.n == g.n and np.array_equal(g.offsets, h.offsets) and np.array_equal(g.targets, h.targets)
+
+
+def test_edgefile_semantics(tmp_path):
+    p = tmp_path / "s.edges"
+    # comments before the header and between edges, CRLF, one direction per
+    # edge, a duplicate and a self-loop; the header's m = 4 bounds the lines read
+    p.write_bytes(b"% c\r\np 6 4\r\ne 1 3\r\n% inner\r\ne 3 1\r\ne 2 2\r\ne 4 5\r\ne 5 6\r\n")
+    g = ca.read_edgefile(p)
+    assert g.n == 6
+    assert list(g.offsets) == [0, 1, 1, 2, 3, 4, 4]
+    assert list(g.targets) == [2, 0, 4, 3]
+
+
+def test_edgefile_simple_fixture_labels(tmp_path):
+    # the reference's example graph, as an edge file: same CSR, same components
+    m = ca.read_metis(os.path.join(GOLDEN, "simple.graph"))
+    u, v = _edges_once(m)
+    p = tmp_path / "simple.edges"
+    _write_edgefile(p, m.n, u, v)
+    g = ca.read_edgefile(p)
+    assert np.array_equal(g.offsets, m.offsets) and np.array_equal(g.targets, m.targets)
+    assert list(oracle.canonical_labels(g.offsets, g.targets)) == [0, 0, 0, 3, 3, 3]
+
+
+def test_partitioned_edgefile(tmp_path):
+    g = ca.generate("rgg_2d", n=2000, r=0.03, seed=2)
+    u, v = _edges_once(g)
+    cuts = [0, len(u) // 3, 2 * len(u) // 3, len(u)]
+    for r in range(3):
+        _write_edgefile(tmp_path / f"part_{r}", g.n, u[cuts[r]:cuts[r + 1]], v[cuts[r]:cuts[r + 1]],
+                        header_m=len(u))
+    h = ca.read_partitioned_edgefile(tmp_path / "part", parts=3)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.targets, h.targets)
+    one = ca.read_partitioned_edgefile(tmp_path / "part", parts=1)  # one rank's share alone
+    assert one.m == cuts[1]
+    (tmp_path / "bad_0").write_text(f"p {g.n} {len(u)}\n")
+    (tmp_path / "bad_1").write_text(f"p {g.n + 1} {len(u)}\n")
+    with pytest.raises(ca.KcError) as e:
+        ca.read_partitioned_edgefile(tmp_path / "bad", parts=2)
+    assert e.value.code == ca.KC_ERR_INVALID
+
+
+def test_binary_roundtrip_and_layout(tmp_path):
+    g = ca.generate("gnm_undirected", n=900, m=2500, seed=8)
+    p = tmp_path / "g.bin"
+    ca.write_binary(p, g)
+    raw = np.fromfile(p, dtype=np.uint64)
+    u, v = _edges_once(g)
+    assert raw[0] == g.n and raw[1] == len(u)
+    assert np.array_equal(raw[2::2], u + 1) and np.array_equal(raw[3::2], v + 1)
+    h = ca.read_binary(p)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.targets, h.targets)
+
+
+def test_binary_and_edgefile_errors(tmp_path):
+    p = tmp_path / "t.bin"
+    np.array([4, 3, 1, 2, 2, 3], dtype=np.uint64).tofile(p)  # 3 edges announced, 2 present
+    with pytest.raises(ca.KcError) as e:
+        ca.read_binary(p)
+    assert e.value.code == ca.KC_ERR_IO
+    np.array([4, 1, 1, 5], dtype=np.uint64).tofile(p)  # id 5 outside [1, 4]
+    with pytest.raises(ca.KcError) as e:
+        ca.read_binary(p)
+    assert e.value.code == ca.KC_ERR_INVALID
+    with pytest.raises(ca.KcError) as e:
+        ca.read_binary(tmp_path / "missing.bin")
+    assert e.value.code == ca.KC_ERR_IO
+    q = tmp_path / "t.edges"
+    q.write_text("p 3 1\ne 1 0\n")
+    with pytest.raises(ca.KcError) as e:
+        ca.read_edgefile(q)
+    assert e.value.code == ca.KC_ERR_INVALID
+    q.write_text("p 3\n")
+    with pytest.raises(ca.KcError) as e:
+        ca.read_edgefile(q)
+    assert e.value.code == ca.KC_ERR_IO
 
 
 def check_csr(g):
