@@ -146,6 +146,7 @@ _SIGNATURES = {
     "kc_csr_free": (None, [_P(Csr)]),
     "kc_graph_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(ctypes.c_void_p)]),
     "kc_graph_download": (ctypes.c_int, [ctypes.c_void_p, _P(Csr)]),
+    "kc_graph_read_binary": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
     "kc_batch_gen_params_init": (None, [_P(BatchGenParams)]),
     "kc_batch_generate": (ctypes.c_int, [_P(BatchGenParams), _P(BatchCsr)]),
     "kc_batch_free": (None, [_P(BatchCsr)]),
@@ -370,6 +371,13 @@ class DeviceGraph:
         """kc_graph_generate: the graph is generated and built in HBM."""
         h = ctypes.c_void_p()
         _check(lib().kc_graph_generate(name.encode(), ctypes.byref(gen_params(**kw)), ctypes.byref(h)))
+        return DeviceGraph(_handle=h)
+
+    @staticmethod
+    def read_binary(path):
+        """kc_graph_read_binary: a sorted binary graph file parsed and built in HBM."""
+        h = ctypes.c_void_p()
+        _check(lib().kc_graph_read_binary(str(path).encode(), ctypes.byref(h)))
         return DeviceGraph(_handle=h)
 
     def download(self):

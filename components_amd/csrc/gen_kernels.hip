@@ -113,7 +113,29 @@ __global__ __launch_bounds__(kBlock) void k_nonzero_rows(uint64_t n, const uint6
   }
 }
 
+// Edges of a sorted binary file (1-based (u, v) u64 pairs) as 0-based u32
+// pairs; a pair outside [1, n] raises *bad and becomes the self-loop (0, 0),
+// which the CSR build drops.
+__global__ void k_binary_edges(const uint64_t *raw, uint64_t count, uint64_t n, uint32_t *src, uint32_t *dst,
+                               int *bad) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t u = raw[2 * i], v = raw[2 * i + 1];
+    const bool ok = u >= 1 && v >= 1 && u <= n && v <= n;
+    if (!ok) *bad = 1;
+    src[i] = ok ? static_cast<uint32_t>(u - 1) : 0u;
+    dst[i] = ok ? static_cast<uint32_t>(v - 1) : 0u;
+  }
+}
+
 }  // namespace
+
+void binary_edges(const uint64_t *raw, uint64_t count, uint64_t n, uint32_t *src, uint32_t *dst, int *bad,
+                  hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_binary_edges, dim3(grid_cap(count)), dim3(kBlock), 0, s, raw, count, n, src, dst, bad);
+  KC_LAUNCH_CHECK();
+}
 
 void gen_gnm(uint64_t count, uint64_t n, int32_t seed, uint32_t *src, uint32_t *dst, hipStream_t s) {
   hipLaunchKernelGGL(k_gen_gnm, dim3(grid_cap(count)), dim3(kBlock), 0, s, count, n, gen_stream(seed, 1), src, dst);

@@ -1249,6 +1249,53 @@ int kc_graph_generate(const char *name, const kc_gen_params *pp, kc_graph **out)
   });
 }
 
+int kc_graph_read_binary(const char *path, kc_graph **out) {
+  return guarded([&] {
+    if (!path || !out) fail(KC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<FILE, int (*)(FILE *)> fp(std::fopen(path, "rb"), std::fclose);
+    if (!fp) fail(KC_ERR_IO, "Error opening %s", path);
+    uint64_t head[2] = {0, 0};
+    if (std::fread(head, sizeof(uint64_t), 2, fp.get()) != 2) fail(KC_ERR_IO, "%s: missing header (n and m as u64)", path);
+    const uint64_t n = head[0], m = head[1];
+    if (n >= 0xFFFFFFFFull) fail(KC_ERR_INVALID, "%s: n=%llu exceeds 32-bit vertex ids", path, (unsigned long long)n);
+    std::fseek(fp.get(), 0, SEEK_END);
+    const long size = std::ftell(fp.get());
+    std::fseek(fp.get(), 2 * sizeof(uint64_t), SEEK_SET);
+    const uint64_t present = size > 16 ? static_cast<uint64_t>(size - 16) / 16 : 0;
+    if (present < m)
+      fail(KC_ERR_IO, "%s: %llu of %llu edges present", path, (unsigned long long)present, (unsigned long long)m);
+    std::unique_ptr<kc_graph> g = new_handle();
+    hipStream_t s = g->stream;
+    {
+      // the pairs go through a pinned staging buffer to the device in chunks,
+      // where they become 0-based u32 edges; the CSR is then built in HBM
+      constexpr uint64_t kChunk = 1ull << 22;  // pairs per chunk (64 MB)
+      DevTemp src(m * 4 + 64), dst(m * 4 + 64), raw(kChunk * 16), bad(64);
+      void *pinned = nullptr;
+      KC_HIP(hipHostMalloc(&pinned, kChunk * 16));
+      std::unique_ptr<void, hipError_t (*)(void *)> pin(pinned, hipHostFree);
+      fill_u32(bad.as<uint32_t>(), 1, 0, s);
+      for (uint64_t done = 0; done < m;) {
+        const uint64_t take = std::min<uint64_t>(kChunk, m - done);
+        if (std::fread(pinned, 16, take, fp.get()) != take) fail(KC_ERR_IO, "%s: read failed", path);
+        KC_HIP(hipMemcpyAsync(raw.p, pinned, take * 16, hipMemcpyHostToDevice, s));
+        binary_edges(raw.as<uint64_t>(), take, n, src.as<uint32_t>() + done, dst.as<uint32_t>() + done,
+                     bad.as<int>(), s);
+        KC_HIP(hipStreamSynchronize(s));  // the staging buffer is refilled next
+        done += take;
+      }
+      int h_bad = 0;
+      KC_HIP(hipMemcpy(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost));
+      if (h_bad) fail(KC_ERR_INVALID, "%s: an edge lies outside [1,%llu]", path, (unsigned long long)n);
+      g->n = n;
+      build_device_csr(g.get(), n, src.as<uint32_t>(), dst.as<uint32_t>(), m);
+    }
+    init_work(g.get());
+    *out = g.release();
+  });
+}
+
 }  // extern "C"
 
 namespace kc {

@@ -126,7 +126,7 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
 // The labels lab[col[e]] of row v that differ from lab[v] (after a best-effort
 // LDS filter of repeats), packed at relab[off[v] ...] (hub chunks: at the
 // chunk start); deg_inter[v] (hub chunks: hub_inter) = their number.
-// One persistent launch; lab[0 .. min(n_hot, 16384)) is served from LDS (for
+// One persistent launch; lab[0 .. min(n_hot, 28672)) is served from LDS (for
 // degree-ordered levels, whose first vertices take most endpoints).
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
              uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s);
@@ -233,6 +233,10 @@ void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_
                       hipStream_t s);
 
 // Device generators (gen_kernels.hip): the edge streams of the host generators.
+// 1-based (u, v) u64 pairs of a binary graph file -> 0-based u32 pairs
+// (*bad = 1 when a pair lies outside [1, n]).
+void binary_edges(const uint64_t *raw, uint64_t count, uint64_t n, uint32_t *src, uint32_t *dst, int *bad,
+                  hipStream_t s);
 void gen_gnm(uint64_t count, uint64_t n, int32_t seed, uint32_t *src, uint32_t *dst, hipStream_t s);
 void gen_rmat(uint64_t count, uint32_t scale, double a, double b, double c, int32_t seed, uint32_t *src, uint32_t *dst,
               hipStream_t s);

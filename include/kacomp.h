@@ -202,6 +202,12 @@ int kc_graph_info(const kc_graph *g, uint64_t *n, uint64_t *m, uint64_t *non_iso
  * as kc_generate; the order inside a row is not fixed).  Input preparation
  * only: IOUtility::LoadGraph's generator branch (lib/io/io_utils.h:46-51). */
 int kc_graph_generate(const char *name, const kc_gen_params *p, kc_graph **out);
+/* Reads a sorted binary graph file (the format of kc_read_binary) straight
+ * into a device graph: the raw pairs are staged to HBM, converted there, and
+ * the CSR is built on the device (same rows as kc_read_binary; the order
+ * inside a row is not fixed).  GraphIO::ReadSortedBinaryFile
+ * (lib/io/graph_io.h:910-1175). */
+int kc_graph_read_binary(const char *path, kc_graph **out);
 /* Copies the level-0 CSR of a graph to host memory (free with kc_csr_free). */
 int kc_graph_download(const kc_graph *g, kc_csr *out);
 

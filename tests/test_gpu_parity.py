@@ -400,3 +400,29 @@ def test_huge_bucket_modes_on_a_hub_graph(mode):
         lab, _ = ca.find_components(g, variant="exp", seed=1001, seq_limit=0, use_bfs=bfs,
                                     stream=ca.STREAM_HASHED, dedup_huge=mode)
         assert np.array_equal(lab, ref), (mode, bfs)
+
+
+def test_device_binary_reader_matches_host(tmp_path):
+    def sorted_rows(g):
+        off = g.offsets.astype(np.int64)
+        return [sorted(g.targets[off[v]:off[v + 1]].tolist()) for v in range(g.n)]
+
+    for name, kw in (("rmat", dict(n=1 << 14, m=1 << 18, seed=5)), ("gnm_undirected", dict(n=5000, m=3000, seed=2))):
+        g = ca.generate(name, **kw)
+        p = tmp_path / f"{name}.bin"
+        ca.write_binary(p, g)
+        host = ca.read_binary(p)
+        dg = ca.DeviceGraph.read_binary(p)
+        try:
+            dev = dg.download()
+            assert dev.n == host.n and np.array_equal(dev.offsets, host.offsets)
+            assert sorted_rows(dev) == sorted_rows(host)
+            lab, _ = dg.find_components(variant="exp", seed=1001)
+            assert np.array_equal(lab, oracle.canonical_labels(host.offsets, host.targets))
+        finally:
+            dg.close()
+    bad = tmp_path / "bad.bin"
+    np.array([3, 2, 1, 2, 2, 4], dtype=np.uint64).tofile(bad)  # id 4 outside [1, 3]
+    with pytest.raises(ca.KcError) as e:
+        ca.DeviceGraph.read_binary(bad)
+    assert e.value.code == ca.KC_ERR_INVALID
