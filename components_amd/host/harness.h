@@ -81,6 +81,11 @@ inline Config ParseParameters(int argc, char **argv) {
   const bool exact_n = a.IsSet("exact_n") || grid;
   const bool exact_m = a.IsSet("exact_m") || grid;
   const uint64_t nv = a.Get<uint64_t>("n", 18), mv = a.Get<uint64_t>("m", 18);
+  if ((!exact_n && nv > 62) || (!exact_m && mv > 62)) {  // -n / -m are log2 sizes (KaGen's convention)
+    std::fprintf(stderr, "-n %llu / -m %llu: log2 sizes above 62 (use -exact_n / -exact_m for absolute sizes)\n",
+                 static_cast<unsigned long long>(nv), static_cast<unsigned long long>(mv));
+    std::exit(1);
+  }
   c.gen_n = exact_n ? nv : (1ull << nv);
   c.gen_m = exact_m ? mv : (1ull << mv);
   c.gen_z = a.Get<uint64_t>("z", 0);
@@ -189,7 +194,9 @@ inline std::vector<uint32_t> HostCanonicalLabels(const kc_csr &g) {
   return lab;
 }
 
-inline int RunApp(int argc, char **argv, int variant, const char *runner) {
+// whole_graph_base_case: every edge goes to the base case (no contraction
+// level), as the all_reduce program gathers the whole graph on one PE.
+inline int RunApp(int argc, char **argv, int variant, const char *runner, bool whole_graph_base_case = false) {
   Config c = ParseParameters(argc, argv);
   PrintParameters(c);
   if (c.replicate_high_degree)
@@ -250,6 +257,7 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner) {
     // below which the single-GPU base case is cheaper than further levels.
     const uint64_t rel = (g.m / (c.pes ? c.pes : 1)) * c.sequential_limit;
     o.seq_limit = c.fixed_limit ? c.sequential_limit : (rel > 4096 ? rel : 4096);
+    if (whole_graph_base_case) o.seq_limit = ~0ull;
     o.deviate_stream = c.hashed ? KC_STREAM_HASHED : KC_STREAM_REFERENCE;
     std::vector<uint32_t> labels(g.n);
     kc_stats st;

@@ -240,11 +240,15 @@ def test_app_check_flag():
     assert r.returncode == 0, r.stdout + r.stderr
     assert "CHECK labels_bit_exact=yes" in r.stdout
     assert "GLOBAL RESULT runner=exp" in r.stdout
-    for app in ("local_contraction", "shortcuts", "labels"):
+    for app in ("local_contraction", "shortcuts", "labels", "all_reduce"):
         r = subprocess.run([os.path.join(ca.BIN_DIR, app), "-input", "file", "-in",
                             os.path.join(GOLDEN, "simple.graph"), "-i", "1", "-check"],
                            capture_output=True, text=True, timeout=300)
         assert r.returncode == 0 and "CHECK labels_bit_exact=yes" in r.stdout, app
+    r = subprocess.run([os.path.join(ca.BIN_DIR, "all_reduce"), "-gen", "rmat", "-n", "15", "-m", "19", "-i", "2",
+                        "-check"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "CHECK labels_bit_exact=yes" in r.stdout, r.stdout + r.stderr
+    assert "GLOBAL RESULT runner=allreduce" in r.stdout and "n=32768" in r.stdout
 
 
 def test_app_reads_edge_and_binary_inputs(tmp_path):

@@ -276,6 +276,10 @@ def test_cli_flag_parsing_without_device():
     assert re.search(r"\bn\s+256\b", r.stdout) and re.search(r"\bm\s+512\b", r.stdout)
     if not os.path.exists("/dev/kfd"):
         assert r.returncode != 0
+    # -n and -m are log2 sizes: a shift past 62 is refused before anything runs
+    r = subprocess.run([os.path.join(ca.BIN_DIR, "all_reduce"), "-gen", "rmat", "-n", "1048576", "-i", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "log2 sizes" in r.stderr
 
 
 def test_batch_generator_valid_and_deterministic():
