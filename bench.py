@@ -54,7 +54,7 @@ def parse():
                     help="rounds schedule: push or pull chosen per early round (0 default, 1 every level, -1 never)")
     ap.add_argument("--tail-sweeps", type=int, default=0,
                     help="rounds schedule: pull sweeps once <= n>>k rows are pending (0 default k, -1 never)")
-    ap.add_argument("--dedup-huge", default="auto", choices=["auto", "tables", "bitmaps", "bytemaps"],
+    ap.add_argument("--dedup-huge", default="auto", choices=["auto", "tables", "bitmaps", "bytemaps", "maps_tables"],
                     help="dedup of the huge cluster-edge buckets (auto: byte maps, bitmaps, tables by size)")
     ap.add_argument("--untimed-kernels", action="store_true",
                     help="time the steps without HIP events on the dominant kernel (no roofline)")

@@ -310,7 +310,7 @@ def write_binary(path, g):
 
 STREAM_REFERENCE, STREAM_HASHED = 0, 1
 # dedup of the huge cluster-edge buckets (kc_options.dedup_huge)
-DEDUP_MODES = {"auto": 0, "tables": 1, "bitmaps": 2, "bytemaps": 3}
+DEDUP_MODES = {"auto": 0, "tables": 1, "bitmaps": 2, "bytemaps": 3, "maps_tables": 4}
 
 
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,

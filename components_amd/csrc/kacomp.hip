@@ -304,7 +304,7 @@ constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulle
 constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average degree or more (sparse,
                                                 // high-diameter levels keep batched pushes)
 constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge buckets use hash tables
-constexpr uint64_t kHugeMapBytes = 4ull << 30;     // byte maps of the largest huge buckets
+constexpr uint64_t kHugeMapBytes = 8ull << 30;     // byte maps of the largest huge buckets
 constexpr uint64_t kMapMinShare = 32;              // byte maps from C / 32 elements on
 constexpr uint64_t kHubPullMin = 65536;            // smallest degree of a hub that pulls round 0
 constexpr uint32_t kMinWaveBin = 256;
@@ -470,33 +470,39 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   // maps' memory before any size is read back; map_min rises until that bound
   // fits the maps' cap.
   const uint64_t words = (label_space + 31) / 32;
-  if (mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_BITMAPS ||
-      (mode == KC_DEDUP_AUTO && n_huge * words * 4 <= kHugeBitmapBytes)) {
-    const uint64_t stride = huge_map_stride(label_space);
-    uint64_t map_min = ~0ull, n_maps = 0;
-    if (mode != KC_DEDUP_BITMAPS && label_space) {
-      map_min = mode == KC_DEDUP_BYTEMAPS ? kDedupBlockCap + 1
-                                          : std::max<uint64_t>(kDedupBlockCap + 1, label_space / kMapMinShare);
-      const uint64_t cap_maps = kHugeMapBytes / stride;
-      while (map_min != ~0ull && std::min(n_huge, total / map_min) > cap_maps)
-        map_min = cap_maps ? total / cap_maps + 1 : ~0ull;
-      if (map_min != ~0ull) n_maps = std::min(n_huge, total / map_min);
-      if (!n_maps) map_min = ~0ull;
-    }
+  const bool bits_fit = n_huge * words * 4 <= kHugeBitmapBytes;
+  const uint64_t stride = huge_map_stride(label_space);
+  uint64_t map_min = ~0ull, n_maps = 0;
+  if ((mode == KC_DEDUP_AUTO || mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_MAPS_TABLES) && label_space) {
+    map_min = mode == KC_DEDUP_BYTEMAPS ? kDedupBlockCap + 1
+                                        : std::max<uint64_t>(kDedupBlockCap + 1, label_space / kMapMinShare);
+    const uint64_t cap_maps = kHugeMapBytes / stride;
+    while (map_min != ~0ull && std::min(n_huge, total / map_min) > cap_maps)
+      map_min = cap_maps ? total / cap_maps + 1 : ~0ull;
+    if (map_min != ~0ull) n_maps = std::min(n_huge, total / map_min);
+    if (!n_maps) map_min = ~0ull;
+  }
+  if (n_maps * stride > g->huge_maps_cap) {  // maps start zero and every call leaves them zero
+    g->huge_maps_cap = std::min(kHugeMapBytes, n_maps * stride + (n_maps * stride >> 2));
+    g->huge_maps = g->dalloc<uint8_t>(g->huge_maps_cap);
+    fill_u32(reinterpret_cast<uint32_t *>(g->huge_maps), g->huge_maps_cap / 4, 0u, s);
+  }
+  if (mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_BITMAPS || (mode == KC_DEDUP_AUTO && bits_fit)) {
     if (map_min > kDedupBlockCap + 1 && n_huge * words > g->huge_bits_cap) {
       g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
       g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
     }
-    if (n_maps * stride > g->huge_maps_cap) {  // maps start zero and every call leaves them zero
-      g->huge_maps_cap = std::min(kHugeMapBytes, n_maps * stride + (n_maps * stride >> 2));
-      g->huge_maps = g->dalloc<uint8_t>(g->huge_maps_cap);
-      fill_u32(reinterpret_cast<uint32_t *>(g->huge_maps), g->huge_maps_cap / 4, 0u, s);
-    }
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
-    dedup_huge_mixed(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, map_min, n_maps,
-                     g->d_huge_info, g->huge_bits, words, g->huge_maps, label_space, bucket, out, uniq, scratch, s);
+    dedup_huge_mixed(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, map_min, n_maps, true,
+                     true, g->d_huge_info, g->huge_bits, words, g->huge_maps, label_space, bucket, out, uniq, scratch,
+                     s);
     return first;
   }
+  // Global hash tables (KC_DEDUP_TABLES, or the bitmaps over their cap): in
+  // AUTO (and MAPS_TABLES) mode the buckets of at least map_min elements still
+  // go to byte maps after the tables, the others to tables sized by their
+  // bucket (a bitmap costs C/8 bytes however small the bucket).
+  if (mode == KC_DEDUP_TABLES) map_min = ~0ull, n_maps = 0;
   huge_sizes(static_cast<uint32_t>(n_huge), huge, seg_off, g->d_huge_info, s);
   std::vector<uint64_t> info(2 * n_huge);
   std::vector<uint32_t> a2s(n_huge);
@@ -507,6 +513,7 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   uint64_t huge_slots = 0;  // tables of at least twice their bucket, one after another
   for (uint64_t h = 0; h < n_huge; ++h) {
     const uint64_t size = info[2 * h + 1];
+    if (size >= map_min) continue;  // a byte map's bucket: its count stays 0 until the maps add to it
     uint64_t tsz = 1;
     while (tsz < 2 * size) tsz <<= 1;
     HugeChunk c{info[2 * h], huge_slots, 0, 0, static_cast<uint32_t>(tsz - 1), static_cast<uint32_t>(h), a2s[h], 0};
@@ -541,6 +548,10 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   dedup_huge(g->d_chunks, static_cast<uint32_t>(elem.size()), g->d_chunks + elem.size(),
              static_cast<uint32_t>(slot.size()), static_cast<uint32_t>(n_huge), huge, g->huge_tables, bucket, out, uniq,
              hcount, s);
+  if (n_maps)  // the rest against byte maps (uniq of their buckets left at 0 by the tables' counts, if any ran)
+    dedup_huge_mixed(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, map_min, n_maps, false,
+                     elem.empty(), g->d_huge_info, g->huge_bits, words, g->huge_maps, label_space, bucket, out, uniq, scratch,
+                     s);
   // the chunk vectors die here: wait until the uploads have been consumed
   sync(g);
   return first;
@@ -1367,7 +1378,7 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
     else
       kc_options_init(&o);
     if (o.variant < KC_VARIANT_EXP || o.variant > KC_VARIANT_LABELS) fail(KC_ERR_INVALID, "unknown variant %d", o.variant);
-    if (o.dedup_huge < KC_DEDUP_AUTO || o.dedup_huge > KC_DEDUP_BYTEMAPS)
+    if (o.dedup_huge < KC_DEDUP_AUTO || o.dedup_huge > KC_DEDUP_MAPS_TABLES)
       fail(KC_ERR_INVALID, "unknown dedup_huge mode %d", o.dedup_huge);
     if (o.time_class < 0 || o.time_class > KC_KERNEL_CLASSES)
       fail(KC_ERR_INVALID, "time_class must be in [0, %d], got %d", KC_KERNEL_CLASSES, o.time_class);

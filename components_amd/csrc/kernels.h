@@ -168,13 +168,15 @@ void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, 
 // Huge buckets over the label space [0, space): those of at least map_min
 // elements (at most n_maps of them) against byte maps of huge_map_stride(space)
 // bytes, all zero on entry and left all zero; the others against bitmaps of
-// `words` u32 per bucket.  info: 3 n_huge + 6 words of scratch for the chunk
-// plans (map_min = ~0: bitmaps only).
+// `words` u32 per bucket (with_bits false: the others are left alone).
+// zero_uniq: uniq of every huge bucket is zeroed first (else the caller has
+// zeroed it for the map buckets).  info: 3 n_huge + 6 words of scratch for
+// the chunk plans (map_min = ~0: bitmaps only).
 uint64_t huge_map_stride(uint64_t space);
 void dedup_huge_mixed(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
-                      uint64_t map_min, uint64_t n_maps, uint64_t *info, uint32_t *bitmaps, uint64_t words,
-                      uint8_t *maps, uint64_t space, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
-                      uint64_t *scratch, hipStream_t s);
+                      uint64_t map_min, uint64_t n_maps, bool with_bits, bool zero_uniq, uint64_t *info,
+                      uint32_t *bitmaps, uint64_t words, uint8_t *maps, uint64_t space, const uint32_t *bucket,
+                      uint32_t *out, uint32_t *uniq, uint64_t *scratch, hipStream_t s);
 // Huge buckets in global open-addressing tables (pre-filled with kNone by the
 // caller, hcount zeroed), for label spaces whose bitmaps would not fit.
 void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *slot_chunks, uint32_t n_slot,

@@ -1886,20 +1886,22 @@ void dedup_huge(const HugeChunk *elem_chunks, uint32_t n_elem, const HugeChunk *
 uint64_t huge_map_stride(uint64_t space) { return (space + kMapPiece - 1) / kMapPiece * kMapPiece; }
 
 void dedup_huge_mixed(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, uint64_t max_chunks,
-                      uint64_t map_min, uint64_t n_maps, uint64_t *info, uint32_t *bitmaps, uint64_t words,
-                      uint8_t *maps, uint64_t space, const uint32_t *bucket, uint32_t *out, uint32_t *uniq,
-                      uint64_t *scratch, hipStream_t s) {
+                      uint64_t map_min, uint64_t n_maps, bool with_bits, bool zero_uniq, uint64_t *info,
+                      uint32_t *bitmaps, uint64_t words, uint8_t *maps, uint64_t space, const uint32_t *bucket,
+                      uint32_t *out, uint32_t *uniq, uint64_t *scratch, hipStream_t s) {
   if (!n_huge) return;
   if (!n_maps || !space) map_min = ~0ull;
   uint64_t *cbits = info, *cmap = info + n_huge + 1;
   Pair64 *mtot = reinterpret_cast<Pair64 *>(info + 2 * n_huge + 2);
   uint32_t *mslot = reinterpret_cast<uint32_t *>(info + 2 * n_huge + 4);
   uint32_t *mlist = mslot + n_huge;
-  hipLaunchKernelGGL(k_zero_huge_uniq, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s, n_huge,
-                     huge, uniq);
-  KC_LAUNCH_CHECK();
+  if (zero_uniq) {
+    hipLaunchKernelGGL(k_zero_huge_uniq, dim3(static_cast<unsigned>(ceil_div(n_huge, kBlock))), dim3(kBlock), 0, s,
+                       n_huge, huge, uniq);
+    KC_LAUNCH_CHECK();
+  }
   const uint64_t g = std::min<uint64_t>(8192, max_chunks);
-  if (map_min > kDedupBlockCap + 1) {  // some huge buckets against bitmaps
+  if (with_bits && map_min > kDedupBlockCap + 1) {  // some huge buckets against bitmaps
     scan_exclusive<uint64_t>(n_huge, HugeChunkLoad{huge, seg_off, 0, map_min}, ArrayStore<uint64_t>{cbits},
                              cbits + n_huge, scratch, s);
     fill_u32(bitmaps, static_cast<uint64_t>(n_huge) * words, 0u, s);

@@ -132,9 +132,11 @@ typedef struct kc_options {
                              0 = on levels of >= 2^16 vertices and average degree >= 8, 1 = on every
                              level, -1 = always pushed (same results) */
   int32_t dedup_huge;     /* dedup of the huge cluster-edge buckets: KC_DEDUP_AUTO (byte maps over the
-                             cluster ids while they take at most 1 GB, then bitmaps up to 2 GB, then
-                             global hash tables), or one of KC_DEDUP_TABLES, KC_DEDUP_BITMAPS,
-                             KC_DEDUP_BYTEMAPS (same results) */
+                             cluster ids for the largest, within 8 GB; the others against bitmaps
+                             while those take at most 2 GB, else in global hash tables), or one of
+                             KC_DEDUP_TABLES, KC_DEDUP_BITMAPS, KC_DEDUP_BYTEMAPS (byte maps as far
+                             as the cap allows, bitmaps for the rest), KC_DEDUP_MAPS_TABLES (byte
+                             maps for the largest, tables for the rest) (same results) */
   int32_t time_class;     /* with time_kernels: 0 = every launch bracketed by HIP events, k > 0 = only
                              the launches of class k - 1 (the others are counted, not timed) */
 } kc_options;
@@ -144,6 +146,7 @@ typedef struct kc_options {
 #define KC_DEDUP_TABLES 1
 #define KC_DEDUP_BITMAPS 2
 #define KC_DEDUP_BYTEMAPS 3
+#define KC_DEDUP_MAPS_TABLES 4
 
 /* Deviate streams.  REFERENCE: the draw of RunContractionLP bit for bit
  * (per-vertex minstd_rand0 seeds; its outputs are linear in the seed, so the

@@ -212,7 +212,7 @@ def test_rmat_scale22_schedules_agree():
     # byte maps
     for bfs, tail, r1, tab in ((False, 0, 0, "auto"), (True, 0, 0, "auto"), (True, -1, -1, "auto"),
                                (True, 1, 0, "auto"), (True, 0, 1, "auto"), (True, 0, 0, "tables"),
-                               (True, 0, 0, "bitmaps"), (True, 0, 0, "bytemaps")):
+                               (True, 0, 0, "bitmaps"), (True, 0, 0, "bytemaps"), (True, 0, 0, "maps_tables")):
         lab, st = dg.find_components(variant="exp", seed=1001, seq_limit=dg.m // 256, stream=ca.STREAM_HASHED,
                                      use_bfs=bfs, tail_sweeps=tail, pull_rounds=r1, dedup_huge=tab,
                                      record_trace=True)
@@ -373,7 +373,7 @@ def test_option_checks_and_timed_class():
     ref = oracle.canonical_labels(g.offsets, g.targets)
     dg = ca.DeviceGraph(g)
     try:
-        for bad in (dict(dedup_huge=7), dict(dedup_huge=-1)):
+        for bad in (dict(dedup_huge=5), dict(dedup_huge=-1)):
             with pytest.raises(ca.KcError):
                 dg.find_components(variant="exp", seed=1001, **bad)
         # every launch timed, then only the relabel launches: same launches, time on one class only
@@ -391,7 +391,7 @@ def test_option_checks_and_timed_class():
         dg.close()
 
 
-@pytest.mark.parametrize("mode", ["bitmaps", "bytemaps", "tables"])
+@pytest.mark.parametrize("mode", ["bitmaps", "bytemaps", "tables", "maps_tables"])
 def test_huge_bucket_modes_on_a_hub_graph(mode):
     # two hubs sharing most of their neighbours: their clusters' buckets are huge and full of repeats
     rng = np.random.default_rng(11)
