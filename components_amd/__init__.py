@@ -53,7 +53,8 @@ class Options(ctypes.Structure):
                 ("deviate_stream", ctypes.c_int32), ("learned_alpha", ctypes.c_float),
                 ("input_order", ctypes.c_int32), ("tail_sweeps", ctypes.c_int32),
                 ("pull_rounds", ctypes.c_int32),
-                ("dedup_huge", ctypes.c_int32), ("time_class", ctypes.c_int32)]
+                ("dedup_huge", ctypes.c_int32), ("time_class", ctypes.c_int32),
+                ("pre_contract", ctypes.c_int32), ("pre_blocks", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -311,11 +312,14 @@ def write_binary(path, g):
 STREAM_REFERENCE, STREAM_HASHED = 0, 1
 # dedup of the huge cluster-edge buckets (kc_options.dedup_huge)
 DEDUP_MODES = {"auto": 0, "tables": 1, "bitmaps": 2, "bytemaps": 3, "maps_tables": 4}
+# kc_options.pre_contract: none, the default branch's local contraction, -single, -contraction
+PRE_MODES = {"none": 0, "local": 1, "single": 2, "double": 3}
 
 
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
             tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
-            alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_huge="auto", time_class=None):
+            alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_huge="auto", time_class=None,
+            pre_contract="none", pre_blocks=1):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -337,6 +341,9 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     o.dedup_huge = DEDUP_MODES[dedup_huge] if isinstance(dedup_huge, str) else int(dedup_huge)
     # with time_kernels: events on the launches of one class only (a name of KERNEL_CLASSES)
     o.time_class = 0 if time_class is None else KERNEL_CLASSES.index(time_class) + 1
+    # local components over pre_blocks blocks of consecutive ids contracted first (a name of PRE_MODES)
+    o.pre_contract = PRE_MODES[pre_contract] if isinstance(pre_contract, str) else int(pre_contract)
+    o.pre_blocks = int(pre_blocks)
     return o
 
 

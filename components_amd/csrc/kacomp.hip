@@ -683,7 +683,10 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   // Level 0 in degree order when built (and the level is not already the base
   // case, whose pointer jumping indexes keys by level index): keys and rows by
   // storage position, key labels and gid_l by level index as before.
-  const bool ordered = g->idx0 && !o->input_order && !(o->seq_limit > 0 && m2_l / 2 <= o->seq_limit) && m2_l > 0;
+  // (the pre-contraction's blocks are ranges of level indices: input order)
+  const bool ordered = g->idx0 && !o->input_order && o->pre_contract == KC_PRE_NONE &&
+                       !(o->seq_limit > 0 && m2_l / 2 <= o->seq_limit) && m2_l > 0;
+  int pre_left = o->pre_contract == KC_PRE_DOUBLE ? 2 : o->pre_contract != KC_PRE_NONE ? 1 : 0;
   const uint32_t *idx_l = nullptr, *pos_l = nullptr;  // storage maps of the current level (null: identity)
   const uint32_t *dev_gid_l = gid_l;                   // input id of each storage position
   if (ordered) {
@@ -709,7 +712,11 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   bool bins_valid = false;  // bins of the current level graph (kept across stall levels)
 
   while (n_l > 0) {
-    if (m2_l == 0 || (o->seq_limit > 0 && m2_l / 2 <= o->seq_limit)) {
+    // a pre-contraction level comes before the cut-off, as the local phase
+    // comes before the decomposition
+    const bool pre = pre_left > 0 && m2_l > 0;
+    if (pre) --pre_left;
+    if (!pre && (m2_l == 0 || (o->seq_limit > 0 && m2_l / 2 <= o->seq_limit))) {
       // Base case (RunSequentialCC, exponential_contraction.h:908-922): min-label
       // propagation with pointer jumping on the remaining graph.
       if (!bins_valid) b = make_bins(g, n_l, off_l, cfg);
@@ -737,47 +744,80 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       b = make_bins(g, n_l, off_l, cfg);
       bins_valid = true;
     }
-    // deviates -> keys
-    LearnedArgs la{nullptr};
-    if (!local && o->learned) {
-      // scores of this level's vertices delay their deviates
-      TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
-      level_degrees(n_l, off_l, g->ldeg, g->stream);
-      level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
-      mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
-      la.delay = g->delay;
-    }
-    {
-      TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l + (gid_l ? 4 * n_l : 0) + (la.delay ? n_l : 0));
-      if (local) {
-        init_keys_uniform(n_l, dev_gid_l, idx_l, o->seed, iteration, g->key, g->stream);
-      } else {
-        da.iteration = iteration;
-        init_keys_exp(n_l, dev_gid_l, idx_l, da, la, g->key, g->stream);
-      }
-    }
     uint64_t sweeps = 0;
-    if (local) {
-      // two synchronous min passes over the closed neighbourhood
-      // (local_contraction.h:195-274), then a counting pass on the final keys
-      for (int pass = 0; pass < 2; ++pass) {
-        uint64_t *kin = pass == 0 ? g->key : g->key2;
-        uint64_t *kout = pass == 0 ? g->key2 : g->key;
-        KC_HIP(hipMemcpyAsync(kout, kin, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
-        TimedScope ts(g->timer, KC_K_SWEEP, sweep_bytes(n_l, m2_l, false));
-        sweep(kSweepJacobi, false, off_l, col_l, kin, kout, 0, b, g->deg_inter, g->hub_inter,
-              &g->flags[kMaxFlags - 1], nullptr, g->stream);
+    if (pre) {
+      // components inside the blocks: min-label propagation over the edges
+      // inside a block, with pointer jumping, to the fixpoint
+      const uint64_t bsize = ceil_div(n_l, static_cast<uint64_t>(o->pre_blocks));
+      {
+        TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l);
+        init_keys_index(n_l, g->key, g->stream);
       }
-      sweeps = 2;
-    } else if (o->use_bfs) {
-      sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base,
-                                  o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps,
-                                  o->pull_rounds > 0 || (o->pull_rounds == 0 && n_l >= kPullRoundsMin &&
-                                                         m2_l >= kDirectionMinDegree * n_l));
+      while (true) {
+        if (flag_base + 2 >= kMaxFlags) {
+          fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
+          flag_base = 0;
+        }
+        int *f = &g->flags[flag_base++];
+        {
+          TimedScope ts(g->timer, KC_K_SWEEP, sweep_bytes(n_l, m2_l, false));
+          block_prop(off_l, col_l, n_l, bsize, g->key, f, g->stream);
+        }
+        {
+          TimedScope ts(g->timer, KC_K_JUMP, 16 * n_l);
+          jump(n_l, g->key, f, g->stream);
+        }
+        d2h(g, g->h_flags, f, sizeof(int));
+        sync(g);
+        ++sweeps;
+        if (g->h_flags[0] == 0) break;
+      }
+      if (o->pre_contract == KC_PRE_LOCAL) {
+        TimedScope ts(g->timer, KC_K_SWEEP, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l);
+        block_interface(off_l, col_l, n_l, bsize, g->key, g->stream);
+      }
     } else {
-      sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
+      // deviates -> keys
+      LearnedArgs la{nullptr};
+      if (!local && o->learned) {
+        // scores of this level's vertices delay their deviates
+        TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
+        level_degrees(n_l, off_l, g->ldeg, g->stream);
+        level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
+        mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
+        la.delay = g->delay;
+      }
+      {
+        TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l + (gid_l ? 4 * n_l : 0) + (la.delay ? n_l : 0));
+        if (local) {
+          init_keys_uniform(n_l, dev_gid_l, idx_l, o->seed, iteration, g->key, g->stream);
+        } else {
+          da.iteration = iteration;
+          init_keys_exp(n_l, dev_gid_l, idx_l, da, la, g->key, g->stream);
+        }
+      }
+      if (local) {
+        // two synchronous min passes over the closed neighbourhood
+        // (local_contraction.h:195-274), then a counting pass on the final keys
+        for (int pass = 0; pass < 2; ++pass) {
+          uint64_t *kin = pass == 0 ? g->key : g->key2;
+          uint64_t *kout = pass == 0 ? g->key2 : g->key;
+          KC_HIP(hipMemcpyAsync(kout, kin, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
+          TimedScope ts(g->timer, KC_K_SWEEP, sweep_bytes(n_l, m2_l, false));
+          sweep(kSweepJacobi, false, off_l, col_l, kin, kout, 0, b, g->deg_inter, g->hub_inter,
+                &g->flags[kMaxFlags - 1], nullptr, g->stream);
+        }
+        sweeps = 2;
+      } else if (o->use_bfs) {
+        sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base,
+                                    o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps,
+                                    o->pull_rounds > 0 || (o->pull_rounds == 0 && n_l >= kPullRoundsMin &&
+                                                           m2_l >= kDirectionMinDegree * n_l));
+      } else {
+        sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
+      }
     }
-    if (!local && !o->use_bfs && sweeps == 1) {
+    if (!pre && !local && !o->use_bfs && sweeps == 1) {
       // Stall: the first sweep changed no key, so every vertex is its own
       // centre and keeps an edge (no level holds an isolated vertex); the
       // contracted graph equals this one.  The level is recorded and the next
@@ -803,7 +843,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     rec.n = n_l;
     rec.m2 = m2_l;
     rec.gid = gid_l;
-    rec.sweeps = sweeps;
+    rec.sweeps = pre ? 0 : sweeps;  // a pre-contraction level has no decomposition sweeps
     rec.lab = g->arena.alloc_n<uint32_t>(n_l);
     rec.newidx = g->arena.alloc_n<uint32_t>(n_l);
     uint32_t *gid_next = g->arena.alloc_n<uint32_t>(n_l);
@@ -814,9 +854,9 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     fill_u32(g->small + 4, 1, 0, g->stream);
     {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
-      labels_radius(n_l, g->key, pos_l, rec.lab, g->small + 4, !local, g->stream);
+      labels_radius(n_l, g->key, pos_l, rec.lab, g->small + 4, !local && !pre, g->stream);
     }
-    if (g->collect) {
+    if (g->collect && !pre) {
       // trace of the plain method: features and the log2 size of each vertex's cluster
       level_degrees(n_l, off_l, g->ldeg, g->stream);
       level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
@@ -891,8 +931,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     const uint64_t n_next = N ? g->h_sizes[0] : 0;
     const uint64_t P = N ? g->h_sizes[1] : 0;
     const uint64_t n_medium = N ? g->h_sizes[3] : 0, n_huge = N ? g->h_sizes[4] : 0;
-    rec.radius = local ? 0 : g->h_small[4];
-    out.sweeps += sweeps;
+    rec.radius = local || pre ? 0 : g->h_small[4];
+    if (!pre) out.sweeps += sweeps;  // the block passes are not decomposition sweeps
     out.radius_sum += rec.radius;
     g->levels.push_back(rec);
     if (n_next == 0) break;  // every cluster is a whole component
@@ -937,7 +977,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     pos_l = nullptr;
     pp ^= 1;
     bins_valid = false;
-    ++iteration;
+    if (!pre) ++iteration;
   }
   out.levels = static_cast<uint32_t>(g->levels.size());
 
@@ -1382,6 +1422,13 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
       fail(KC_ERR_INVALID, "unknown dedup_huge mode %d", o.dedup_huge);
     if (o.time_class < 0 || o.time_class > KC_KERNEL_CLASSES)
       fail(KC_ERR_INVALID, "time_class must be in [0, %d], got %d", KC_KERNEL_CLASSES, o.time_class);
+    if (o.pre_contract < KC_PRE_NONE || o.pre_contract > KC_PRE_DOUBLE)
+      fail(KC_ERR_INVALID, "unknown pre_contract mode %d", o.pre_contract);
+    if (o.pre_contract != KC_PRE_NONE) {
+      if (o.variant != KC_VARIANT_EXP && o.variant != KC_VARIANT_LOCAL)
+        fail(KC_ERR_UNSUPPORTED, "pre-contraction applies to the contraction variants only");
+      if (o.pre_blocks < 1) fail(KC_ERR_INVALID, "pre_blocks must be >= 1 with pre_contract %d", o.pre_contract);
+    }
     if (o.learned) {
       if (o.variant != KC_VARIANT_EXP) fail(KC_ERR_UNSUPPORTED, "learned centre proposals apply to exponential contraction only");
       if (!g->mlp) fail(KC_ERR_INVALID, "learned centre proposals need a model (kc_graph_set_mlp)");

@@ -100,6 +100,15 @@ void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uin
 // Pointer jumping on keys: key[v] <- min(key[v], key[label(v)]).
 void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
 
+// Components inside blocks of bsize consecutive indices (keys start as
+// init_keys_index): one pass of min-key propagation over the edges inside a
+// block, *flag = 1 when a key fell.  block_interface then gives every vertex
+// with a neighbour outside its block its own index back.
+void block_prop(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key, int *flag,
+                hipStream_t s);
+void block_interface(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key,
+                     hipStream_t s);
+
 // lab[v] = low half of key[v]; max cluster radius (atomicMax into *radius).
 void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
                    bool with_radius,

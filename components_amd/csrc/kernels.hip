@@ -546,6 +546,55 @@ __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
 }
 
 // ---------------------------------------------------------------------------
+// Components inside blocks of consecutive level indices (block b holds
+// [b*bsize, (b+1)*bsize)), the local components of a vertex-block distribution
+// over PEs.  One wave per row: key[v] (dev 0, label in the low half) takes the
+// least key among v's neighbours in v's block; *flag = 1 when a key fell.
+__global__ __launch_bounds__(kBlock) void k_block_prop(const uint64_t *off, const uint32_t *col, uint64_t n,
+                                                      uint64_t bsize, uint64_t *key, int *flag) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  bool changed = false;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock / kWave) + threadIdx.x / kWave; v < n; v += waves) {
+    const uint64_t lo = v / bsize * bsize, hi = lo + bsize < n ? lo + bsize : n;
+    uint64_t best = key[v];
+    const uint64_t e1 = off[v + 1];
+    for (uint64_t e = off[v] + lane_id(); e < e1; e += kWave) {
+      const uint32_t u = col[e];
+      if (u >= lo && u < hi) {
+        const uint64_t k = key[u];
+        best = k < best ? k : best;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t t = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(best), o, 64));
+      best = t < best ? t : best;
+    }
+    if (lane_id() == 0 && best < key[v]) {
+      key[v] = best;
+      changed = true;
+    }
+  }
+  if (__ballot(changed) != 0 && lane_id() == 0) *flag = 1;
+}
+
+// Interface vertices (a neighbour outside the block) keep their own index as
+// label: key[v] = v.
+__global__ __launch_bounds__(kBlock) void k_block_interface(const uint64_t *off, const uint32_t *col, uint64_t n,
+                                                           uint64_t bsize, uint64_t *key) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock / kWave) + threadIdx.x / kWave; v < n; v += waves) {
+    const uint64_t lo = v / bsize * bsize, hi = lo + bsize < n ? lo + bsize : n;
+    bool out = false;
+    const uint64_t e1 = off[v + 1];
+    for (uint64_t e = off[v] + lane_id(); e < e1; e += kWave) {
+      const uint32_t u = col[e];
+      out = out || u < lo || u >= hi;
+    }
+    if (__ballot(out) != 0 && lane_id() == 0) key[v] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Centre labels and the maximum cluster radius (hops from the centre).
 // lab[v] = the centre's level index; the centre's key is at pos[l] when the
 // level is stored in another order (null: l).
@@ -1717,6 +1766,24 @@ void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uin
   if (!b.nH) return;
   scan_exclusive<uint64_t>(b.nH, ArrLoadU32{hub_inter}, ArrayStore<uint64_t>{prefix}, prefix + b.nH, scratch, s);
   hipLaunchKernelGGL(k_hub_fold, dim3(grid_for(b.nH)), dim3(kBlock), 0, s, b.hubs, b.nH, prefix, deg_inter, hub_pos);
+  KC_LAUNCH_CHECK();
+}
+
+static uint32_t row_wave_grid(uint64_t n) {
+  return static_cast<uint32_t>(std::min<uint64_t>(ceil_div(n, static_cast<uint64_t>(kBlock / kWave)), 1u << 16));
+}
+
+void block_prop(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key, int *flag,
+                hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_block_prop, dim3(row_wave_grid(n)), dim3(kBlock), 0, s, off, col, n, bsize, key, flag);
+  KC_LAUNCH_CHECK();
+}
+
+void block_interface(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key,
+                     hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_block_interface, dim3(row_wave_grid(n)), dim3(kBlock), 0, s, off, col, n, bsize, key);
   KC_LAUNCH_CHECK();
 }
 

@@ -259,6 +259,13 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner, bool w
     o.seq_limit = c.fixed_limit ? c.sequential_limit : (rel > 4096 ? rel : 4096);
     if (whole_graph_base_case) o.seq_limit = ~0ull;
     o.deviate_stream = c.hashed ? KC_STREAM_HASHED : KC_STREAM_REFERENCE;
+    // -single takes precedence over -contraction (exponential_contraction.h:76,134): the
+    // components inside -pes blocks of consecutive ids are contracted first, once or twice
+    if (variant == KC_VARIANT_EXP && !whole_graph_base_case &&
+        (c.single_level_contraction || c.use_contraction)) {
+      o.pre_contract = c.single_level_contraction ? KC_PRE_SINGLE : KC_PRE_DOUBLE;
+      o.pre_blocks = static_cast<uint32_t>(c.pes == 0 ? 1 : c.pes > 0xFFFFFFFFull ? 0xFFFFFFFFull : c.pes);
+    }
     std::vector<uint32_t> labels(g.n);
     kc_stats st;
     if (kc_find_components(dg, &o, labels.data(), &st) != KC_OK) {

@@ -139,7 +139,28 @@ typedef struct kc_options {
                              maps for the largest, tables for the rest) (same results) */
   int32_t time_class;     /* with time_kernels: 0 = every launch bracketed by HIP events, k > 0 = only
                              the launches of class k - 1 (the others are counted, not timed) */
+  int32_t pre_contract;   /* local components contracted before the decomposition, the local phase
+                             of ExponentialContraction::FindComponents (exponential_contraction.h:
+                             60-319) with the vertices split over pre_blocks PEs by blocks of
+                             consecutive ids: KC_PRE_NONE (default: every vertex its own PE),
+                             KC_PRE_LOCAL, KC_PRE_SINGLE or KC_PRE_DOUBLE (below) */
+  uint32_t pre_blocks;    /* PEs (blocks) of the pre-contraction, >= 1 */
 } kc_options;
+
+/* Pre-contraction (kc_options.pre_contract), each a level of the contraction
+ * whose clusters are the components inside the blocks:
+ *   LOCAL  - the default branch (:218-317, CAGBuilder::BuildLocalComponentGraph,
+ *            cag_builder.h:160-230): vertices without a neighbour in another
+ *            block go to their block component, the others stay themselves;
+ *   SINGLE - -single (:76-133, BuildComponentAdjacencyGraph): every vertex goes
+ *            to its block component;
+ *   DOUBLE - -contraction (:134-217): SINGLE, then SINGLE again on the
+ *            component graph with its vertices (in component order) split over
+ *            the same number of blocks. */
+#define KC_PRE_NONE 0
+#define KC_PRE_LOCAL 1
+#define KC_PRE_SINGLE 2
+#define KC_PRE_DOUBLE 3
 
 /* Dedup of the huge cluster-edge buckets (kc_options.dedup_huge). */
 #define KC_DEDUP_AUTO 0

@@ -32,7 +32,7 @@ def lib():
             "or_std_fma_compiled": (i32, []),
             "or_lp_fixpoint": (u64, [u64, _P(u64), _P(u32), _P(u64), u64, _P(u64)]),
             "or_lp_closed_form": (None, [u64, _P(u64), _P(u32), _P(u64), u64, _P(u64)]),
-            "or_run": (ctypes.c_void_p, [u64, _P(u64), _P(u32), i32, i32, dbl, i32, i32, u64, i32]),
+            "or_run": (ctypes.c_void_p, [u64, _P(u64), _P(u32), i32, i32, dbl, i32, i32, u64, i32, i32, u64]),
             "or_hashed_deviate": (u32, [u32, i32, u32, dbl]),
             "or_free": (None, [ctypes.c_void_p]),
             "or_labels": (None, [ctypes.c_void_p, _P(u32)]),
@@ -103,14 +103,17 @@ class Run:
     """Result of or_run: labels and per-level trace."""
 
     VARIANTS = {"exp": 0, "local": 1, "shortcut": 2, "labels": 3}
+    PRE_MODES = {"none": 0, "local": 1, "single": 2, "double": 3}
 
     def __init__(self, offsets, targets, variant="exp", seed=1, beta=0.1, fma=True, seq_limit=4096, keep_keys=False,
-                 stream=0):
+                 stream=0, pre_contract="none", pre_blocks=1):
         off, tgt = _arrs(offsets, targets)
         self.n = len(off) - 1
         v = self.VARIANTS[variant] if isinstance(variant, str) else int(variant)
         self.h = lib().or_run(self.n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
-                              v, seed, beta, 1 if fma else 0, stream, seq_limit, 1 if keep_keys else 0)
+                              v, seed, beta, 1 if fma else 0, stream, seq_limit, 1 if keep_keys else 0,
+                              self.PRE_MODES[pre_contract] if isinstance(pre_contract, str) else int(pre_contract),
+                              pre_blocks)
 
     def labels(self):
         out = np.empty(max(self.n, 1), np.uint32)

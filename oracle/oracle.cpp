@@ -206,8 +206,40 @@ void contract(const Csr &g, const std::vector<uint32_t> &lab, std::vector<uint32
   for (uint32_t a = 0; a < k; ++a) next.off[a + 1] += next.off[a];
 }
 
+// Local components of a vertex-block distribution (FindLocalComponents,
+// exponential_contraction.h:392-411, with PE p holding the ids
+// [p*bs, (p+1)*bs)): the components of the edges inside a block, each
+// represented by its smallest member, as the reference's local BFS started
+// from its vertices in id order.  Interface vertices (a neighbour in another
+// block) keep themselves when only_inner is set: the default branch's
+// BuildLocalComponentGraph (cag_builder.h:160-230) contracts the others only;
+// -single's BuildComponentAdjacencyGraph contracts every vertex.
+std::vector<uint32_t> block_components(const Csr &g, uint64_t blocks, bool only_inner) {
+  const uint64_t bs = (g.n + blocks - 1) / blocks;
+  std::vector<uint32_t> parent(g.n);
+  for (uint64_t v = 0; v < g.n; ++v) parent[v] = static_cast<uint32_t>(v);
+  auto find = [&](uint32_t v) {
+    while (parent[v] != v) v = parent[v] = parent[parent[v]];
+    return v;
+  };
+  std::vector<char> inner(g.n, 1);
+  for (uint64_t v = 0; v < g.n; ++v)
+    for (uint64_t e = g.off[v]; e < g.off[v + 1]; ++e) {
+      const uint32_t u = g.tgt[e];
+      if (u / bs != v / bs) {
+        inner[v] = 0;
+        continue;
+      }
+      const uint32_t a = find(static_cast<uint32_t>(v)), b = find(u);
+      if (a != b) parent[std::max(a, b)] = std::min(a, b);  // the root stays the smallest member
+    }
+  std::vector<uint32_t> lab(g.n);
+  for (uint64_t v = 0; v < g.n; ++v) lab[v] = only_inner && !inner[v] ? static_cast<uint32_t>(v) : find(static_cast<uint32_t>(v));
+  return lab;
+}
+
 void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool fma, int stream,
-                     uint64_t seq_limit, bool keep_keys, or_result &res) {
+                     uint64_t seq_limit, bool keep_keys, int pre_mode, uint64_t pre_blocks, or_result &res) {
   // Level 0 holds the vertices of degree > 0 in id order: the reference sets
   // degree-0 vertices inactive at load (lib/io/graph_io.h:380) and its vertex
   // loops skip inactive vertices (lib/data_structures/dynamic_graph.h:113-117).
@@ -230,9 +262,13 @@ void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool
   uint32_t iteration = 1;
   std::vector<uint32_t> top_rep;
   bool have_top = false;
+  // pre-contraction levels: 1 local (default branch) or single, 2 for -contraction
+  int pre_left = pre_mode == 3 ? 2 : pre_mode != 0 ? 1 : 0;
   while (cur.n > 0) {
     const uint64_t m2 = cur.off[cur.n];
-    if (m2 == 0 || (seq_limit > 0 && m2 / 2 <= seq_limit)) {
+    const bool pre = pre_left > 0 && m2 > 0;
+    if (pre) --pre_left;
+    if (!pre && (m2 == 0 || (seq_limit > 0 && m2 / 2 <= seq_limit))) {
       // RunSequentialCC (exponential_contraction.h:908-922): components of the
       // remaining graph, represented by their smallest member.
       std::vector<uint32_t> l(cur.n);
@@ -246,7 +282,12 @@ void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool
     t.n = cur.n;
     t.m = m2 / 2;
     std::vector<uint64_t> key(cur.n);
-    if (variant == 1) {
+    if (pre) {
+      // the clusters are the block components; distance 0 everywhere
+      const std::vector<uint32_t> comp = block_components(cur, pre_blocks, pre_mode == 1);
+      for (uint64_t v = 0; v < cur.n; ++v) key[v] = comp[v];
+      t.sweeps = 0;
+    } else if (variant == 1) {
       // LocalContraction::RunContraction (local_contraction.h:157-300): uniform
       // deviates, min over N[v], then min over N[N[v]], both synchronous.
       for (uint64_t v = 0; v < cur.n; ++v)
@@ -269,7 +310,7 @@ void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool
     t.lab.resize(cur.n);
     for (uint64_t v = 0; v < cur.n; ++v) {
       t.lab[v] = static_cast<uint32_t>(key[v]);
-      if (variant != 1) {
+      if (variant != 1 && !pre) {
         const uint32_t r = static_cast<uint32_t>(key[v] >> 32) - static_cast<uint32_t>(key[t.lab[v]] >> 32);
         t.radius = std::max(t.radius, r);
       }
@@ -285,7 +326,7 @@ void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool
     res.levels.push_back(std::move(t));
     cur = std::move(next);
     gid.swap(gid_next);
-    ++iteration;
+    if (!pre) ++iteration;
   }
   // UndoContraction (dynamic_contraction.h:568-658): every vertex takes the
   // representative of its centre one level up, or the centre itself when the
@@ -367,7 +408,8 @@ void or_lp_closed_form(uint64_t n, const uint64_t *off, const uint32_t *tgt, con
 }
 
 or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t variant, int32_t seed, double beta,
-                  int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys) {
+                  int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys, int32_t pre_mode,
+                  uint64_t pre_blocks) {
   Csr g;
   g.n = n;
   g.off.assign(off, off + n + 1);
@@ -375,7 +417,8 @@ or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t 
   or_result *r = new or_result();
   const auto t0 = std::chrono::steady_clock::now();
   if (variant == 0 || variant == 1)
-    run_contraction(g, variant, seed, beta, fma != 0, stream, seq_limit, keep_keys != 0, *r);
+    run_contraction(g, variant, seed, beta, fma != 0, stream, seq_limit, keep_keys != 0, pre_mode,
+                    pre_blocks ? pre_blocks : 1, *r);
   else
     run_propagation(g, variant == 2, *r);
   r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -54,9 +54,13 @@ void or_lp_closed_form(uint64_t n, const uint64_t *off, const uint32_t *tgt, con
 
 /* Full runs.  variant: 0 exp, 1 local, 2 shortcut, 3 labels (kc_variant). */
 typedef struct or_result or_result;
-/* stream: 0 reference draw, 1 hashed draw (the library's KC_STREAM_*). */
+/* stream: 0 reference draw, 1 hashed draw (the library's KC_STREAM_*).
+ * pre_mode: 0 none, 1 local, 2 single, 3 double (the library's KC_PRE_*): the
+ * components inside pre_blocks blocks of consecutive level indices contracted
+ * first (levels with distance-0 keys, sweeps 0, radius 0). */
 or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t variant, int32_t seed, double beta,
-                  int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys);
+                  int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys, int32_t pre_mode,
+                  uint64_t pre_blocks);
 void or_free(or_result *r);
 void or_labels(const or_result *r, uint32_t *labels);
 uint32_t or_levels(const or_result *r);

@@ -430,3 +430,60 @@ def test_device_binary_reader_matches_host(tmp_path):
     with pytest.raises(ca.KcError) as e:
         ca.DeviceGraph.read_binary(bad)
     assert e.value.code == ca.KC_ERR_INVALID
+
+
+# -single / -contraction / the default branch's local contraction, over a vertex-block distribution
+@pytest.mark.parametrize("bfs", [False, True])
+@pytest.mark.parametrize("blocks", [1, 5, 64, 1 << 20])
+@pytest.mark.parametrize("mode", ["local", "single", "double"])
+def test_pre_contraction_trace_matches_oracle(mode, blocks, bfs):
+    graphs = [ca.generate("gnm_undirected", n=1 << 14, m=1 << 15, seed=3),
+              ca.generate("rmat", n=1 << 13, m=1 << 16, seed=3)]
+    for g in graphs:
+        print(f"pre_contract={mode} blocks={blocks} bfs={bfs} n={g.n}", flush=True)
+        run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1005, seq_limit=64, keep_keys=True,
+                         stream=ca.STREAM_HASHED, pre_contract=mode, pre_blocks=blocks)
+        dg = ca.DeviceGraph(g)
+        lab, st = dg.find_components(variant="exp", seed=1005, seq_limit=64, record_trace=True,
+                                     stream=ca.STREAM_HASHED, use_bfs=bfs, pre_contract=mode, pre_blocks=blocks)
+        tr = dg.trace()
+        dg.close()
+        want_levels = run.levels()
+        assert len(tr) == len(want_levels)
+        n_pre = {"local": 1, "single": 1, "double": 2}[mode]
+        for lvl, ((info, keys, gid), winfo) in enumerate(zip(tr, want_levels)):
+            # sizes and radius are order-independent; sweep counts are not (in-place sweeps)
+            assert (info[0], info[1], info[3]) == (winfo[0], winfo[1], winfo[3]), f"level {lvl} info"
+            if lvl < n_pre:
+                assert info[2] == 0 and info[3] == 0, f"pre-contraction level {lvl}"
+            wkeys, wgid = run.level_keys(lvl)
+            assert np.array_equal(gid, wgid), f"level {lvl} gid"
+            assert np.array_equal(keys, wkeys), f"level {lvl} keys"
+        assert np.array_equal(lab, run.labels())
+        assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets))
+        assert st["radius_sum"] == sum(w[3] for w in want_levels)
+
+
+def test_pre_contraction_option_checks():
+    g = ca.generate("gnm_undirected", n=1 << 10, m=1 << 11, seed=1)
+    dg = ca.DeviceGraph(g)
+    try:
+        with pytest.raises(ca.KcError):
+            dg.find_components(pre_contract=7, pre_blocks=4)
+        with pytest.raises(ca.KcError):
+            dg.find_components(pre_contract="single", pre_blocks=0)
+        with pytest.raises(ca.KcError):
+            dg.find_components(variant="labels", pre_contract="single", pre_blocks=4)
+        lab, _ = dg.find_components(variant="local", pre_contract="double", pre_blocks=4)
+        assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets))
+    finally:
+        dg.close()
+
+
+def test_app_single_and_contraction_flags():
+    exe = os.path.join(ca.BIN_DIR, "exp_contraction")
+    for flag in ("-single", "-contraction"):
+        r = subprocess.run([exe, "-gen", "rmat", "-n", "15", "-m", "19", "-i", "2", "-pes", "8", flag, "-check"],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "CHECK labels_bit_exact=yes" in r.stdout, flag

@@ -189,3 +189,70 @@ def test_oracle_exp_levels_shrink_and_radius():
     hops = (keys >> np.uint64(32)).astype(np.int64) - (keys[lab] >> np.uint64(32)).astype(np.int64)
     assert hops.max() == lv[0][3]
     assert np.all(lab[lab] == lab)  # centres are their own centre
+
+
+def _level0(g):
+    """Level 0 of the contraction: the vertices of degree > 0, renumbered in id order."""
+    off = np.asarray(g.offsets, np.int64)
+    deg = np.diff(off)
+    keep = np.flatnonzero(deg > 0)
+    newid = np.full(len(deg), -1, np.int64)
+    newid[keep] = np.arange(len(keep))
+    tgt = newid[np.asarray(g.targets, np.int64)]
+    return np.concatenate([[0], np.cumsum(deg[keep])]).astype(np.uint64), tgt.astype(np.uint32)
+
+
+def _block_reference(off, tgt, blocks):
+    """Per vertex: smallest member of its component over the edges inside its block
+    (scipy's connected_components on those edges), and whether it has an edge leaving its block."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    n = len(off) - 1
+    bs = -(-n // blocks)
+    src = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+    dst = tgt.astype(np.int64)
+    inside = src // bs == dst // bs
+    a = sp.coo_matrix((np.ones(int(inside.sum())), (src[inside], dst[inside])), shape=(n, n))
+    _, comp = connected_components(a, directed=False)
+    smallest = np.full(comp.max() + 1, n, np.int64)
+    np.minimum.at(smallest, comp, np.arange(n))
+    leaves = np.zeros(n, bool)
+    leaves[src[~inside]] = True
+    return smallest[comp], leaves, src, dst
+
+
+@pytest.mark.parametrize("mode", ["local", "single", "double"])
+def test_oracle_pre_contraction_pinned(mode):
+    """The pre-contraction levels of the oracle against closed forms: labels equal union-find
+    for every block count; one block leaves no edge (the contracted graph is empty); a block per
+    vertex contracts nothing; the level-0 clusters equal scipy's components inside the blocks
+    (interface vertices on their own in local mode), and the contracted graph's sizes equal the
+    distinct cluster pairs."""
+    graphs = [ca.generate("gnm_undirected", n=1 << 11, m=1 << 12, seed=7),
+              ca.generate("grid_2d", nx=40, ny=30, p=0.6, seed=2),
+              ca.generate("rmat", n=1 << 11, m=1 << 14, seed=3)]
+    for g in graphs:
+        ref = oracle.canonical_labels(g.offsets, g.targets)
+        off, tgt = _level0(g)
+        n0, m0 = len(off) - 1, int(off[-1]) // 2
+        for blocks in (1, 3, 16, n0):
+            run = oracle.Run(g.offsets, g.targets, seed=3, stream=1, seq_limit=0, keep_keys=True,
+                             pre_contract=mode, pre_blocks=blocks)
+            assert np.array_equal(run.labels(), ref), (mode, blocks)
+            lv = run.levels()
+            assert lv[0] == (n0, m0, 0, 0)
+            if blocks == 1 and mode != "local":
+                assert len(lv) == 1
+            if blocks == 1 and mode == "local":
+                assert len(lv) == 1  # no vertex has a neighbour in another block
+            if blocks == n0:
+                assert lv[1][:2] == (n0, m0)
+            keys, _ = run.level_keys(0)
+            assert np.all(keys >> np.uint64(32) == 0)
+            smallest, leaves, src, dst = _block_reference(off, tgt, blocks)
+            want = np.where(leaves, np.arange(n0), smallest) if mode == "local" else smallest
+            assert np.array_equal(keys.astype(np.int64), want), (mode, blocks)
+            cl = want
+            pairs = np.unique(np.stack([cl[src], cl[dst]], 1)[cl[src] != cl[dst]], axis=0)
+            if len(pairs):
+                assert lv[1][0] == len(np.unique(pairs[:, 0])) and lv[1][1] == len(pairs) // 2
