@@ -148,6 +148,7 @@ _SIGNATURES = {
     "kc_graph_generate": (ctypes.c_int, [ctypes.c_char_p, _P(GenParams), _P(ctypes.c_void_p)]),
     "kc_graph_download": (ctypes.c_int, [ctypes.c_void_p, _P(Csr)]),
     "kc_graph_read_binary": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
+    "kc_graph_read_metis": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
     "kc_batch_gen_params_init": (None, [_P(BatchGenParams)]),
     "kc_batch_generate": (ctypes.c_int, [_P(BatchGenParams), _P(BatchCsr)]),
     "kc_batch_free": (None, [_P(BatchCsr)]),
@@ -385,6 +386,13 @@ class DeviceGraph:
         """kc_graph_read_binary: a sorted binary graph file parsed and built in HBM."""
         h = ctypes.c_void_p()
         _check(lib().kc_graph_read_binary(str(path).encode(), ctypes.byref(h)))
+        return DeviceGraph(_handle=h)
+
+    @staticmethod
+    def read_metis(path):
+        """kc_graph_read_metis: a METIS file parsed and built in HBM."""
+        h = ctypes.c_void_p()
+        _check(lib().kc_graph_read_metis(str(path).encode(), ctypes.byref(h)))
         return DeviceGraph(_handle=h)
 
     def download(self):

@@ -13,7 +13,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cmath>
 #include <cstdarg>
@@ -1341,6 +1343,92 @@ int kc_graph_read_binary(const char *path, kc_graph **out) {
       if (h_bad) fail(KC_ERR_INVALID, "%s: an edge lies outside [1,%llu]", path, (unsigned long long)n);
       g->n = n;
       build_device_csr(g.get(), n, src.as<uint32_t>(), dst.as<uint32_t>(), m);
+    }
+    init_work(g.get());
+    *out = g.release();
+  });
+}
+
+int kc_graph_read_metis(const char *path, kc_graph **out) {
+  return guarded([&] {
+    if (!path || !out) fail(KC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<FILE, int (*)(FILE *)> fp(std::fopen(path, "rb"), std::fclose);
+    if (!fp) fail(KC_ERR_IO, "Error opening %s", path);
+    std::fseek(fp.get(), 0, SEEK_END);
+    const long size = std::ftell(fp.get());
+    std::fseek(fp.get(), 0, SEEK_SET);
+    if (size < 0) fail(KC_ERR_IO, "%s: cannot size the file", path);
+    const uint64_t fsize = static_cast<uint64_t>(size);
+    // the whole file in pinned memory: one copy to HBM
+    void *pinned = nullptr;
+    KC_HIP(hipHostMalloc(&pinned, fsize + 1));
+    std::unique_ptr<void, hipError_t (*)(void *)> pin(pinned, hipHostFree);
+    char *text = static_cast<char *>(pinned);
+    if (fsize && std::fread(text, 1, fsize, fp.get()) != fsize) fail(KC_ERR_IO, "%s: read failed", path);
+    text[fsize] = '\0';
+    // header: the first line not starting with '%' (its first number is n)
+    uint64_t p = 0, hs = 0, he = 0;
+    bool have = false;
+    while (p < fsize) {
+      const char *nl = static_cast<const char *>(std::memchr(text + p, '\n', fsize - p));
+      const uint64_t e = nl ? static_cast<uint64_t>(nl - text) : fsize;
+      if (text[p] != '%' || e == p) {
+        hs = p;
+        he = e;
+        have = true;
+        p = nl ? e + 1 : fsize;
+        break;
+      }
+      p = nl ? e + 1 : fsize;
+    }
+    if (!have) fail(KC_ERR_IO, "%s: missing header", path);
+    const std::string header(text + hs, text + he);
+    char *q = nullptr;
+    errno = 0;
+    const unsigned long long n = std::strtoull(header.c_str(), &q, 10);
+    if (q == header.c_str() || errno) fail(KC_ERR_IO, "%s: malformed header '%s'", path, header.c_str());
+    if (n >= 0xFFFFFFFFull) fail(KC_ERR_INVALID, "%s: n=%llu exceeds 32-bit vertex ids", path, n);
+    const uint64_t bytes = fsize - p;
+    std::unique_ptr<kc_graph> g = new_handle();
+    hipStream_t s = g->stream;
+    {
+      DevTemp body(bytes + 64), scratch((scan_scratch_words(bytes + 1) + 1024) * 8), total(64), bad(64);
+      const uint8_t *b = body.as<uint8_t>();
+      if (bytes) KC_HIP(hipMemcpyAsync(body.p, text + p, bytes, hipMemcpyHostToDevice, s));
+      uint64_t *sc = scratch.as<uint64_t>(), *tot = total.as<uint64_t>();
+      // lines: positions of the newlines; a last line without one counts too
+      const uint64_t nl = byte_positions(b, bytes, kBytesNewline, nullptr, tot, sc, s);
+      DevTemp nlpos(nl * 8 + 64);
+      byte_positions(b, bytes, kBytesNewline, nlpos.as<uint64_t>(), tot, sc, s);
+      const uint64_t lines = nl + (bytes && text[fsize - 1] != '\n' ? 1 : 0);
+      DevTemp vline(lines * 4 + 64);
+      metis_vertex_lines(b, bytes, nlpos.as<uint64_t>(), lines, n, vline.as<uint32_t>(), tot, sc, s);
+      fill_u32(vline.as<uint32_t>() + lines, 1, kNone, s);  // past the last line (not reached by a number)
+      // numbers, and bytes that are neither digits nor blanks
+      const uint64_t count = byte_positions(b, bytes, kBytesToken, nullptr, tot, sc, s);
+      const uint64_t other = byte_positions(b, bytes, kBytesOther, nullptr, tot, sc, s);
+      fill_u32(bad.as<uint32_t>(), 1, 0, s);
+      if (other) {
+        DevTemp opos(other * 8 + 64);
+        byte_positions(b, bytes, kBytesOther, opos.as<uint64_t>(), tot, sc, s);
+        metis_other(opos.as<uint64_t>(), other, nlpos.as<uint64_t>(), nl, vline.as<uint32_t>(), bad.as<int>(), s);
+        KC_HIP(hipStreamSynchronize(s));
+      }
+      DevTemp tokpos(count * 8 + 64), tv(count * 4 + 64), tval(count * 4 + 64);
+      byte_positions(b, bytes, kBytesToken, tokpos.as<uint64_t>(), tot, sc, s);
+      metis_tokens(b, bytes, tokpos.as<uint64_t>(), count, nlpos.as<uint64_t>(), nl, vline.as<uint32_t>(), n,
+                   tv.as<uint32_t>(), tval.as<uint32_t>(), bad.as<int>(), s);
+      int h_bad = 0;
+      KC_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      KC_HIP(hipStreamSynchronize(s));
+      if (h_bad & 2) fail(KC_ERR_IO, "%s: a vertex line holds a character other than digits and blanks", path);
+      if (h_bad & 1) fail(KC_ERR_INVALID, "%s: a neighbour id is 0 or outside [1,%llu]", path, n);
+      DevTemp src(count * 4 + 64), dst(count * 4 + 64);
+      const uint64_t edges = metis_edges(count, tv.as<uint32_t>(), tval.as<uint32_t>(), src.as<uint32_t>(),
+                                         dst.as<uint32_t>(), tot, sc, s);
+      g->n = n;
+      build_device_csr(g.get(), n, src.as<uint32_t>(), dst.as<uint32_t>(), edges);
     }
     init_work(g.get());
     *out = g.release();

@@ -262,4 +262,26 @@ void edge_scatter(uint64_t count, const uint32_t *src, const uint32_t *dst, cons
                   uint32_t *bucket, hipStream_t s);
 void nonzero_rows(uint64_t n, const uint64_t *off, unsigned long long *counter, hipStream_t s);
 
+
+// METIS text on the device.  byte_positions: the positions i < bytes of one
+// byte class in increasing order into pos (null: count only); returns the
+// count.  metis_vertex_lines: vline[j] = index of line j among the vertex
+// lines (not starting with '%'), kNone for comments and lines past the n-th.
+// metis_tokens: per number its vertex (kNone off the vertex lines) and value,
+// *bad |= 1 for 0, > n or > 10 digits on a vertex line; metis_other: *bad |= 2
+// for a byte other than digits and blanks on a vertex line; metis_edges: the
+// (vertex, value - 1) pairs of the kept numbers in order, returns their count.
+constexpr int kBytesNewline = 0, kBytesToken = 1, kBytesOther = 2;
+uint64_t byte_positions(const uint8_t *b, uint64_t bytes, int what, uint64_t *pos, uint64_t *total, uint64_t *scratch,
+                        hipStream_t s);
+void metis_vertex_lines(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t lines, uint64_t n,
+                        uint32_t *vline, uint64_t *total, uint64_t *scratch, hipStream_t s);
+void metis_tokens(const uint8_t *b, uint64_t bytes, const uint64_t *tokpos, uint64_t count, const uint64_t *nlpos,
+                  uint64_t nl, const uint32_t *vline, uint64_t n, uint32_t *tv, uint32_t *tval, int *bad,
+                  hipStream_t s);
+void metis_other(const uint64_t *otherpos, uint64_t count, const uint64_t *nlpos, uint64_t nl, const uint32_t *vline,
+                 int *bad, hipStream_t s);
+uint64_t metis_edges(uint64_t count, const uint32_t *tv, const uint32_t *tval, uint32_t *src, uint32_t *dst,
+                     uint64_t *total, uint64_t *scratch, hipStream_t s);
+
 }  // namespace kc

@@ -1620,6 +1620,112 @@ __global__ __launch_bounds__(kBlock) void k_reduce_sum(const uint32_t *partial, 
   if (threadIdx.x == 0) *out = static_cast<uint32_t>(tot);
 }
 
+
+// ---------------------------------------------------------------------------
+// METIS text parsed on the device (ReadMETISFile, lib/io/graph_io.h:281-382,
+// ParseVertexFilestream :1207-1248): the bytes after the header line are
+// split at '\n'; lines starting with '%' are comments; the k-th other line
+// lists the 1-based neighbours of vertex k, for k < n.
+__device__ __forceinline__ bool byte_digit(uint8_t c) { return c >= '0' && c <= '9'; }
+__device__ __forceinline__ bool byte_blank(uint8_t c) {
+  return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f' || c == '\n';
+}
+
+struct ByteClassLoad {
+  const uint8_t *b;
+  int what;  // kBytesNewline, kBytesToken (first digit of a number), kBytesOther (neither digit nor blank)
+  __device__ uint64_t operator()(uint64_t i) const {
+    const uint8_t c = b[i];
+    if (what == kBytesNewline) return c == '\n' ? 1ull : 0ull;
+    if (what == kBytesToken) return byte_digit(c) && (i == 0 || !byte_digit(b[i - 1])) ? 1ull : 0ull;
+    return !byte_digit(c) && !byte_blank(c) ? 1ull : 0ull;
+  }
+};
+struct PosStore {
+  uint64_t *pos;  // null: count only
+  __device__ void operator()(uint64_t i, uint64_t prefix, uint64_t v) const {
+    if (v && pos) pos[prefix] = i;
+  }
+};
+
+// Line j starts after the j-th newline; it is a vertex line unless its first
+// byte is '%' (an empty line is a vertex without neighbours).
+struct VertexLineLoad {
+  const uint8_t *b;
+  const uint64_t *nlpos;
+  uint64_t bytes;
+  __device__ uint64_t operator()(uint64_t j) const {
+    const uint64_t st = j ? nlpos[j - 1] + 1 : 0;
+    return st < bytes && b[st] == '%' ? 0ull : 1ull;
+  }
+};
+struct VertexLineStore {
+  uint32_t *vline;
+  uint64_t n;
+  __device__ void operator()(uint64_t j, uint64_t prefix, uint64_t v) const {
+    vline[j] = v && prefix < n ? static_cast<uint32_t>(prefix) : kNone;
+  }
+};
+
+// Line of byte position pos: the number of newlines before it.
+__device__ __forceinline__ uint64_t line_of(const uint64_t *nlpos, uint64_t nl, uint64_t pos) {
+  uint64_t lo = 0, hi = nl;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (nlpos[mid] < pos)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Every number: its vertex (kNone outside the first n vertex lines) and its
+// value; *bad |= 1 for a value of 0, above n or of more than 10 digits on a
+// vertex line.
+__global__ void k_metis_tokens(const uint8_t *b, uint64_t bytes, const uint64_t *tokpos, uint64_t count,
+                               const uint64_t *nlpos, uint64_t nl, const uint32_t *vline, uint64_t n, uint32_t *tv,
+                               uint32_t *tval, int *bad) {
+  const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  bool wrong = false;
+  if (t < count) {
+    const uint64_t pos = tokpos[t];
+    const uint32_t v = vline[line_of(nlpos, nl, pos)];
+    uint64_t x = 0;
+    if (v != kNone) {
+      int d = 0;
+      for (uint64_t p = pos; p < bytes && byte_digit(b[p]) && d < 11; ++p, ++d) x = x * 10 + (b[p] - '0');
+      wrong = d > 10 || x == 0 || x > n;
+    }
+    tv[t] = v;
+    tval[t] = wrong ? 0u : static_cast<uint32_t>(x);
+  }
+  if (__ballot(wrong) != 0 && lane_id() == 0) atomicOr(bad, 1);
+}
+
+// *bad |= 2 when a byte that is neither digit nor blank lies on a vertex line.
+__global__ void k_metis_other(const uint64_t *otherpos, uint64_t count, const uint64_t *nlpos, uint64_t nl,
+                              const uint32_t *vline, int *bad) {
+  const uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const bool wrong = k < count && vline[line_of(nlpos, nl, otherpos[k])] != kNone;
+  if (__ballot(wrong) != 0 && lane_id() == 0) atomicOr(bad, 2);
+}
+
+struct EdgeKeepLoad {
+  const uint32_t *tv;
+  __device__ uint64_t operator()(uint64_t t) const { return tv[t] != kNone ? 1ull : 0ull; }
+};
+struct EdgeStore {
+  const uint32_t *tv, *tval;
+  uint32_t *src, *dst;
+  __device__ void operator()(uint64_t t, uint64_t prefix, uint64_t v) const {
+    if (v) {
+      src[prefix] = tv[t];
+      dst[prefix] = tval[t] - 1;
+    }
+  }
+};
+
 }  // namespace
 
 // ===========================================================================
@@ -2183,6 +2289,47 @@ void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_
   KC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partial, static_cast<uint64_t>(g), roots);
   KC_LAUNCH_CHECK();
+}
+
+
+uint64_t byte_positions(const uint8_t *b, uint64_t bytes, int what, uint64_t *pos, uint64_t *total, uint64_t *scratch,
+                        hipStream_t s) {
+  scan_exclusive<uint64_t>(bytes, ByteClassLoad{b, what}, PosStore{pos}, total, scratch, s);
+  uint64_t h = 0;
+  KC_HIP(hipMemcpyAsync(&h, total, sizeof(h), hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void metis_vertex_lines(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t lines, uint64_t n,
+                        uint32_t *vline, uint64_t *total, uint64_t *scratch, hipStream_t s) {
+  scan_exclusive<uint64_t>(lines, VertexLineLoad{b, nlpos, bytes}, VertexLineStore{vline, n}, total, scratch, s);
+}
+
+void metis_tokens(const uint8_t *b, uint64_t bytes, const uint64_t *tokpos, uint64_t count, const uint64_t *nlpos,
+                  uint64_t nl, const uint32_t *vline, uint64_t n, uint32_t *tv, uint32_t *tval, int *bad,
+                  hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_metis_tokens, dim3(static_cast<unsigned>(ceil_div(count, static_cast<uint64_t>(kBlock)))),
+                     dim3(kBlock), 0, s, b, bytes, tokpos, count, nlpos, nl, vline, n, tv, tval, bad);
+  KC_LAUNCH_CHECK();
+}
+
+void metis_other(const uint64_t *otherpos, uint64_t count, const uint64_t *nlpos, uint64_t nl, const uint32_t *vline,
+                 int *bad, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_metis_other, dim3(static_cast<unsigned>(ceil_div(count, static_cast<uint64_t>(kBlock)))),
+                     dim3(kBlock), 0, s, otherpos, count, nlpos, nl, vline, bad);
+  KC_LAUNCH_CHECK();
+}
+
+uint64_t metis_edges(uint64_t count, const uint32_t *tv, const uint32_t *tval, uint32_t *src, uint32_t *dst,
+                     uint64_t *total, uint64_t *scratch, hipStream_t s) {
+  scan_exclusive<uint64_t>(count, EdgeKeepLoad{tv}, EdgeStore{tv, tval, src, dst}, total, scratch, s);
+  uint64_t h = 0;
+  KC_HIP(hipMemcpyAsync(&h, total, sizeof(h), hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  return h;
 }
 
 }  // namespace kc

@@ -232,6 +232,14 @@ int kc_graph_generate(const char *name, const kc_gen_params *p, kc_graph **out);
  * inside a row is not fixed).  GraphIO::ReadSortedBinaryFile
  * (lib/io/graph_io.h:910-1175). */
 int kc_graph_read_binary(const char *path, kc_graph **out);
+/* Reads a METIS graph file into a device graph: the header is read on the
+ * host, the rest of the file is copied to HBM once and parsed there (lines,
+ * comments, numbers), and the CSR is built on the device (same rows as
+ * kc_read_metis; the order inside a row is not fixed).  A vertex line may hold
+ * only digits and blanks: a sign, a letter or a 0 on one is an error here
+ * (kc_read_metis stops the line there).  GraphIO::ReadMETISFile
+ * (lib/io/graph_io.h:281-382) with ParseVertexFilestream (:1207-1248). */
+int kc_graph_read_metis(const char *path, kc_graph **out);
 /* Copies the level-0 CSR of a graph to host memory (free with kc_csr_free). */
 int kc_graph_download(const kc_graph *g, kc_csr *out);
 

@@ -432,6 +432,56 @@ def test_device_binary_reader_matches_host(tmp_path):
     assert e.value.code == ca.KC_ERR_INVALID
 
 
+def _sorted_rows(g):
+    off = g.offsets.astype(np.int64)
+    return [sorted(g.targets[off[v]:off[v + 1]].tolist()) for v in range(g.n)]
+
+
+def test_device_metis_reader_matches_host(tmp_path):
+    """kc_graph_read_metis against kc_read_metis on written graphs and on hand-made files with
+    comments, CR line ends, blank runs, a missing last newline, fewer vertex lines than n,
+    lines past the n-th (ignored, whatever they hold) and an empty file body."""
+    files = []
+    for name, kw in (("rmat", dict(n=1 << 14, m=1 << 18, seed=5)), ("gnm_undirected", dict(n=5000, m=3000, seed=2)),
+                     ("grid_2d", dict(nx=60, ny=50, p=0.5, seed=1))):
+        p = tmp_path / f"{name}.graph"
+        ca.write_metis(p, ca.generate(name, **kw))
+        files.append(p)
+    hand = {
+        "comments.graph": "% a comment\n%another 1 2 x\n6 7\n2 3\n% inside\n1 3\n1 2\n5  6\n4\t6\n4 5",
+        "crlf.graph": "4 3\r\n2\r\n1 3 \r\n2 4\r\n3\r\n",
+        "short.graph": "10 2\n2\n1\n\n5\n4\n",
+        "extra.graph": "3 1\n2\n1\n\nthis line is past n: 7 x 0\n9 9 9\n",
+        "nobody.graph": "% only a header\n5 0",
+        "big_ids.graph": "3 2\n0000003\n\n1\n",
+    }
+    for fname, text in hand.items():
+        p = tmp_path / fname
+        p.write_bytes(text.encode())
+        files.append(p)
+    files.append(os.path.join(GOLDEN, "simple.graph"))
+    for p in files:
+        print(f"metis file {os.path.basename(str(p))}", flush=True)
+        host = ca.read_metis(p)
+        dg = ca.DeviceGraph.read_metis(p)
+        try:
+            dev = dg.download()
+            assert dev.n == host.n and np.array_equal(dev.offsets, host.offsets), p
+            assert _sorted_rows(dev) == _sorted_rows(host), p
+            lab, _ = dg.find_components(variant="exp", seed=1001)
+            assert np.array_equal(lab, oracle.canonical_labels(host.offsets, host.targets)), p
+        finally:
+            dg.close()
+    for text, code in (("3 1\n2\n1 x\n\n", ca.KC_ERR_IO), ("3 1\n2\n1 -3\n\n", ca.KC_ERR_IO),
+                       ("3 1\n2\n1 0\n\n", ca.KC_ERR_INVALID), ("3 1\n4\n\n\n", ca.KC_ERR_INVALID),
+                       ("3 1\n99999999999\n\n\n", ca.KC_ERR_INVALID), ("% nothing\n", ca.KC_ERR_IO)):
+        p = tmp_path / "bad.graph"
+        p.write_bytes(text.encode())
+        with pytest.raises(ca.KcError) as e:
+            ca.DeviceGraph.read_metis(p)
+        assert e.value.code == code, text
+
+
 # -single / -contraction / the default branch's local contraction, over a vertex-block distribution
 @pytest.mark.parametrize("bfs", [False, True])
 @pytest.mark.parametrize("blocks", [1, 5, 64, 1 << 20])
