@@ -149,6 +149,8 @@ _SIGNATURES = {
     "kc_graph_download": (ctypes.c_int, [ctypes.c_void_p, _P(Csr)]),
     "kc_graph_read_binary": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
     "kc_graph_read_metis": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
+    "kc_graph_read_edgefile": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
+    "kc_graph_read_partitioned_edgefile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P(ctypes.c_void_p)]),
     "kc_batch_gen_params_init": (None, [_P(BatchGenParams)]),
     "kc_batch_generate": (ctypes.c_int, [_P(BatchGenParams), _P(BatchCsr)]),
     "kc_batch_free": (None, [_P(BatchCsr)]),
@@ -393,6 +395,17 @@ class DeviceGraph:
         """kc_graph_read_metis: a METIS file parsed and built in HBM."""
         h = ctypes.c_void_p()
         _check(lib().kc_graph_read_metis(str(path).encode(), ctypes.byref(h)))
+        return DeviceGraph(_handle=h)
+
+    @staticmethod
+    def read_edgefile(path, parts=None):
+        """kc_graph_read_edgefile (parts None) or kc_graph_read_partitioned_edgefile: edge-list text
+        parsed and built in HBM."""
+        h = ctypes.c_void_p()
+        if parts is None:
+            _check(lib().kc_graph_read_edgefile(str(path).encode(), ctypes.byref(h)))
+        else:
+            _check(lib().kc_graph_read_partitioned_edgefile(str(path).encode(), int(parts), ctypes.byref(h)))
         return DeviceGraph(_handle=h)
 
     def download(self):

@@ -1349,6 +1349,158 @@ int kc_graph_read_binary(const char *path, kc_graph **out) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Edges of one sorted edge file, parsed on the device.
+struct EdgeTextPart {
+  std::unique_ptr<DevTemp> src, dst;
+  uint64_t count = 0;
+};
+
+// Next whitespace-separated number of [p, e) (spaces and tabs only), as the
+// host reader takes it; false when missing or malformed.
+bool header_number(const char *&p, const char *e, unsigned long long &v) {
+  while (p < e && (*p == ' ' || *p == '\t')) ++p;
+  if (p >= e || *p < '0' || *p > '9') return false;
+  unsigned long long x = 0;
+  for (; p < e && *p >= '0' && *p <= '9'; ++p) {
+    const unsigned d = static_cast<unsigned>(*p - '0');
+    if (x > (~0ull - d) / 10) return false;
+    x = x * 10 + d;
+  }
+  v = x;
+  return true;
+}
+
+// ReadSortedEdgeFile (lib/io/graph_io.h:385-669): header "<type> n m" (checked
+// against n and m when check is set), then the "<type> u v" lines, at most m
+// of them when bounded.  The file goes to HBM in one copy and is parsed there.
+void device_edge_file(kc_graph *g, const char *path, bool bounded, bool check, uint64_t &n, uint64_t &m,
+                      EdgeTextPart &out) {
+  std::unique_ptr<FILE, int (*)(FILE *)> fp(std::fopen(path, "rb"), std::fclose);
+  if (!fp) fail(KC_ERR_IO, "Error opening %s", path);
+  std::fseek(fp.get(), 0, SEEK_END);
+  const long size = std::ftell(fp.get());
+  std::fseek(fp.get(), 0, SEEK_SET);
+  if (size < 0) fail(KC_ERR_IO, "%s: cannot size the file", path);
+  const uint64_t fsize = static_cast<uint64_t>(size);
+  void *pinned = nullptr;
+  KC_HIP(hipHostMalloc(&pinned, fsize + 1));
+  std::unique_ptr<void, hipError_t (*)(void *)> pin(pinned, hipHostFree);
+  char *text = static_cast<char *>(pinned);
+  if (fsize && std::fread(text, 1, fsize, fp.get()) != fsize) fail(KC_ERR_IO, "%s: read failed", path);
+  // header: the first line that is empty or does not start with '%'
+  uint64_t p = 0, hs = 0, he = 0;
+  bool have = false;
+  while (p < fsize) {
+    const char *nl = static_cast<const char *>(std::memchr(text + p, '\n', fsize - p));
+    uint64_t e = nl ? static_cast<uint64_t>(nl - text) : fsize;
+    const uint64_t next = nl ? e + 1 : fsize;
+    if (e > p && text[e - 1] == '\r') --e;
+    if (e == p || text[p] != '%') {
+      hs = p;
+      he = e;
+      have = true;
+      p = next;
+      break;
+    }
+    p = next;
+  }
+  if (!have) fail(KC_ERR_IO, "%s: missing header", path);
+  const char *h = text + hs, *he_p = text + he;
+  while (h < he_p && (*h == ' ' || *h == '\t')) ++h;
+  while (h < he_p && *h != ' ' && *h != '\t') ++h;
+  unsigned long long hn = 0, hm = 0;
+  if (!header_number(h, he_p, hn) || !header_number(h, he_p, hm))
+    fail(KC_ERR_IO, "%s: malformed header '%.*s' (expected '<type> n m')", path, static_cast<int>(he - hs), text + hs);
+  if (hn >= 0xFFFFFFFFull) fail(KC_ERR_INVALID, "%s: n=%llu exceeds 32-bit vertex ids", path, hn);
+  if (check && (hn != n || hm != m))
+    fail(KC_ERR_INVALID, "%s: header n=%llu m=%llu differs from the first part's n=%llu m=%llu", path, hn, hm,
+         static_cast<unsigned long long>(n), static_cast<unsigned long long>(m));
+  n = hn;
+  m = hm;
+  const uint64_t bytes = fsize - p;
+  hipStream_t s = g->stream;
+  DevTemp body(bytes + 64), scratch((scan_scratch_words(bytes + 1) + 1024) * 8), total(64), bad(64);
+  const uint8_t *b = body.as<uint8_t>();
+  if (bytes) KC_HIP(hipMemcpyAsync(body.p, text + p, bytes, hipMemcpyHostToDevice, s));
+  uint64_t *sc = scratch.as<uint64_t>(), *tot = total.as<uint64_t>();
+  const uint64_t nl = byte_positions(b, bytes, kBytesNewline, nullptr, tot, sc, s);
+  DevTemp nlpos(nl * 8 + 64);
+  byte_positions(b, bytes, kBytesNewline, nlpos.as<uint64_t>(), tot, sc, s);
+  const uint64_t lines = nl + (bytes && text[fsize - 1] != '\n' ? 1 : 0);
+  uint64_t count = edge_text_lines(b, bytes, nlpos.as<uint64_t>(), nl, lines, nullptr, tot, sc, s);
+  DevTemp eline(count * 8 + 64);
+  edge_text_lines(b, bytes, nlpos.as<uint64_t>(), nl, lines, eline.as<uint64_t>(), tot, sc, s);
+  if (bounded && count > m) count = m;
+  out.src = std::make_unique<DevTemp>(count * 4 + 64);
+  out.dst = std::make_unique<DevTemp>(count * 4 + 64);
+  out.count = count;
+  fill_u32(bad.as<uint32_t>(), 1, 0, s);
+  edge_text(b, bytes, nlpos.as<uint64_t>(), nl, eline.as<uint64_t>(), count, n, out.src->as<uint32_t>(),
+            out.dst->as<uint32_t>(), bad.as<int>(), s);
+  int h_bad = 0;
+  KC_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  if (h_bad & 1) fail(KC_ERR_IO, "%s: malformed edge line (expected '<type> u v')", path);
+  if (h_bad & 2) fail(KC_ERR_INVALID, "%s: an edge lies outside [1,%llu]", path, static_cast<unsigned long long>(n));
+}
+
+// The edges of all parts in one list, then the CSR in HBM.
+void device_edge_graph(kc_graph *g, uint64_t n, std::vector<EdgeTextPart> &parts) {
+  uint64_t total = 0;
+  for (const EdgeTextPart &e : parts) total += e.count;
+  DevTemp src(total * 4 + 64), dst(total * 4 + 64);
+  uint64_t at = 0;
+  for (const EdgeTextPart &e : parts) {
+    if (e.count) {
+      KC_HIP(hipMemcpyAsync(src.as<uint32_t>() + at, e.src->p, e.count * 4, hipMemcpyDeviceToDevice, g->stream));
+      KC_HIP(hipMemcpyAsync(dst.as<uint32_t>() + at, e.dst->p, e.count * 4, hipMemcpyDeviceToDevice, g->stream));
+    }
+    at += e.count;
+  }
+  parts.clear();
+  g->n = n;
+  build_device_csr(g, n, src.as<uint32_t>(), dst.as<uint32_t>(), total);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kc_graph_read_edgefile(const char *path, kc_graph **out) {
+  return guarded([&] {
+    if (!path || !out) fail(KC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<kc_graph> g = new_handle();
+    uint64_t n = 0, m = 0;
+    std::vector<EdgeTextPart> parts(1);
+    device_edge_file(g.get(), path, true, false, n, m, parts[0]);
+    device_edge_graph(g.get(), n, parts);
+    init_work(g.get());
+    *out = g.release();
+  });
+}
+
+int kc_graph_read_partitioned_edgefile(const char *path, int parts_n, kc_graph **out) {
+  return guarded([&] {
+    if (!path || !out || parts_n < 1) fail(KC_ERR_INVALID, "null argument or parts < 1");
+    *out = nullptr;
+    std::unique_ptr<kc_graph> g = new_handle();
+    uint64_t n = 0, m = 0;
+    std::vector<EdgeTextPart> parts(static_cast<size_t>(parts_n));
+    for (int r = 0; r < parts_n; ++r) {
+      const std::string part = std::string(path) + "_" + std::to_string(r);
+      device_edge_file(g.get(), part.c_str(), false, r > 0, n, m, parts[static_cast<size_t>(r)]);
+    }
+    device_edge_graph(g.get(), n, parts);
+    init_work(g.get());
+    *out = g.release();
+  });
+}
+
 int kc_graph_read_metis(const char *path, kc_graph **out) {
   return guarded([&] {
     if (!path || !out) fail(KC_ERR_INVALID, "null argument");

@@ -284,4 +284,14 @@ void metis_other(const uint64_t *otherpos, uint64_t count, const uint64_t *nlpos
 uint64_t metis_edges(uint64_t count, const uint32_t *tv, const uint32_t *tval, uint32_t *src, uint32_t *dst,
                      uint64_t *total, uint64_t *scratch, hipStream_t s);
 
+// Edge-list text on the device: edge_text_lines writes the indices of the
+// lines that are neither empty nor comments into eline (null: count only) and
+// returns their count; edge_text parses "<type> u v" from the first count of
+// them into 0-based (src, dst), *bad |= 1 for a malformed line, |= 2 for an
+// id outside [1, n].
+uint64_t edge_text_lines(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t nl, uint64_t lines,
+                         uint64_t *eline, uint64_t *total, uint64_t *scratch, hipStream_t s);
+void edge_text(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t nl, const uint64_t *eline,
+               uint64_t count, uint64_t n, uint32_t *src, uint32_t *dst, int *bad, hipStream_t s);
+
 }  // namespace kc

@@ -1711,6 +1711,61 @@ __global__ void k_metis_other(const uint64_t *otherpos, uint64_t count, const ui
   if (__ballot(wrong) != 0 && lane_id() == 0) atomicOr(bad, 2);
 }
 
+
+// Edge lines of a sorted edge file (ReadSortedEdgeFile, lib/io/graph_io.h:
+// 385-669, ParseEdgeFilestream :1250-1300): a line that is neither empty nor
+// a comment ('%') holds "<type> u v"; a trailing '\r' is dropped.
+struct EdgeLineSpan {
+  const uint8_t *b;
+  const uint64_t *nlpos;
+  uint64_t bytes, nl;
+  __device__ void operator()(uint64_t j, uint64_t &st, uint64_t &en) const {
+    st = j ? nlpos[j - 1] + 1 : 0;
+    en = j < nl ? nlpos[j] : bytes;
+    if (en > st && b[en - 1] == '\r') --en;
+  }
+};
+struct EdgeLineLoad {
+  EdgeLineSpan span;
+  __device__ uint64_t operator()(uint64_t j) const {
+    uint64_t st, en;
+    span(j, st, en);
+    return st < en && span.b[st] != '%' ? 1ull : 0ull;
+  }
+};
+
+// Edge k comes from the k-th edge line: the type token is skipped, then two
+// numbers are read.  *bad |= 1 for a line without two numbers, |= 2 for an id
+// outside [1, n].
+__global__ void k_edge_text(EdgeLineSpan span, const uint64_t *eline, uint64_t count, uint64_t n, uint32_t *src,
+                            uint32_t *dst, int *bad) {
+  const uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  int wrong = 0;
+  if (k < count) {
+    uint64_t st, en;
+    span(eline[k], st, en);
+    const uint8_t *b = span.b;
+    uint64_t p = st;
+    while (p < en && (b[p] == ' ' || b[p] == '\t')) ++p;
+    while (p < en && b[p] != ' ' && b[p] != '\t') ++p;
+    uint64_t x[2] = {0, 0};
+    for (int i = 0; i < 2 && !wrong; ++i) {
+      while (p < en && (b[p] == ' ' || b[p] == '\t')) ++p;
+      if (p >= en || !byte_digit(b[p])) wrong = 1;
+      for (; !wrong && p < en && byte_digit(b[p]); ++p) {
+        const unsigned d = b[p] - '0';
+        if (x[i] > (~0ull - d) / 10) wrong = 1;
+        x[i] = x[i] * 10 + d;
+      }
+    }
+    if (!wrong && (x[0] == 0 || x[1] == 0 || x[0] > n || x[1] > n)) wrong = 2;
+    src[k] = wrong ? 0u : static_cast<uint32_t>(x[0] - 1);
+    dst[k] = wrong ? 0u : static_cast<uint32_t>(x[1] - 1);
+  }
+  const uint64_t m1 = __ballot(wrong == 1), m2 = __ballot(wrong == 2);
+  if (lane_id() == 0 && (m1 | m2)) atomicOr(bad, (m1 ? 1 : 0) | (m2 ? 2 : 0));
+}
+
 struct EdgeKeepLoad {
   const uint32_t *tv;
   __device__ uint64_t operator()(uint64_t t) const { return tv[t] != kNone ? 1ull : 0ull; }
@@ -2330,6 +2385,25 @@ uint64_t metis_edges(uint64_t count, const uint32_t *tv, const uint32_t *tval, u
   KC_HIP(hipMemcpyAsync(&h, total, sizeof(h), hipMemcpyDeviceToHost, s));
   KC_HIP(hipStreamSynchronize(s));
   return h;
+}
+
+
+uint64_t edge_text_lines(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t nl, uint64_t lines,
+                         uint64_t *eline, uint64_t *total, uint64_t *scratch, hipStream_t s) {
+  scan_exclusive<uint64_t>(lines, EdgeLineLoad{EdgeLineSpan{b, nlpos, bytes, nl}}, PosStore{eline}, total, scratch,
+                           s);
+  uint64_t h = 0;
+  KC_HIP(hipMemcpyAsync(&h, total, sizeof(h), hipMemcpyDeviceToHost, s));
+  KC_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void edge_text(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t nl, const uint64_t *eline,
+               uint64_t count, uint64_t n, uint32_t *src, uint32_t *dst, int *bad, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_edge_text, dim3(static_cast<unsigned>(ceil_div(count, static_cast<uint64_t>(kBlock)))),
+                     dim3(kBlock), 0, s, EdgeLineSpan{b, nlpos, bytes, nl}, eline, count, n, src, dst, bad);
+  KC_LAUNCH_CHECK();
 }
 
 }  // namespace kc

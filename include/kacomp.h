@@ -240,6 +240,16 @@ int kc_graph_read_binary(const char *path, kc_graph **out);
  * (kc_read_metis stops the line there).  GraphIO::ReadMETISFile
  * (lib/io/graph_io.h:281-382) with ParseVertexFilestream (:1207-1248). */
 int kc_graph_read_metis(const char *path, kc_graph **out);
+/* Read a sorted edge file (the format of kc_read_edgefile) or its partitioned
+ * form (parts files "<path>_<r>", as kc_read_partitioned_edgefile) straight into
+ * a device graph. Each file goes to HBM in one copy. There its lines are
+ * found, every edge line's "<type> u v" is parsed by one thread, and the CSR
+ * is built, with the same rows as the host readers and no fixed order inside
+ * a row. GraphIO::ReadSortedEdgeFile (lib/io/graph_io.h:385-669) and
+ * ReadPartitionedSortedEdgeFile (:672-907) with ParseEdgeFilestream
+ * (:1250-1300). */
+int kc_graph_read_edgefile(const char *path, kc_graph **out);
+int kc_graph_read_partitioned_edgefile(const char *path, int parts, kc_graph **out);
 /* Copies the level-0 CSR of a graph to host memory (free with kc_csr_free). */
 int kc_graph_download(const kc_graph *g, kc_csr *out);
 

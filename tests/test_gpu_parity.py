@@ -482,6 +482,72 @@ def test_device_metis_reader_matches_host(tmp_path):
         assert e.value.code == code, text
 
 
+def test_device_edgefile_readers_match_host(tmp_path):
+    """kc_graph_read_edgefile and kc_graph_read_partitioned_edgefile against the host readers:
+    generated graphs, comments, CRLF, blank lines, lines past the header's m (ignored by the
+    plain reader, read by the partitioned one), a missing last newline, and the errors."""
+    def edges_once(g):
+        src = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.offsets.astype(np.int64)))
+        keep = src < g.targets
+        return src[keep], g.targets.astype(np.int64)[keep]
+
+    def write(path, n, u, v, header_m=None):
+        lines = [f"p {n} {len(u) if header_m is None else header_m}"]
+        lines += [f"e {a + 1} {b + 1}" for a, b in zip(u.tolist(), v.tolist())]
+        path.write_text("\n".join(lines) + "\n")
+
+    def same(dg, host):
+        try:
+            dev = dg.download()
+            assert dev.n == host.n and np.array_equal(dev.offsets, host.offsets)
+            assert _sorted_rows(dev) == _sorted_rows(host)
+            lab, _ = dg.find_components(variant="exp", seed=1001)
+            assert np.array_equal(lab, oracle.canonical_labels(host.offsets, host.targets))
+        finally:
+            dg.close()
+
+    files = []
+    for name, kw in (("rmat", dict(n=1 << 13, m=1 << 17, seed=5)), ("rgg_2d", dict(n=3000, r=0.03, seed=2))):
+        g = ca.generate(name, **kw)
+        u, v = edges_once(g)
+        p = tmp_path / f"{name}.edges"
+        write(p, g.n, u, v)
+        files.append(p)
+    hand = {"sem.edges": b"% c\r\np 6 4\r\ne 1 3\r\n% inner\r\ne 3 1\r\ne 2 2\r\n\r\ne 4 5\r\ne 5 6\r\nx y z\r\n",
+            "tabs.edges": b"%x\np\t5 3\n\t e   1\t2  \n\ne 2 3 trailing words\ne 4 5",
+            "empty.edges": b"p 7 0\n"}
+    for fname, raw in hand.items():
+        (tmp_path / fname).write_bytes(raw)
+        files.append(tmp_path / fname)
+    for p in files:
+        print(f"edge file {os.path.basename(str(p))}", flush=True)
+        same(ca.DeviceGraph.read_edgefile(p), ca.read_edgefile(p))
+    g = ca.generate("rgg_2d", n=2000, r=0.03, seed=2)
+    u, v = edges_once(g)
+    cuts = [0, len(u) // 3, 2 * len(u) // 3, len(u)]
+    for r in range(3):
+        write(tmp_path / f"part_{r}", g.n, u[cuts[r]:cuts[r + 1]], v[cuts[r]:cuts[r + 1]], header_m=len(u))
+    for parts in (1, 3):
+        same(ca.DeviceGraph.read_edgefile(tmp_path / "part", parts=parts),
+             ca.read_partitioned_edgefile(tmp_path / "part", parts=parts))
+    (tmp_path / "bad_0").write_text(f"p {g.n} {len(u)}\n")
+    (tmp_path / "bad_1").write_text(f"p {g.n + 1} {len(u)}\n")
+    for path, parts, code in ((tmp_path / "bad", 2, ca.KC_ERR_INVALID),):
+        with pytest.raises(ca.KcError) as e:
+            ca.DeviceGraph.read_edgefile(path, parts=parts)
+        assert e.value.code == code
+    for text, code in (("p 3 1\ne 1 4\n", ca.KC_ERR_INVALID), ("p 3 1\ne 1\n", ca.KC_ERR_IO),
+                       ("p 3 1\ne 1 x\n", ca.KC_ERR_IO), ("p 3\ne 1 2\n", ca.KC_ERR_IO),
+                       ("% only\n", ca.KC_ERR_IO), ("p 3 1\ne 0 2\n", ca.KC_ERR_INVALID)):
+        p = tmp_path / "bad.edges"
+        p.write_text(text)
+        with pytest.raises(ca.KcError) as e1:
+            ca.read_edgefile(p)
+        with pytest.raises(ca.KcError) as e2:
+            ca.DeviceGraph.read_edgefile(p)
+        assert e1.value.code == code and e2.value.code == code, text
+
+
 # -single / -contraction / the default branch's local contraction, over a vertex-block distribution
 @pytest.mark.parametrize("bfs", [False, True])
 @pytest.mark.parametrize("blocks", [1, 5, 64, 1 << 20])
