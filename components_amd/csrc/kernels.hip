@@ -208,6 +208,7 @@ struct SweepArgs {
 
 // Edges whose gathers one lane keeps in flight together in the row loops.
 constexpr int kGather = 4;
+constexpr int kRelabelGather = 16;
 
 template <int MODE, bool COUNT, int W>
 __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uint32_t v, uint64_t e0,
@@ -431,19 +432,19 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const Lab &lab
     rounds = t > rounds ? t : rounds;
   }
   uint32_t base = 0;  // cluster edges of the row (chunk) written so far
-  for (uint64_t r = 0; r < rounds; r += kGather) {
-    uint32_t c[kGather], l[kGather];
-    bool ok[kGather];
+  for (uint64_t r = 0; r < rounds; r += kRelabelGather) {
+    uint32_t c[kRelabelGather], l[kRelabelGather];
+    bool ok[kRelabelGather];
 #pragma unroll
-    for (int j = 0; j < kGather; ++j) {
+    for (int j = 0; j < kRelabelGather; ++j) {
       const uint64_t e = e0 + (r + j) * W + gl;
       ok[j] = active && e < e1;
       c[j] = ok[j] ? a.col[e] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < kGather; ++j) l[j] = ok[j] ? lab(c[j]) : lv;
+    for (int j = 0; j < kRelabelGather; ++j) l[j] = ok[j] ? lab(c[j]) : lv;
 #pragma unroll
-    for (int j = 0; j < kGather; ++j) {
+    for (int j = 0; j < kRelabelGather; ++j) {
       bool inter = l[j] != lv;
       if (W > 1 && inter) {
         const uint32_t slot = hash32(l[j]) & fmask;
