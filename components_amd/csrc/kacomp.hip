@@ -395,7 +395,7 @@ uint64_t sweep_bytes(uint64_t n, uint64_t m2, bool count) {
 // (the last of them found no change).
 uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                            uint64_t *key, uint64_t hop, const Bins &b, bool count, bool with_jump,
-                           uint32_t &flag_base, bool book_bytes = true) {
+                           uint32_t &flag_base, bool book_bytes = true, uint64_t block = 0) {
   uint64_t done = 0;
   const uint32_t batch = n < (1u << 16) ? 8u : 3u;
   const uint64_t bytes = book_bytes ? sweep_bytes(n, m2, count) : 0;
@@ -410,8 +410,8 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
       // the first sweep of a batch always runs: the previous batch ended with a change
       const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
       slots[i] = g->timer.start(KC_K_SWEEP);
-      sweep(kSweepInPlace, count, off, col, key, key, hop, b, g->deg_inter, g->hub_inter, &g->flags[f], prev,
-            g->stream);
+      sweep(block ? kSweepBlock : kSweepInPlace, count, off, col, key, key, hop, b, g->deg_inter, g->hub_inter,
+            &g->flags[f], prev, g->stream, block);
       g->timer.stop(slots[i], bytes);
       if (with_jump) {
         TimedScope js(g->timer, KC_K_JUMP, 16 * n);
@@ -755,25 +755,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         TimedScope ts(g->timer, KC_K_DEVIATES, 8 * n_l);
         init_keys_index(n_l, g->key, g->stream);
       }
-      while (true) {
-        if (flag_base + 2 >= kMaxFlags) {
-          fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
-          flag_base = 0;
-        }
-        int *f = &g->flags[flag_base++];
-        {
-          TimedScope ts(g->timer, KC_K_SWEEP, sweep_bytes(n_l, m2_l, false));
-          block_prop(off_l, col_l, n_l, bsize, g->key, f, g->stream);
-        }
-        {
-          TimedScope ts(g->timer, KC_K_JUMP, 16 * n_l);
-          jump(n_l, g->key, f, g->stream);
-        }
-        d2h(g, g->h_flags, f, sizeof(int));
-        sync(g);
-        ++sweeps;
-        if (g->h_flags[0] == 0) break;
-      }
+      // (sweeps over the edges inside the blocks, with pointer jumping, in batches)
+      sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, 0, b, false, true, flag_base, true, bsize);
       if (o->pre_contract == KC_PRE_LOCAL) {
         TimedScope ts(g->timer, KC_K_SWEEP, 8 * (n_l + 1) + 4 * m2_l + 8 * n_l);
         block_interface(off_l, col_l, n_l, bsize, g->key, g->stream);

@@ -34,7 +34,8 @@ struct Bins {
 enum SweepMode : int {
   kSweepInPlace = 0,  // key[v] <- min(key[v], key[w] + hop), updates visible at once
   kSweepJacobi = 1,   // kout[v] <- min over closed neighbourhood of kin (kout pre-copied)
-  kSweepCountOnly = 2 // no update: count neighbours with a different label
+  kSweepCountOnly = 2, // no update: count neighbours with a different label
+  kSweepBlock = 3      // in place, hop 0, over the neighbours in the row's block of consecutive indices
 };
 
 struct DeviateArgs {
@@ -84,7 +85,7 @@ uint64_t bin_scratch_words(uint64_t n);
 // null): when it reads 0 the launch returns at once (converged earlier).
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin,
            uint64_t *kout, uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter,
-           int *flag, const int *prev_flag, hipStream_t s);
+           int *flag, const int *prev_flag, hipStream_t s, uint64_t block = 0);
 // Round t of the time-stepped schedule: vertices at key distance t push
 // key + hop; *pending = 1 while a vertex is still above distance t;
 // prev_pending (may be null) reading 0 makes the launch return at once.
@@ -100,12 +101,8 @@ void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uin
 // Pointer jumping on keys: key[v] <- min(key[v], key[label(v)]).
 void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
 
-// Components inside blocks of bsize consecutive indices (keys start as
-// init_keys_index): one pass of min-key propagation over the edges inside a
-// block, *flag = 1 when a key fell.  block_interface then gives every vertex
-// with a neighbour outside its block its own index back.
-void block_prop(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key, int *flag,
-                hipStream_t s);
+// Interface vertices of blocks of bsize consecutive indices (a neighbour in
+// another block) get their own index back as key: key[v] = v.
 void block_interface(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key,
                      hipStream_t s);
 

@@ -204,6 +204,7 @@ struct SweepArgs {
   uint32_t *hub_inter;
   int *flag;
   const int *prev_flag;
+  uint64_t block = 0;  // kSweepBlock: indices per block
 };
 
 // Edges whose gathers one lane keeps in flight together in the row loops.
@@ -218,6 +219,9 @@ __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uin
   uint32_t cnt = 0;
   if (active) {
     const uint32_t lv = static_cast<uint32_t>(kv);
+    // kSweepBlock reads the keys of the neighbours in v's block only
+    const uint64_t lo = MODE == kSweepBlock ? v / a.block * a.block : 0;
+    const uint64_t hi = MODE == kSweepBlock ? lo + a.block : ~0ull;
     uint64_t e = e0 + gl;
     // kGather edges per lane per step: their neighbour ids, then their keys,
     // are all in flight together
@@ -227,7 +231,7 @@ __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uin
 #pragma unroll
       for (int j = 0; j < kGather; ++j) c[j] = a.col[e + j * W];
 #pragma unroll
-      for (int j = 0; j < kGather; ++j) k[j] = a.kin[c[j]];
+      for (int j = 0; j < kGather; ++j) k[j] = c[j] >= lo && c[j] < hi ? a.kin[c[j]] : ~0ull;
 #pragma unroll
       for (int j = 0; j < kGather; ++j) {
         best = min_u64(best, k[j] + a.hop);
@@ -235,7 +239,8 @@ __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uin
       }
     }
     for (; e < e1; e += W) {
-      const uint64_t kw = a.kin[a.col[e]];
+      const uint32_t c = a.col[e];
+      const uint64_t kw = c >= lo && c < hi ? a.kin[c] : ~0ull;
       best = min_u64(best, kw + a.hop);
       if (COUNT) cnt += static_cast<uint32_t>(kw) != lv;
     }
@@ -547,37 +552,9 @@ __global__ void k_jump(uint64_t n, uint64_t *key, int *flag) {
 }
 
 // ---------------------------------------------------------------------------
-// Components inside blocks of consecutive level indices (block b holds
-// [b*bsize, (b+1)*bsize)), the local components of a vertex-block distribution
-// over PEs.  One wave per row: key[v] (dev 0, label in the low half) takes the
-// least key among v's neighbours in v's block; *flag = 1 when a key fell.
-__global__ __launch_bounds__(kBlock) void k_block_prop(const uint64_t *off, const uint32_t *col, uint64_t n,
-                                                      uint64_t bsize, uint64_t *key, int *flag) {
-  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
-  bool changed = false;
-  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock / kWave) + threadIdx.x / kWave; v < n; v += waves) {
-    const uint64_t lo = v / bsize * bsize, hi = lo + bsize < n ? lo + bsize : n;
-    uint64_t best = key[v];
-    const uint64_t e1 = off[v + 1];
-    for (uint64_t e = off[v] + lane_id(); e < e1; e += kWave) {
-      const uint32_t u = col[e];
-      if (u >= lo && u < hi) {
-        const uint64_t k = key[u];
-        best = k < best ? k : best;
-      }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t t = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(best), o, 64));
-      best = t < best ? t : best;
-    }
-    if (lane_id() == 0 && best < key[v]) {
-      key[v] = best;
-      changed = true;
-    }
-  }
-  if (__ballot(changed) != 0 && lane_id() == 0) *flag = 1;
-}
-
+// Blocks of consecutive level indices (block b holds [b*bsize, (b+1)*bsize)),
+// the PEs of a vertex-block distribution; their components come from
+// kSweepBlock sweeps with pointer jumping.
 // Interface vertices (a neighbour outside the block) keep their own index as
 // label: key[v] = v.
 __global__ __launch_bounds__(kBlock) void k_block_interface(const uint64_t *off, const uint32_t *col, uint64_t n,
@@ -1895,9 +1872,9 @@ void finish_bins(Bins &b) {
 
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin, uint64_t *kout,
            uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter, int *flag, const int *prev_flag,
-           hipStream_t s) {
+           hipStream_t s, uint64_t block) {
   if (b.blocks == 0) return;
-  SweepArgs a{off, col, kin, kout, hop, b, deg_inter, hub_inter, flag, prev_flag};
+  SweepArgs a{off, col, kin, kout, hop, b, deg_inter, hub_inter, flag, prev_flag, block};
   dim3 g(b.blocks), t(kBlock);
   if (mode == kSweepInPlace && count)
     hipLaunchKernelGGL((k_sweep<kSweepInPlace, true>), g, t, 0, s, a);
@@ -1905,6 +1882,8 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
     hipLaunchKernelGGL((k_sweep<kSweepInPlace, false>), g, t, 0, s, a);
   else if (mode == kSweepJacobi)
     hipLaunchKernelGGL((k_sweep<kSweepJacobi, false>), g, t, 0, s, a);
+  else if (mode == kSweepBlock)
+    hipLaunchKernelGGL((k_sweep<kSweepBlock, false>), g, t, 0, s, a);
   else
     hipLaunchKernelGGL((k_sweep<kSweepCountOnly, true>), g, t, 0, s, a);
   KC_LAUNCH_CHECK();
@@ -1933,13 +1912,6 @@ void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uin
 
 static uint32_t row_wave_grid(uint64_t n) {
   return static_cast<uint32_t>(std::min<uint64_t>(ceil_div(n, static_cast<uint64_t>(kBlock / kWave)), 1u << 16));
-}
-
-void block_prop(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key, int *flag,
-                hipStream_t s) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_block_prop, dim3(row_wave_grid(n)), dim3(kBlock), 0, s, off, col, n, bsize, key, flag);
-  KC_LAUNCH_CHECK();
 }
 
 void block_interface(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key,
