@@ -831,8 +831,17 @@ __device__ __forceinline__ void scatter_group(const ScatterArgs &a, bool active,
   if (active && gl == 0) pos = static_cast<uint64_t>(a.pos_of[v]) + extra;
   if (W > 1) pos = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(pos), gbase, 64));
   if (!active) return;
-  // the row's (chunk's) d cluster edges sit at its start: a straight copy
-  for (uint32_t i = gl; i < d; i += W) a.bucket[pos + i] = a.relab[e0 + i];
+  // the row's (chunk's) d cluster edges sit at its start: a straight copy,
+  // four loads in flight per lane before their stores
+  uint32_t i = gl;
+  for (; i + 3 * W < d; i += 4 * W) {
+    uint32_t x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = a.relab[e0 + i + j * W];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a.bucket[pos + i + j * W] = x[j];
+  }
+  for (; i < d; i += W) a.bucket[pos + i] = a.relab[e0 + i];
 }
 
 // Rows of the one-lane bin (16 slots or fewer with the default bins): each
