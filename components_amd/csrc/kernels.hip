@@ -1426,7 +1426,20 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(uint64_t n_next, const u
     const uint64_t s0 = seg_off[a2];
     if (seg_off[a2 + 1] - s0 > kDedupWaveCap) continue;  // listed: copied in chunks
     const uint64_t o0 = off_next[a2], len = off_next[a2 + 1] - o0;
-    for (uint64_t j = gl; j < len; j += 16) {
+    // four values per lane in flight, then their next-level ids
+    uint64_t j = gl;
+    for (; j + 48 < len; j += 64) {
+      uint32_t v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = vals[s0 + j + 16 * k];
+      if (map) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = map[v[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) col_next[o0 + j + 16 * k] = v[k];
+    }
+    for (; j < len; j += 16) {
       const uint32_t v = vals[s0 + j];
       col_next[o0 + j] = map ? map[v] : v;
     }
