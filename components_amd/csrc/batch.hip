@@ -25,6 +25,7 @@
 #include <memory>
 #include <vector>
 
+#include "csr_build.h"
 #include "deviates.h"
 #include "kc_common.h"
 #include "kernels.h"
@@ -801,6 +802,14 @@ int kc_batch_create(const kc_batch_host *h, kc_batch **out) {
       uint64_t tw = 64;
       while (tw < 2 * slots) tw <<= 1;
       toff[g + 1] = toff[g] + tw;
+    }
+    {
+      // the contraction reads every cluster edge from both of its ends
+      uint32_t bg = 0;
+      uint64_t bu = 0, bv = 0;
+      if (G && find_one_way_edge_batch(G, h->vtx_off, h->offsets, h->targets, bg, bu, bv))
+        fail(KC_ERR_INVALID, "batch: graph %u is not symmetric: edge %llu -> %llu has no reverse", bg,
+             (unsigned long long)bu, (unsigned long long)bv);
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(KC_ERR_HIP, "no HIP device available");

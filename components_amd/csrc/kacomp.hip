@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "csr_build.h"
 #include "deviates.h"
 #include "kc_common.h"
 #include "kernels.h"
@@ -1216,6 +1217,11 @@ int kc_graph_create(const kc_csr_host *csr, kc_graph **out) {
     }
     for (uint64_t e = 0; e < m2; ++e)
       if (csr->targets[e] >= n) fail(KC_ERR_INVALID, "target %u out of range at edge %llu", csr->targets[e], (unsigned long long)e);
+    // the contraction reads every cluster edge from both of its ends
+    uint64_t bu = 0, bv = 0;
+    if (n && find_one_way_edge(csr->offsets, csr->targets, 0, n, bu, bv))
+      fail(KC_ERR_INVALID, "CSR is not symmetric: edge %llu -> %llu has no reverse", (unsigned long long)bu,
+           (unsigned long long)bv);
     std::unique_ptr<kc_graph> g = new_handle();
     g->n = n;
     g->m2 = m2;

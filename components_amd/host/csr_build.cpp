@@ -66,4 +66,52 @@ void build_symmetric_csr(uint64_t n, const uint32_t *src, const uint32_t *dst, u
     std::memcpy(out->targets + out->offsets[v], tgt.data() + off[v], udeg[v] * sizeof(uint32_t));
 }
 
+bool find_one_way_edge(const uint64_t *offsets, const uint32_t *targets, uint64_t a, uint64_t z, uint64_t &u,
+                       uint64_t &v) {
+  const uint64_t rows = z - a, base = offsets[a], slots = offsets[z] - base;
+  if (!slots) return false;
+  const bool par = slots > (1u << 16);
+  std::vector<uint32_t> sorted(targets + base, targets + base + slots);
+#pragma omp parallel for schedule(dynamic, 1024) if (par)
+  for (int64_t r = 0; r < static_cast<int64_t>(rows); ++r)
+    std::sort(sorted.data() + (offsets[a + r] - base), sorted.data() + (offsets[a + r + 1] - base));
+  uint64_t bad = ~0ull;  // smallest row holding a one-way edge
+#pragma omp parallel for schedule(dynamic, 1024) reduction(min : bad) if (par)
+  for (int64_t r = 0; r < static_cast<int64_t>(rows); ++r) {
+    for (uint64_t e = offsets[a + r]; e < offsets[a + r + 1]; ++e) {
+      const uint32_t w = targets[e];
+      const uint32_t *b = sorted.data() + (offsets[a + w] - base), *end = sorted.data() + (offsets[a + w + 1] - base);
+      if (!std::binary_search(b, end, static_cast<uint32_t>(r))) {
+        bad = std::min<uint64_t>(bad, static_cast<uint64_t>(r));
+        break;
+      }
+    }
+  }
+  if (bad == ~0ull) return false;
+  u = bad;
+  for (uint64_t e = offsets[a + bad]; e < offsets[a + bad + 1]; ++e) {
+    const uint32_t w = targets[e];
+    const uint32_t *b = sorted.data() + (offsets[a + w] - base), *end = sorted.data() + (offsets[a + w + 1] - base);
+    if (!std::binary_search(b, end, static_cast<uint32_t>(bad))) {
+      v = w;
+      break;
+    }
+  }
+  return true;
+}
+
+bool find_one_way_edge_batch(uint32_t graphs, const uint64_t *vtx_off, const uint64_t *offsets,
+                             const uint32_t *targets, uint32_t &graph, uint64_t &u, uint64_t &v) {
+  uint32_t bad = ~0u;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : bad)
+  for (int64_t g = 0; g < static_cast<int64_t>(graphs); ++g) {
+    uint64_t a = 0, b = 0;
+    if (find_one_way_edge(offsets, targets, vtx_off[g], vtx_off[g + 1], a, b))
+      bad = std::min(bad, static_cast<uint32_t>(g));
+  }
+  if (bad == ~0u) return false;
+  graph = bad;
+  return find_one_way_edge(offsets, targets, vtx_off[bad], vtx_off[bad + 1], u, v);
+}
+
 }  // namespace kc

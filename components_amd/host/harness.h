@@ -255,6 +255,10 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner, bool w
     // ComputeSequentialLimit (exponential_contraction.h:1287-1292) with P = -pes
     // (one PE per compute unit); the relative limit is floored at 4096 edges,
     // below which the single-GPU base case is cheaper than further levels.
+    // Both limits are in undirected edges: GatherNumberOfGlobalEdges sums the
+    // adjacency entries of all PEs and halves the sum (static_graph.h:353-360,
+    // dynamic_graph.h:335-342), and the level loop compares that count with
+    // the limit (exponential_contraction.h:620-627).
     const uint64_t rel = (g.m / (c.pes ? c.pes : 1)) * c.sequential_limit;
     o.seq_limit = c.fixed_limit ? c.sequential_limit : (rel > 4096 ? rel : 4096);
     if (whole_graph_base_case) o.seq_limit = ~0ull;

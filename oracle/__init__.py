@@ -42,6 +42,8 @@ def lib():
             "or_total_sweeps": (u64, [ctypes.c_void_p]),
             "or_seconds": (dbl, [ctypes.c_void_p]),
             "or_uniform_deviate_100": (u32, [u32, i32, u32]),
+            "or_splitmix64_mix": (u64, [u64]),
+            "or_hashed_uniform": (dbl, [u32, i32, u32]),
             "or_mlp_forward": (None, [_P(dbl), _P(ctypes.c_float), u64, i32, _P(dbl)]),
             "or_mlp_grad": (None, [_P(dbl), _P(ctypes.c_float), _P(ctypes.c_float), u64, i32, _P(dbl), _P(dbl)]),
             "or_level_features": (None, [u64, _P(u64), _P(u32), u32, _P(ctypes.c_float)]),
@@ -97,6 +99,18 @@ def exp_deviate(gid, seed, iteration, rng_offset, beta=0.1, fma=True):
 
 def hashed_deviate(gid, seed, iteration, beta=0.1):
     return int(lib().or_hashed_deviate(gid, seed, iteration, beta))
+
+
+def hashed_uniform(gid, seed, iteration):
+    return float(lib().or_hashed_uniform(gid, seed, iteration))
+
+
+def uniform_deviate_100(gid, seed, iteration):
+    return int(lib().or_uniform_deviate_100(gid, seed, iteration))
+
+
+def splitmix64_mix(z):
+    return int(lib().or_splitmix64_mix(z))
 
 
 class Run:

@@ -91,27 +91,32 @@ uint32_t exp_deviate(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rn
 }
 
 // Hashed stream of the library (SplitMix64 over seed, iteration, gid), then
-// the exponential law evaluated directly with std::log.
+// the exponential law evaluated directly with std::log.  The law is the one
+// of exponential_contraction.h:467-471 (exponential_distribution(beta),
+// truncated); the stream is not the reference's minstd_rand0 draw.
+// SplitMix64's output function (Steele, Lea, Flood 2014).
 uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
   return z ^ (z >> 31);
 }
-uint32_t hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta) {
+double hashed_uniform(uint32_t gid, int32_t seed, uint32_t iteration) {
   const uint64_t key = ((static_cast<uint64_t>(static_cast<uint32_t>(seed)) << 32) | iteration) * 0x9e3779b97f4a7c15ull;
   const uint64_t z = mix64(mix64(key) + static_cast<uint64_t>(gid) * 0xd1b54a32d192ed03ull);
-  const double u = static_cast<double>(z >> 11) * 0x1p-53;
+  return static_cast<double>(z >> 11) * 0x1p-53;
+}
+uint32_t hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta) {
+  const double u = hashed_uniform(gid, seed, iteration);
   return static_cast<uint32_t>(static_cast<uint64_t>(-std::log(1.0 - u) / beta));
 }
 
+// Uniform deviate in [0, 99] of local contraction: the law of
+// uniform_int_distribution<unsigned>(0, 99) (local_contraction.h:173-176),
+// drawn from a SplitMix64 stream over (seed, iteration, gid) and reduced by a
+// multiply-shift of its upper 32 bits.
 uint32_t uniform100(uint32_t gid, int32_t seed, uint32_t iteration) {
-  auto mix = [](uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-  };
-  const uint64_t z = mix((0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(static_cast<uint32_t>(seed)) + 1)) ^
-                         (static_cast<uint64_t>(iteration) << 32) ^ gid);
+  const uint64_t z = mix64((0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(static_cast<uint32_t>(seed)) + 1)) ^
+                           (static_cast<uint64_t>(iteration) << 32) ^ gid);
   return static_cast<uint32_t>(((z >> 32) * 100ull) >> 32);
 }
 
@@ -393,6 +398,10 @@ uint32_t or_hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, doubl
 uint32_t or_uniform_deviate_100(uint32_t gid, int32_t seed, uint32_t iteration) {
   return uniform100(gid, seed, iteration);
 }
+
+uint64_t or_splitmix64_mix(uint64_t z) { return mix64(z); }
+
+double or_hashed_uniform(uint32_t gid, int32_t seed, uint32_t iteration) { return hashed_uniform(gid, seed, iteration); }
 
 uint64_t or_lp_fixpoint(uint64_t n, const uint64_t *off, const uint32_t *tgt, const uint64_t *key0, uint64_t hop,
                         uint64_t *key_out) {

@@ -13,6 +13,18 @@
  *                         and without fused multiply-add contraction
  *   or_lp_fixpoint        against or_lp_closed_form (multi-source Dijkstra over the
  *                         same deviates), which is the fixed point's closed form
+ *   or_splitmix64_mix     the published SplitMix64 output sequences (seeds 0 and
+ *                         1234567), bijectivity on a sample, avalanche
+ *                         (tests/test_oracle_streams.py, as the next three)
+ *   or_hashed_uniform /   Kolmogorov-Smirnov against U(0,1) and against the
+ *   or_hashed_deviate     continuous exponential law; the truncated tail law
+ *                         P(dev >= k) = exp(-beta k) (exponential_contraction.h:
+ *                         467-471); no correlation across ids and iterations
+ *   or_uniform_deviate_100  chi-squared against the uniform law on 0..99 of
+ *                         local_contraction.h:173-176
+ *   or_run (LOCAL) keys   level 0 against the closed form of two synchronous
+ *                         passes: the minimum key over the closed 2-hop
+ *                         neighbourhood, from the sparse product (A+I)^2
  *   or_run (EXP/LOCAL)    labels against or_canonical_labels; per-level sizes and
  *                         keys against independent recomputation in the tests
  *   or_run (LABELS/SHORTCUT) labels against or_canonical_labels
@@ -74,6 +86,10 @@ double or_seconds(const or_result *r);
 uint32_t or_hashed_deviate(uint32_t gid, int32_t seed, uint32_t iteration, double beta);
 /* Uniform deviate in [0,99] used by local contraction (SplitMix64). */
 uint32_t or_uniform_deviate_100(uint32_t gid, int32_t seed, uint32_t iteration);
+/* SplitMix64's output function, shared by both streams above. */
+uint64_t or_splitmix64_mix(uint64_t z);
+/* The 53-bit uniform in [0,1) behind or_hashed_deviate. */
+double or_hashed_uniform(uint32_t gid, int32_t seed, uint32_t iteration);
 
 /* Learned centre proposals (no reference counterpart).  w: KC_ORACLE_MLP_PARAMS
  * parameters in the library's public order (W1[6][64], b1, W2[64][64], b2, w3, b3). */

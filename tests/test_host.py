@@ -323,3 +323,35 @@ def test_batch_create_rejects_bad_input():
     with pytest.raises(ca.KcError) as e:
         ca.DeviceBatch(bad_off)
     assert e.value.code == ca.KC_ERR_INVALID
+
+
+def test_create_rejects_one_way_edges():
+    # Both contraction paths read every cluster edge from both of its ends, so raw CSR handed over
+    # by a caller must hold each edge in both rows; a one-way edge is refused before any device work.
+    one_way = ca.Graph(np.array([0, 1, 1, 2], np.uint64), np.array([1, 1], np.uint32))  # 0->1, 2->1
+    with pytest.raises(ca.KcError) as e:
+        ca.DeviceGraph(one_way)
+    assert e.value.code == ca.KC_ERR_INVALID and "not symmetric" in str(e.value) and "0 -> 1" in str(e.value)
+    # a larger graph with a single reverse slot redirected
+    g = ca.generate("gnm_undirected", n=1 << 12, m=1 << 14, seed=5)
+    tgt = g.targets.copy()
+    row = int(np.flatnonzero(np.diff(g.offsets.astype(np.int64)) >= 2)[7])
+    a = int(g.offsets[row])
+    tgt[a] = tgt[a + 1]  # the row now names one neighbour twice and loses the other
+    with pytest.raises(ca.KcError) as e:
+        ca.DeviceGraph(ca.Graph(g.offsets, tgt))
+    assert e.value.code == ca.KC_ERR_INVALID and "not symmetric" in str(e.value)
+    # packed batches are checked per graph, with local ids
+    good = ca.BatchGraphs.pack([ca.Graph.from_edges(3, [(0, 1), (1, 2)]), ca.Graph.from_edges(3, [(0, 2)])])
+    bad = ca.BatchGraphs(good.vtx_off, good.edge_off, good.offsets, good.targets.copy())
+    e0 = int(good.edge_off[1])
+    bad.targets[e0 + 1] = 1  # graph 1: 2 -> 1 instead of 2 -> 0
+    with pytest.raises(ca.KcError) as e:
+        ca.DeviceBatch(bad)
+    assert e.value.code == ca.KC_ERR_INVALID and "graph 1 is not symmetric" in str(e.value)
+    # symmetric input passes the check and fails only for want of a device here
+    if not os.path.exists("/dev/kfd"):
+        for build in (lambda: ca.DeviceGraph(g), lambda: ca.DeviceBatch(good)):
+            with pytest.raises(ca.KcError) as e:
+                build()
+            assert e.value.code == ca.KC_ERR_HIP
