@@ -43,7 +43,10 @@ def parse():
     ap.add_argument("--schedule", choices=["rounds", "sweeps"], default="rounds",
                     help="rounds: time-stepped push (each edge pushed once); sweeps: in-place pull sweeps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the label check")
-    ap.add_argument("--cpu-scale", type=int, default=21, help="R-MAT scale of the bounded CPU sample")
+    ap.add_argument("--cpu-scale", type=int, default=24,
+                    help="R-MAT scale of the CPU baseline's graph (the bench workload's scale by default)")
+    ap.add_argument("--cpu-ranks", type=int, default=0, help="MPI ranks of the CPU baseline (0: min(16, nproc))")
+    ap.add_argument("--cpu-iterations", type=int, default=1, help="runs of the CPU baseline (seeds 1001, 1002, ...)")
     ap.add_argument("--tmax", type=int, default=0)
     ap.add_argument("--smax", type=int, default=0)
     ap.add_argument("--wmax", type=int, default=0)
@@ -197,6 +200,7 @@ def main():
         "cpu_baseline": None,
     }
 
+    out["forms"] = forms_leg(a, dg, run_kw, m)
     if not a.no_cpu:
         out["cpu_baseline"], out["labels_bit_exact"] = cpu_leg(a, dg, run_kw, stream)
     dg.close()
@@ -207,6 +211,35 @@ def main():
     if not a.no_learned:
         out["learned"] = learned_leg(a, stream, run_kw)
     print(json.dumps(out), flush=True)
+
+
+def forms_leg(a, dg, run_kw, m):
+    """The headline workload under every (deviate stream, schedule) pair: the
+    reference's own draw (exponential_contraction.h:467-471) and the hashed
+    one, the in-place pull sweeps of RunContractionLP's default loop (:518-586)
+    and the time-stepped rounds.  Both schedules reach the same fixed point, so
+    levels and labels depend on the stream only."""
+    import numpy as np
+
+    out = {}
+    for stream_name, st_id in (("reference", 0), ("hashed", 1)):
+        for sched in ("rounds", "sweeps"):
+            kw = dict(run_kw, stream=st_id, use_bfs=sched == "rounds")
+            for i in range(a.warmup):
+                dg.find_components(copy_labels=False, seed=1 + i + 1000, **kw)
+            sts = []
+            t0 = time.perf_counter()
+            for i in range(a.steps):
+                sts.append(dg.find_components(copy_labels=False, seed=1 + i + 1000, **kw)[1])
+            ms = (time.perf_counter() - t0) * 1e3 / a.steps
+            out[f"{stream_name}/{sched}"] = {
+                "value": round(m / (ms / 1e3) / 1e9, 4), "unit": "GTEPS", "ms_per_step": round(ms, 4),
+                "levels": float(np.mean([s["levels"] for s in sts])),
+                "sweeps": float(np.mean([s["sweeps"] for s in sts])),
+                "radius_sum": float(np.mean([s["radius_sum"] for s in sts]))}
+            print(f"[bench] forms: {stream_name}/{sched}: {ms:.2f} ms, {out[f'{stream_name}/{sched}']['levels']} "
+                  f"levels", file=sys.stderr, flush=True)
+    return out
 
 
 def configs_leg(a, stream, run_kw):
@@ -387,14 +420,11 @@ def learned_leg(a, stream, run_kw):
                          "levels": float(np.mean([s["levels"] for s in sts])),
                          "sweeps": float(np.mean([s["sweeps"] for s in sts])),
                          "radius_sum": float(np.mean([s["radius_sum"] for s in sts]))}
-        if host is not None:
-            ref = oracle.canonical_labels(host.offsets, host.targets)
-            res["labels_bit_exact"] = bool(np.array_equal(labels["plain"], ref) and
-                                           np.array_equal(labels["learned"], ref))
-            del ref
-        else:
-            res["labels_bit_exact"] = bool(np.array_equal(labels["plain"], labels["learned"]))
-            res["labels_note"] = "learned labels equal the plain path's (checked against the oracle elsewhere)"
+        if host is None:
+            host = dg.download()
+        ref = oracle.canonical_labels(host.offsets, host.targets)
+        res["labels_bit_exact"] = bool(np.array_equal(labels["plain"], ref) and np.array_equal(labels["learned"], ref))
+        del ref, host
         dg.close()
         evals[label] = res
         print(f"[bench] learned: {label}: plain {res['plain']['ms_per_step']:.2f} ms {res['plain']['levels']} levels, "
@@ -408,10 +438,34 @@ def learned_leg(a, stream, run_kw):
             "alpha": 1.0, "eval": evals}
 
 
+def host_info():
+    """CPU model (lscpu) and the number of online cores (nproc) of this host."""
+    import subprocess
+
+    model = "unknown"
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=30).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = os.cpu_count() or 1
+    return model, nproc
+
+
 def cpu_leg(a, dg, run_kw, stream):
-    """The oracle to check the GPU labels (its union-find labels of the full
-    workload) and as the CPU baseline (its exponential contraction timed on a
-    bounded sample of the same workload, R-MAT at --cpu-scale)."""
+    """The GPU labels of the full workload against the oracle's union-find, and
+    the host-CPU baseline: the MPI form of exponential contraction
+    (oracle/mpi_exp.cpp, the reference's default FindComponents path over
+    `mpirun -np P` ranks, P = --cpu-ranks) on the same R-MAT graph, timed as the
+    reference times it (max over ranks), its labels checked too.  The
+    union-find's single-thread time is the floor beside it."""
+    import tempfile
+
     import numpy as np
 
     import components_amd as ca
@@ -419,20 +473,46 @@ def cpu_leg(a, dg, run_kw, stream):
 
     labels, _ = dg.find_components(seed=1001, **run_kw)
     g = dg.download()
-    ref = oracle.canonical_labels(g.offsets, g.targets)
-    exact = bool(np.array_equal(labels, ref))
-    del g, ref
-    sample = ca.generate("rmat", n=1 << a.cpu_scale, m=a.edge_factor << a.cpu_scale, seed=3, rmat_a=0.57,
-                         rmat_b=0.19, rmat_c=0.19)
-    seq = max(sample.m // 256, 4096)
     t0 = time.perf_counter()
-    run = oracle.Run(sample.offsets, sample.targets, variant="exp", seed=1001, seq_limit=seq, stream=stream)
-    secs = time.perf_counter() - t0
-    ok = bool(np.array_equal(run.labels(), oracle.canonical_labels(sample.offsets, sample.targets)))
-    cpu = {"value": round(sample.m / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-           "sample": f"R-MAT scale {a.cpu_scale}, edge factor {a.edge_factor}, same (a,b,c), seed 3 "
-                     f"(n={sample.n}, m={sample.m}); oracle exponential contraction, 1 thread, "
-                     f"{secs:.2f} s, labels canonical={ok}"}
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    uf_s = time.perf_counter() - t0
+    exact = bool(np.array_equal(labels, ref))
+    model, nproc = host_info()
+    ranks = a.cpu_ranks if a.cpu_ranks > 0 else max(1, min(16, nproc))
+    print(f"[bench] cpu: {model}, nproc {nproc}; union-find {uf_s:.2f} s", file=sys.stderr, flush=True)
+    if a.cpu_scale != a.scale:  # a bounded sample: the same generator at a smaller scale
+        del g
+        g = ca.generate("rmat", n=1 << a.cpu_scale, m=a.edge_factor << a.cpu_scale, seed=3, rmat_a=0.57,
+                        rmat_b=0.19, rmat_c=0.19)
+        ref = oracle.canonical_labels(g.offsets, g.targets)
+    if not oracle.mpi_available():
+        raise RuntimeError("the MPI baseline (oracle/build/mpi_exp, /opt/conda/bin/mpirun) is missing")
+    tmp = "/dev/shm" if os.path.isdir("/dev/shm") else tempfile.gettempdir()
+    csr = os.path.join(tmp, f"kc_bench_{os.getpid()}.csr")
+    lab = os.path.join(tmp, f"kc_bench_{os.getpid()}.labels")
+    try:
+        oracle.write_csr_file(csr, g.offsets, g.targets)
+        t0 = time.perf_counter()
+        res = oracle.mpi_exp(csr, ranks, seed=1, iterations=a.cpu_iterations, labels_path=lab, timeout=900)
+        wall = time.perf_counter() - t0
+    finally:
+        for f in (csr, lab):
+            if os.path.exists(f):
+                os.remove(f)
+    ok = bool(np.array_equal(res["labels"], ref))
+    del g, ref
+    print(f"[bench] cpu: mpirun -np {ranks}: {res['time']:.3f} s per run, {res['levels']} levels, "
+          f"{res['sweeps']} sweeps, labels {ok} ({wall:.1f} s)", file=sys.stderr, flush=True)
+    cpu = {"value": round(res["m"] / res["time"] / 1e9, 5), "unit": "GTEPS", "cores": ranks, "kind": "port",
+           "form": f"mpi: mpirun -np {ranks} (MPICH 3.3.2), one rank per core",
+           "sample": f"R-MAT scale {a.cpu_scale}, edge factor {a.edge_factor}, (0.57,0.19,0.19,0.05), seed 3 "
+                     f"(n={res['n']}, m={res['m']}), the {'same graph as value' if a.cpu_scale == a.scale else 'a smaller graph'}; "
+                     f"oracle/mpi_exp.cpp: block partition, local phase, ghost-key exchange per sweep, reference "
+                     f"deviate stream, seq limit m/P = {res['seq_limit']}; {res['iterations']} run(s) with seed 1001, "
+                     f"max over ranks",
+           "seconds": round(res["time"], 4), "levels": res["levels"], "sweeps": res["sweeps"],
+           "labels_bit_exact": ok, "host_cpu": model, "nproc": nproc,
+           "union_find_1thread_s": round(uf_s, 3)}
     return cpu, exact
 
 

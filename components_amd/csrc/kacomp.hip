@@ -310,7 +310,6 @@ constexpr uint64_t kHugeBitmapBytes = 2ull << 30;  // above this the huge bucket
 constexpr uint64_t kHugeMapBytes = 8ull << 30;     // byte maps of the largest huge buckets
 constexpr uint64_t kMapMinShare = 32;              // byte maps from C / 32 elements on
 constexpr uint64_t kHubPullMin = 65536;            // smallest degree of a hub that pulls round 0
-constexpr uint32_t kMinWaveBin = 256;
 
 
 void sync(kc_graph *g) { KC_HIP(hipStreamSynchronize(g->stream)); }
@@ -334,14 +333,7 @@ void ensure_table(kc_graph *g, double beta) {
 }
 
 BinCfg bin_cfg(const kc_options *o) {
-  BinCfg c;
-  c.tmax = o->sweep_group_small > 0 ? static_cast<uint32_t>(o->sweep_group_small) : 16u;
-  c.smax = o->sweep_group_mid > 0 ? static_cast<uint32_t>(o->sweep_group_mid) : 256u;
-  c.wmax = o->sweep_group_large > 0 ? static_cast<uint32_t>(o->sweep_group_large) : 4096u;
-  if (c.wmax < kMinWaveBin) c.wmax = kMinWaveBin;
-  if (c.smax > c.wmax) c.smax = c.wmax;
-  if (c.tmax > c.smax) c.tmax = c.smax;
-  return c;
+  return bin_config(o->sweep_group_small, o->sweep_group_mid, o->sweep_group_large);
 }
 
 // Bins of the level CSR (one sync to size the binned launches).  With key,
@@ -1073,7 +1065,7 @@ void init_work(kc_graph *g) {
   g->key2 = g->dalloc<uint64_t>(n);
   g->deg_inter = g->dalloc<uint32_t>(n);
   g->lists = g->dalloc<uint32_t>(3 * n);
-  g->hubs_cap = m2 / (kMinWaveBin / 2) + 2;
+  g->hubs_cap = hub_chunk_cap(m2);
   g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
   g->flists = g->dalloc<uint32_t>(3 * n);
   g->fhubs = g->dalloc<HubChunk>(g->hubs_cap);

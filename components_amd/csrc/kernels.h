@@ -22,6 +22,24 @@ struct BinCfg {
   uint32_t wmax;  // degree <= wmax: one wave per vertex; above: wave chunks of wmax edges
 };
 
+constexpr uint32_t kMinWaveBin = 256;  // smallest one-wave bin bound
+
+// Bin thresholds from the options' tuning fields (0 = built-in defaults 16,
+// 256, 4096), clamped to tmax <= smax <= wmax and wmax >= kMinWaveBin.
+inline BinCfg bin_config(int32_t small, int32_t mid, int32_t large) {
+  BinCfg c;
+  c.tmax = small > 0 ? static_cast<uint32_t>(small) : 16u;
+  c.smax = mid > 0 ? static_cast<uint32_t>(mid) : 256u;
+  c.wmax = large > 0 ? static_cast<uint32_t>(large) : 4096u;
+  if (c.wmax < kMinWaveBin) c.wmax = kMinWaveBin;
+  if (c.smax > c.wmax) c.smax = c.wmax;
+  if (c.tmax > c.smax) c.tmax = c.smax;
+  return c;
+}
+// Hub chunks a level of m2 edge slots can need: rows above wmax are cut into
+// chunks of wmax slots, so at most m2 / (kMinWaveBin / 2) + 2 of them.
+inline uint64_t hub_chunk_cap(uint64_t m2) { return m2 / (kMinWaveBin / 2) + 2; }
+
 // Degree bins of one level; lists and chunks live on the device.
 struct Bins {
   uint32_t *listT = nullptr, *listS = nullptr, *listW = nullptr;

@@ -196,3 +196,53 @@ def level_features(offsets, targets, round_index=0):
     lib().or_level_features(n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
                             round_index, X.ctypes.data_as(_P(ctypes.c_float)))
     return X[:n]
+
+
+MPI_EXP = os.path.join(_HERE, "build", "mpi_exp")
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+def write_csr_file(path, offsets, targets):
+    """CSR file read by mpi_exp: n and 2m as u64, the offsets (u64[n+1]), the targets (u32[2m])."""
+    off = np.ascontiguousarray(offsets, np.uint64)
+    tgt = np.ascontiguousarray(targets, np.uint32)
+    with open(path, "wb") as f:
+        np.array([len(off) - 1, int(off[-1])], np.uint64).tofile(f)
+        off.tofile(f)
+        tgt[: int(off[-1])].tofile(f)
+
+
+def mpi_available():
+    return os.path.exists(MPI_EXP) and os.path.exists(MPIRUN)
+
+
+def mpi_exp(csr_path, ranks, seed=1, beta=0.1, iterations=1, seq=1, fixed_seq=False, local_phase=True,
+            labels_path=None, timeout=600):
+    """Runs the MPI form of exponential contraction on `ranks` processes (mpirun -np ranks) over a
+    CSR file; returns the GLOBAL RESULT fields (time is the mean over iterations of the maximum over
+    the ranks, in seconds) and, with labels_path, the canonical labels."""
+    import subprocess
+    cmd = [MPIRUN, "-np", str(ranks), MPI_EXP, csr_path, "-seed", str(seed), "-beta", repr(beta), "-i", str(iterations),
+           "-seq", str(seq)]
+    if fixed_seq:
+        cmd.append("-fixed-seq")
+    if not local_phase:
+        cmd.append("-no-local")
+    if labels_path:
+        cmd += ["-labels", labels_path]
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", "1")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"mpi_exp failed ({r.returncode}): {r.stderr[-2000:]}")
+    line = [x for x in r.stdout.splitlines() if x.startswith("GLOBAL RESULT")][-1]
+    res = {}
+    for tok in line.split()[2:]:
+        k, _, v = tok.partition("=")
+        res.setdefault(k, v)
+    out = {"time": float(res["time"]), "stddev": float(res["stddev"]), "ranks": int(res["ranks"]),
+           "n": int(res["n"]), "m": int(res["m"]), "seq_limit": int(res["seq_limit"]), "levels": int(res["levels"]),
+           "sweeps": int(res["sweeps"]), "iterations": int(res["iterations"])}
+    if labels_path:
+        out["labels"] = np.fromfile(labels_path, dtype=np.uint32)
+    return out

@@ -124,8 +124,11 @@ def test_learned_labels_exact():
     try:
         rng = np.random.default_rng(2)
         graphs = [ca.generate("rmat", n=1 << 13, m=1 << 16, seed=9), ca.generate("gnm_undirected", n=3000, m=2500, seed=4),
-                  ca.Graph.from_edges(500, rng.integers(0, 500, size=(400, 2)))]
+                  ca.Graph.from_edges(500, rng.integers(0, 500, size=(400, 2))),
+                  # the learned evaluation's graph class: hyperbolic, gamma 2.8, average degree 16
+                  ca.generate("rhg", n=1 << 15, gamma=2.8, d=16.0, seed=5)]
         for g in graphs:
+            print(f"learned labels: n={g.n} m={g.m}", flush=True)
             ref = oracle.canonical_labels(g.offsets, g.targets)
             dg = ca.DeviceGraph(g)
             dg.set_mlp(m)
