@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--tmax", type=int, default=0)
     ap.add_argument("--smax", type=int, default=0)
     ap.add_argument("--wmax", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in degree-bin thresholds")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other plain workloads")
@@ -134,6 +135,12 @@ def main():
     print(f"[bench] rmat scale {a.scale}: n={n} m={m} non_isolated={dg.non_isolated} built in {gen_s:.1f}s",
           file=sys.stderr, flush=True)
 
+    # degree-bin thresholds tuned on this graph (kept in the handle for every call below that leaves
+    # them at 0); results do not depend on them
+    bins = None
+    if not (a.tmax or a.smax or a.wmax) and not a.no_autotune:
+        bins = dg.autotune(seed=1001, **run_kw)
+        print(f"[bench] bins tuned: {bins}", file=sys.stderr, flush=True)
     for i in range(a.warmup):
         dg.find_components(copy_labels=False, seed=1 + i + 1000, **run_kw)
     # per-kernel-class breakdown, outside the timed region: every launch
@@ -184,6 +191,10 @@ def main():
         "config": {"workload": f"rmat-scale{a.scale}-ef{a.edge_factor}-abcd(0.57,0.19,0.19,0.05)-seed3",
                    "n": n, "m": m, "non_isolated": dg.non_isolated, "form": "plain",
                    "deviate_stream": a.stream, "schedule": a.schedule, "beta": 0.1, "seq_limit": seq_limit,
+                   "bins": {"small": stats[-1]["bin_small"], "mid": stats[-1]["bin_mid"],
+                            "large": stats[-1]["bin_large"],
+                            "tuned": None if bins is None else {k: (round(v, 4) if isinstance(v, float) else v)
+                                                                for k, v in bins.items()}},
                    "parallelism": "1 GPU"},
         "levels": stats[-1]["levels"],
         "sweeps": stats[-1]["sweeps"],

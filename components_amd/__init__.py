@@ -54,7 +54,8 @@ class Options(ctypes.Structure):
                 ("input_order", ctypes.c_int32), ("tail_sweeps", ctypes.c_int32),
                 ("pull_rounds", ctypes.c_int32),
                 ("dedup_huge", ctypes.c_int32), ("time_class", ctypes.c_int32),
-                ("pre_contract", ctypes.c_int32), ("pre_blocks", ctypes.c_uint32)]
+                ("pre_contract", ctypes.c_int32), ("pre_blocks", ctypes.c_uint32),
+                ("replicate", ctypes.c_int32), ("hd_threshold", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -63,7 +64,14 @@ class Stats(ctypes.Structure):
                 ("m", ctypes.c_uint64), ("levels", ctypes.c_uint32), ("components", ctypes.c_uint32),
                 ("sweeps", ctypes.c_uint64), ("radius_sum", ctypes.c_uint64), ("alg_bytes", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_double * _NK), ("kernel_launches", ctypes.c_uint64 * _NK),
-                ("kernel_bytes", ctypes.c_uint64 * _NK)]
+                ("kernel_bytes", ctypes.c_uint64 * _NK), ("replicas", ctypes.c_uint64),
+                ("bin_small", ctypes.c_uint32), ("bin_mid", ctypes.c_uint32), ("bin_large", ctypes.c_uint32),
+                ("hubs", ctypes.c_uint32)]
+
+
+class BinTuning(ctypes.Structure):
+    _fields_ = [("small", ctypes.c_int32), ("mid", ctypes.c_int32), ("large", ctypes.c_int32), ("ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double), ("candidates", ctypes.c_int32)]
 
 
 class GenParams(ctypes.Structure):
@@ -129,6 +137,8 @@ _SIGNATURES = {
     "kc_graph_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
     "kc_find_components": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(ctypes.c_uint32), _P(Stats)]),
     "kc_copy_labels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
+    "kc_graph_autotune": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(BinTuning)]),
+    "kc_graph_set_bins": (ctypes.c_int, [ctypes.c_void_p, _P(BinTuning)]),
     "kc_component_histogram": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_uint64,
                                               _P(ctypes.c_uint64)]),
     "kc_trace_levels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
@@ -322,7 +332,7 @@ PRE_MODES = {"none": 0, "local": 1, "single": 2, "double": 3}
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
             tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
             alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_huge="auto", time_class=None,
-            pre_contract="none", pre_blocks=1):
+            pre_contract="none", pre_blocks=1, replicate=False, hd_threshold=16.0):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -347,6 +357,8 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     # local components over pre_blocks blocks of consecutive ids contracted first (a name of PRE_MODES)
     o.pre_contract = PRE_MODES[pre_contract] if isinstance(pre_contract, str) else int(pre_contract)
     o.pre_blocks = int(pre_blocks)
+    o.replicate = 1 if replicate else 0
+    o.hd_threshold = float(hd_threshold)
     return o
 
 
@@ -432,6 +444,23 @@ class DeviceGraph:
         _check(lib().kc_find_components(self.h, ctypes.byref(o), _u32p(labels) if copy_labels else None,
                                         ctypes.byref(st)))
         return (labels[: self.n] if copy_labels else None), stats_dict(st)
+
+    def autotune(self, **kw):
+        """kc_graph_autotune: times the degree-bin threshold candidates with these options and keeps
+        the fastest in the handle; returns the choice."""
+        o = options(**kw)
+        t = BinTuning()
+        _check(lib().kc_graph_autotune(self.h, ctypes.byref(o), ctypes.byref(t)))
+        return {"small": t.small, "mid": t.mid, "large": t.large, "ms": t.ms, "default_ms": t.default_ms,
+                "candidates": t.candidates}
+
+    def set_bins(self, small=None, mid=None, large=None):
+        """Thresholds kept in the handle (all None: cleared)."""
+        if small is None:
+            _check(lib().kc_graph_set_bins(self.h, None))
+        else:
+            t = BinTuning(small, mid, large, 0.0, 0.0, 0)
+            _check(lib().kc_graph_set_bins(self.h, ctypes.byref(t)))
 
     def labels(self):
         labels = np.empty(max(self.n, 1), np.uint32)

@@ -118,6 +118,17 @@ KC_HD inline uint32_t uniform_deviate_100(uint32_t gid, int32_t seed, uint32_t i
   return static_cast<uint32_t>(((z >> 32) * 100ull) >> 32);
 }
 
+// Proxy slot in [0, k) of the edge (v, w) of a hub v split into k proxies
+// (kc_options.replicate): SplitMix64 over the ids and the seed, reduced by a
+// multiply-shift of the upper 32 bits.  The reference draws it from
+// default_random_engine(seed + v * n + w) (exponential_contraction.h:1066-1072);
+// only the spread over the proxies matters, not the stream.
+KC_HD inline uint32_t replica_pick(uint32_t gv, uint32_t gw, int32_t seed, uint32_t k) {
+  const uint64_t z = splitmix64_mix((static_cast<uint64_t>(gv) << 32 | gw) ^
+                                    (0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(static_cast<uint32_t>(seed)) + 1)));
+  return static_cast<uint32_t>(((z >> 32) * k) >> 32);
+}
+
 // Deviate of one vertex: stream 0 = reference draw, 1 = hashed draw.
 KC_HD inline uint32_t exp_deviate(uint32_t gid, int32_t seed, uint32_t iteration, uint64_t rng_offset,
                                   bool fused, int stream, const double *table, uint32_t k_max) {

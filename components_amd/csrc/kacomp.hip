@@ -275,18 +275,50 @@ struct kc_graph {
   float *feat = nullptr, *target = nullptr;
   uint32_t *ldeg = nullptr, *csize = nullptr;
   uint8_t *delay = nullptr;
-  std::vector<void *> owned;
+  uint64_t learned_cap = 0;  // vertices the learned buffers hold
+  BinCfg tuned{0, 0, 0};     // thresholds kept by kc_graph_autotune (tmax 0: none)
+  // level-sized work buffers hold levels of up to cap_n vertices and cap_m2
+  // edge slots (the input's level 0, or a larger replicated one)
+  uint64_t cap_n = 0, cap_m2 = 0;
+  std::vector<void *> owned;       // freed with the handle
+  std::vector<void *> work_owned;  // the level-sized work set, freed when it grows
 
   template <class T>
-  T *dalloc(uint64_t count) {
+  T *alloc_into(std::vector<void *> &list, uint64_t count) {
     void *p = nullptr;
     size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T) + 64;
     if (hipMalloc(&p, bytes) != hipSuccess) fail(KC_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
-    owned.push_back(p);
+    list.push_back(p);
     return static_cast<T *>(p);
+  }
+  template <class T>
+  T *dalloc(uint64_t count) {
+    return alloc_into<T>(owned, count);
+  }
+  template <class T>
+  T *walloc(uint64_t count) {
+    return alloc_into<T>(work_owned, count);
+  }
+  // Replaces a grown buffer: the stream is drained, the old one freed.
+  template <class T>
+  void drealloc(T *&p, uint64_t count) {
+    if (p) {
+      (void)hipStreamSynchronize(stream);
+      auto it = std::find(owned.begin(), owned.end(), static_cast<void *>(p));
+      if (it != owned.end()) owned.erase(it);
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    p = dalloc<T>(count);
+  }
+  void free_work() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : work_owned) (void)hipFree(p);
+    work_owned.clear();
   }
   ~kc_graph() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : work_owned) (void)hipFree(p);
     for (void *p : owned) (void)hipFree(p);
     if (h_sizes) (void)hipHostFree(h_sizes);
     if (h_small) (void)hipHostFree(h_small);
@@ -324,7 +356,7 @@ void ensure_table(kc_graph *g, double beta) {
   std::vector<double> t = build_deviate_table(beta);
   if (t.size() + 1 > g->table_cap) {
     g->table_cap = t.size() + 1;
-    g->d_table = g->dalloc<double>(g->table_cap);
+    g->drealloc(g->d_table, g->table_cap);
   }
   if (!t.empty())
     KC_HIP(hipMemcpy(g->d_table, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -332,7 +364,11 @@ void ensure_table(kc_graph *g, double beta) {
   g->table_beta = beta;
 }
 
-BinCfg bin_cfg(const kc_options *o) {
+// The options' thresholds, or the handle's tuned ones when the options leave
+// all three at 0.
+BinCfg bin_cfg(const kc_graph *g, const kc_options *o) {
+  if (g && g->tuned.tmax && o->sweep_group_small <= 0 && o->sweep_group_mid <= 0 && o->sweep_group_large <= 0)
+    return g->tuned;
   return bin_config(o->sweep_group_small, o->sweep_group_mid, o->sweep_group_large);
 }
 
@@ -347,14 +383,18 @@ Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg, 
       g->l0_cfg.wmax == cfg.wmax)
     return g->l0_bins;
   if (level0 && !g->l0_lists) {
-    g->l0_lists = g->dalloc<uint32_t>(3 * g->n);
-    g->l0_hubs = g->dalloc<HubChunk>(g->hubs_cap);
+    g->l0_lists = g->walloc<uint32_t>(3 * g->cap_n);
+    g->l0_hubs = g->walloc<HubChunk>(g->hubs_cap);
   }
+  // three sections of cap_n entries each: a level holds at most cap_n vertices
+  // (a replicated level 0 can hold more than the input's n)
+  if (n > g->cap_n) fail(KC_ERR_INVALID, "internal: level of %llu vertices above the work set's %llu",
+                         static_cast<unsigned long long>(n), static_cast<unsigned long long>(g->cap_n));
   Bins b;
   uint32_t *lists = key ? g->flists : level0 ? g->l0_lists : g->lists;
   b.listT = lists;
-  b.listS = lists + g->n;
-  b.listW = lists + 2 * g->n;
+  b.listS = lists + g->cap_n;
+  b.listW = lists + 2 * g->cap_n;
   b.hubs = key ? g->fhubs : level0 ? g->l0_hubs : g->hubs;
   {
     TimedScope ts(g->timer, KC_K_BIN, (key ? 24 : 16) * n + 8);
@@ -452,7 +492,7 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   if (!n_huge) return first;
   if (3 * n_huge + 6 > g->huge_info_cap) {
     g->huge_info_cap = 4 * n_huge + 64;
-    g->d_huge_info = g->dalloc<uint64_t>(g->huge_info_cap);
+    g->drealloc(g->d_huge_info, g->huge_info_cap);
   }
   // Bitmaps over the label space when they fit their cap, the buckets of at
   // least map_min elements against byte maps instead: those cost two passes
@@ -479,13 +519,13 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   }
   if (n_maps * stride > g->huge_maps_cap) {  // maps start zero and every call leaves them zero
     g->huge_maps_cap = std::min(kHugeMapBytes, n_maps * stride + (n_maps * stride >> 2));
-    g->huge_maps = g->dalloc<uint8_t>(g->huge_maps_cap);
+    g->drealloc(g->huge_maps, g->huge_maps_cap);
     fill_u32(reinterpret_cast<uint32_t *>(g->huge_maps), g->huge_maps_cap / 4, 0u, s);
   }
   if (mode == KC_DEDUP_BYTEMAPS || mode == KC_DEDUP_BITMAPS || (mode == KC_DEDUP_AUTO && bits_fit)) {
     if (map_min > kDedupBlockCap + 1 && n_huge * words > g->huge_bits_cap) {
       g->huge_bits_cap = n_huge * words + (n_huge * words >> 2);
-      g->huge_bits = g->dalloc<uint32_t>(g->huge_bits_cap);
+      g->drealloc(g->huge_bits, g->huge_bits_cap);
     }
     TimedScope ts(g->timer, KC_K_DEDUP, 0);
     dedup_huge_mixed(static_cast<uint32_t>(n_huge), huge, seg_off, g->m2 / kBitsChunk + n_huge, map_min, n_maps, true,
@@ -526,14 +566,14 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   }
   if (elem.size() + slot.size() > g->chunks_cap) {
     g->chunks_cap = elem.size() + slot.size() + 1024;
-    g->d_chunks = g->dalloc<HugeChunk>(g->chunks_cap);
+    g->drealloc(g->d_chunks, g->chunks_cap);
   }
   KC_HIP(hipMemcpyAsync(g->d_chunks, elem.data(), elem.size() * sizeof(HugeChunk), hipMemcpyHostToDevice, s));
   KC_HIP(hipMemcpyAsync(g->d_chunks + elem.size(), slot.data(), slot.size() * sizeof(HugeChunk),
                         hipMemcpyHostToDevice, s));
   if (huge_slots > g->huge_cap) {
     g->huge_cap = huge_slots + (huge_slots >> 2);
-    g->huge_tables = g->dalloc<uint32_t>(g->huge_cap);
+    g->drealloc(g->huge_tables, g->huge_cap);
   }
   if (n_huge > g->huge_info_cap) fail(KC_ERR_INVALID, "huge bucket list overflow");
   fill_u32(g->huge_tables, huge_slots, kNone, s);
@@ -658,15 +698,138 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
   return done;
 }
 
+void ensure_work(kc_graph *g, uint64_t n, uint64_t m2);
+void ensure_learned_buffers(kc_graph *g, uint64_t n = 0);
+
 struct RunOut {
   uint64_t sweeps = 0, radius_sum = 0;
   uint32_t levels = 0;
+  uint64_t replicas = 0;
+  uint32_t hubs = 0;
 };
+
+// Level 0 with its hubs split into proxies (kc_options.replicate): the
+// reference's DistributeHighDegreeVertices (exponential_contraction.h:935-1253)
+// with every vertex its own PE, so that every edge of a hub is a remote one.
+// A hub (degree >= hd_threshold * sqrt(n0), utils.h:108-117) of degree d gets
+// floor(sqrt(d)) proxy slots; each of its edges goes to the slot replica_pick
+// names, and an edge between two hubs joins the two proxies.  The slots that
+// receive an edge become proxies, numbered n0, n0 + 1, ... in hub order, each
+// joined to its hub; the hub's row lists its proxies.  Proxies take their
+// hub's id (they draw its deviate, :483-503) and, with indices above every
+// vertex, lose every tie.  Returns false when there is no hub.
+struct Replicated {
+  uint64_t n = 0, m2 = 0;
+  uint64_t *off = nullptr;
+  uint32_t *col = nullptr, *gid = nullptr;
+  uint32_t H = 0, R = 0;
+};
+
+template <class T>
+T *upload(kc_graph *g, const std::vector<T> &v) {
+  T *d = g->arena.alloc_n<T>(v.size() + 1);
+  if (!v.empty()) KC_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+bool build_replicated(kc_graph *g, const kc_options *o, uint64_t n0, uint64_t m2, const uint64_t *off,
+                      const uint32_t *col, const uint32_t *gid, Replicated &out) {
+  constexpr uint64_t kChunk = 16384;  // hub edges per block
+  hipStream_t s = g->stream;
+  if (!(o->hd_threshold >= 0.0)) fail(KC_ERR_INVALID, "hd_threshold must be >= 0");
+  const uint64_t limit = std::max<uint64_t>(1, static_cast<uint64_t>(o->hd_threshold * std::sqrt(static_cast<double>(n0))));
+  uint32_t *list = g->arena.alloc_n<uint32_t>(std::min<uint64_t>(n0, m2 / limit + 1) + 1);
+  find_hubs(n0, off, limit, list, g->small + 6, s);
+  uint32_t H = 0;
+  KC_HIP(hipMemcpyAsync(&H, g->small + 6, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  sync(g);
+  if (H == 0) return false;
+  std::vector<uint32_t> hubs(H);
+  KC_HIP(hipMemcpy(hubs.data(), list, 4ull * H, hipMemcpyDeviceToHost));
+  std::sort(hubs.begin(), hubs.end());
+  // the hubs' row bounds
+  std::vector<uint64_t> rows(2ull * H);
+  const uint32_t *d_hubs = upload(g, hubs);
+  uint64_t *d_rows = g->arena.alloc_n<uint64_t>(2ull * H);
+  row_bounds(H, d_hubs, off, d_rows, s);
+  KC_HIP(hipMemcpyAsync(rows.data(), d_rows, 16ull * H, hipMemcpyDeviceToHost, s));
+  sync(g);
+  std::vector<uint32_t> k(H);
+  std::vector<uint64_t> slot_base(H + 1, 0);
+  std::vector<HubEdgeChunk> chunks;
+  for (uint32_t h = 0; h < H; ++h) {
+    const uint64_t d = rows[2ull * h + 1] - rows[2ull * h];
+    // min(P - 1, sqrt(d)) proxies (:1040) with P = n0 PEs
+    k[h] = static_cast<uint32_t>(std::min<uint64_t>(n0 - 1, static_cast<uint64_t>(std::sqrt(static_cast<double>(d)))));
+    if (k[h] == 0) k[h] = 1;
+    slot_base[h + 1] = slot_base[h] + k[h];
+    for (uint64_t e = rows[2ull * h]; e < rows[2ull * h + 1]; e += kChunk)
+      chunks.push_back({h, 0, e, std::min(rows[2ull * h + 1], e + kChunk)});
+  }
+  const uint64_t S = slot_base[H];
+  ReplicaArgs r{};
+  r.off = off;
+  r.col = col;
+  r.gid = gid;
+  r.seed = o->seed;
+  r.hubs = d_hubs;
+  r.k = upload(g, k);
+  r.slot_base = upload(g, slot_base);
+  const HubEdgeChunk *d_chunks = upload(g, chunks);
+  r.hubidx = g->arena.alloc_n<uint32_t>(n0);
+  r.cnt = g->arena.alloc_n<uint32_t>(S);
+  fill_u32(r.cnt, S, 0, s);
+  replica_count(r, n0, H, d_chunks, static_cast<uint32_t>(chunks.size()), s);
+  std::vector<uint32_t> cnt(S);
+  KC_HIP(hipMemcpyAsync(cnt.data(), r.cnt, 4 * S, hipMemcpyDeviceToHost, s));
+  sync(g);
+  // the slots with an edge become proxies, in slot (= hub, then slot) order
+  std::vector<uint32_t> rep_of_slot(S, kNone), nexist(H, 0), first_rep(H, 0), slot_of_rep, root_of_rep;
+  for (uint32_t h = 0; h < H; ++h) {
+    first_rep[h] = static_cast<uint32_t>(n0 + slot_of_rep.size());
+    for (uint64_t sl = slot_base[h]; sl < slot_base[h + 1]; ++sl)
+      if (cnt[sl]) {
+        rep_of_slot[sl] = static_cast<uint32_t>(n0 + slot_of_rep.size());
+        slot_of_rep.push_back(static_cast<uint32_t>(sl));
+        root_of_rep.push_back(hubs[h]);
+        ++nexist[h];
+      }
+  }
+  const uint64_t R = slot_of_rep.size();
+  if (n0 + R >= 0xFFFFFFFFull || m2 + 2 * R >= (1ull << 32))
+    fail(KC_ERR_INVALID, "replicated level 0 exceeds 32-bit ids (%llu proxies)", static_cast<unsigned long long>(R));
+  r.rep_of_slot = upload(g, rep_of_slot);
+  r.nexist = upload(g, nexist);
+  r.first_rep = upload(g, first_rep);
+  r.slot_of_rep = upload(g, slot_of_rep);
+  r.root_of_rep = upload(g, root_of_rep);
+  out.n = n0 + R;
+  out.m2 = m2 + 2 * R;
+  out.H = H;
+  out.R = static_cast<uint32_t>(R);
+  ensure_work(g, out.n, out.m2);
+  uint32_t *len = g->arena.alloc_n<uint32_t>(out.n);
+  out.off = g->arena.alloc_n<uint64_t>(out.n + 1);
+  out.col = g->arena.alloc_n<uint32_t>(out.m2);
+  out.gid = g->arena.alloc_n<uint32_t>(out.n);
+  replica_lengths(r, n0, static_cast<uint32_t>(R), len, s);
+  csr_offsets(out.n, len, out.off, g->sizes, g->scan_scratch, s);
+  uint32_t *cursor = g->arena.alloc_n<uint32_t>(S);
+  fill_u32(cursor, S, 0, s);
+  replica_fill(r, n0, H, static_cast<uint32_t>(R), d_chunks, static_cast<uint32_t>(chunks.size()), out.off, cursor,
+               out.col, out.gid, s);
+  d2h(g, g->h_sizes + 2, g->sizes + 2, sizeof(uint64_t));
+  sync(g);
+  if (g->h_sizes[2] != out.m2)
+    fail(KC_ERR_INVALID, "internal: replicated level 0 has %llu slots, expected %llu",
+         static_cast<unsigned long long>(g->h_sizes[2]), static_cast<unsigned long long>(out.m2));
+  return true;
+}
 
 // Exponential (or local) contraction down to an edgeless graph or the base case.
 void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   const bool local = o->variant == KC_VARIANT_LOCAL;
-  const BinCfg cfg = bin_cfg(o);
+  const BinCfg cfg = bin_cfg(g, o);
   // degree-0 vertices are their own components and take no part in the
   // contraction (the reference sets them inactive at load, graph_io.h:380)
   identity_u32(g->n, g->labels, g->stream);
@@ -679,7 +842,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   // case, whose pointer jumping indexes keys by level index): keys and rows by
   // storage position, key labels and gid_l by level index as before.
   // (the pre-contraction's blocks are ranges of level indices: input order)
-  const bool ordered = g->idx0 && !o->input_order && o->pre_contract == KC_PRE_NONE &&
+  const bool ordered = g->idx0 && !o->input_order && o->pre_contract == KC_PRE_NONE && !o->replicate &&
                        !(o->seq_limit > 0 && m2_l / 2 <= o->seq_limit) && m2_l > 0;
   int pre_left = o->pre_contract == KC_PRE_DOUBLE ? 2 : o->pre_contract != KC_PRE_NONE ? 1 : 0;
   const uint32_t *idx_l = nullptr, *pos_l = nullptr;  // storage maps of the current level (null: identity)
@@ -692,6 +855,22 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     dev_gid_l = g->gidp0;
   }
   const uint32_t *canon_gid = ordered ? g->gidp0 : g->gid0;  // input id of each level-0 storage position
+  if (o->replicate && !local && m2_l > 0) {
+    // level 0 with its hubs split into proxies (input order); the canonical
+    // labels below read the first n0 positions, the real vertices
+    Replicated rp;
+    if (build_replicated(g, o, n_l, m2_l, off_l, col_l, gid_l, rp)) {
+      n_l = rp.n;
+      m2_l = rp.m2;
+      off_l = rp.off;
+      col_l = rp.col;
+      gid_l = rp.gid;
+      dev_gid_l = rp.gid;
+      out.hubs = rp.H;
+      out.replicas = rp.R;
+      if (o->learned || g->collect) ensure_learned_buffers(g, n_l);
+    }
+  }
   uint32_t iteration = 1;
   uint32_t flag_base = 0;
   int pp = 0;
@@ -987,20 +1166,22 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   }
 }
 
-// Buffers of the learned centre proposals (level-0 sized, allocated once).
-void ensure_learned_buffers(kc_graph *g) {
-  if (g->feat) return;
-  const uint64_t n = g->n0;
-  g->feat = g->dalloc<float>(n * kMlpIn);
-  g->target = g->dalloc<float>(n);
-  g->ldeg = g->dalloc<uint32_t>(n);
-  g->csize = g->dalloc<uint32_t>(n);
-  g->delay = g->dalloc<uint8_t>(n);
+// Buffers of the learned centre proposals for levels of up to n vertices
+// (level 0's size unless a larger replicated level 0 needs more).
+void ensure_learned_buffers(kc_graph *g, uint64_t n) {
+  n = std::max(n, g->n0);
+  if (g->feat && n <= g->learned_cap) return;
+  g->drealloc(g->feat, n * kMlpIn);
+  g->drealloc(g->target, n);
+  g->drealloc(g->ldeg, n);
+  g->drealloc(g->csize, n);
+  g->drealloc(g->delay, n);
+  g->learned_cap = n;
 }
 
 // Min-label propagation (labels) or propagation + pointer jumping (shortcuts).
 void run_propagation(kc_graph *g, const kc_options *o, RunOut &out) {
-  const BinCfg cfg = bin_cfg(o);
+  const BinCfg cfg = bin_cfg(g, o);
   uint32_t flag_base = 0;
   fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
   g->levels.clear();
@@ -1035,6 +1216,7 @@ void kc_options_init(kc_options *o) {
   o->deviate_stream = KC_STREAM_REFERENCE;
   o->max_levels = 4096;
   o->learned_alpha = 1.0f;
+  o->hd_threshold = 16.0;
 }
 
 }  // extern "C"
@@ -1058,45 +1240,65 @@ std::unique_ptr<kc_graph> new_handle() {
   return g;
 }
 
+// The level-sized work set for levels of up to cap_n vertices and cap_m2 edge
+// slots (the previous set, if any, is freed first).
+void alloc_work(kc_graph *g, uint64_t cap_n, uint64_t cap_m2) {
+  g->free_work();
+  const uint64_t n = cap_n, m2 = cap_m2;
+  g->cap_n = n;
+  g->cap_m2 = m2;
+  g->key = g->walloc<uint64_t>(n);
+  g->key2 = g->walloc<uint64_t>(n);
+  g->deg_inter = g->walloc<uint32_t>(n);
+  g->lists = g->walloc<uint32_t>(3 * n);
+  g->hubs_cap = hub_chunk_cap(m2);
+  g->hubs = g->walloc<HubChunk>(g->hubs_cap);
+  g->flists = g->walloc<uint32_t>(3 * n);
+  g->fhubs = g->walloc<HubChunk>(g->hubs_cap);
+  g->hub_inter = g->walloc<uint32_t>(g->hubs_cap);
+  g->slots = g->walloc<uint32_t>(m2);
+  g->seg_of_cid = g->walloc<uint32_t>(n);
+  g->crank = g->walloc<uint32_t>(n);
+  g->cid = g->walloc<uint32_t>(n);
+  g->uniq = g->walloc<uint32_t>(n);
+  g->pos_of = g->walloc<uint32_t>(n);
+  g->hub_pos = g->walloc<uint32_t>(g->hubs_cap);
+  g->hub_prefix = g->walloc<uint64_t>(g->hubs_cap + 1);
+  g->sk = g->walloc<uint32_t>(n);
+  g->sv = g->walloc<uint32_t>(n);
+  g->sk2 = g->walloc<uint32_t>(n);
+  g->sv2 = g->walloc<uint32_t>(n);
+  g->seg_off = g->walloc<uint64_t>(n + 1);
+  g->sort_scratch = g->walloc<uint64_t>(radix_sort_scratch_words(n));
+  g->rep_a = g->walloc<uint32_t>(n);
+  g->rep_b = g->walloc<uint32_t>(n);
+  g->scan_scratch = g->walloc<uint64_t>(std::max(scan_scratch_words(n + m2) + 1024, bin_scratch_words(n)));
+  for (int i = 0; i < 2; ++i) {
+    g->off_pp[i] = g->walloc<uint64_t>(n + 1);
+    g->col_pp[i] = g->walloc<uint32_t>(m2);
+  }
+  g->dl_medium = g->walloc<uint32_t>(n);
+  g->dl_huge = g->walloc<uint32_t>(n);
+  g->row_cstart = g->walloc<uint64_t>(n + 2);
+  g->hcount = g->walloc<uint32_t>(n);
+  // level 0's cached bins live in the work set too
+  g->l0_lists = nullptr;
+  g->l0_hubs = nullptr;
+  g->l0_off = nullptr;
+}
+
+// Grows the work set to hold a level of n vertices and m2 edge slots.
+void ensure_work(kc_graph *g, uint64_t n, uint64_t m2) {
+  if (n <= g->cap_n && m2 <= g->cap_m2) return;
+  alloc_work(g, std::max(n, g->cap_n), std::max(m2, g->cap_m2));
+}
+
 // Work buffers sized for level 0 (g->n, g->m2 set, off0/col0 in place).
 void init_work(kc_graph *g) {
   const uint64_t n = g->n, m2 = g->m2;
-  g->key = g->dalloc<uint64_t>(n);
-  g->key2 = g->dalloc<uint64_t>(n);
-  g->deg_inter = g->dalloc<uint32_t>(n);
-  g->lists = g->dalloc<uint32_t>(3 * n);
-  g->hubs_cap = hub_chunk_cap(m2);
-  g->hubs = g->dalloc<HubChunk>(g->hubs_cap);
-  g->flists = g->dalloc<uint32_t>(3 * n);
-  g->fhubs = g->dalloc<HubChunk>(g->hubs_cap);
-  g->hub_inter = g->dalloc<uint32_t>(g->hubs_cap);
-  g->slots = g->dalloc<uint32_t>(m2);
-  g->seg_of_cid = g->dalloc<uint32_t>(n);
-  g->crank = g->dalloc<uint32_t>(n);
-  g->cid = g->dalloc<uint32_t>(n);
-  g->uniq = g->dalloc<uint32_t>(n);
-  g->pos_of = g->dalloc<uint32_t>(n);
-  g->hub_pos = g->dalloc<uint32_t>(g->hubs_cap);
-  g->hub_prefix = g->dalloc<uint64_t>(g->hubs_cap + 1);
-  g->sk = g->dalloc<uint32_t>(n);
-  g->sv = g->dalloc<uint32_t>(n);
-  g->sk2 = g->dalloc<uint32_t>(n);
-  g->sv2 = g->dalloc<uint32_t>(n);
-  g->seg_off = g->dalloc<uint64_t>(n + 1);
-  g->sort_scratch = g->dalloc<uint64_t>(radix_sort_scratch_words(n));
-  g->rep_a = g->dalloc<uint32_t>(n);
-  g->rep_b = g->dalloc<uint32_t>(n);
+  alloc_work(g, n, m2);
   g->minid = g->dalloc<uint32_t>(n);
   g->labels = g->dalloc<uint32_t>(n);
-  g->scan_scratch = g->dalloc<uint64_t>(std::max(scan_scratch_words(n + m2) + 1024, bin_scratch_words(n)));
-  for (int i = 0; i < 2; ++i) {
-    g->off_pp[i] = g->dalloc<uint64_t>(n + 1);
-    g->col_pp[i] = g->dalloc<uint32_t>(m2);
-  }
-  g->dl_medium = g->dalloc<uint32_t>(n);
-  g->dl_huge = g->dalloc<uint32_t>(n);
-  g->row_cstart = g->dalloc<uint64_t>(n + 2);
-  g->hcount = g->dalloc<uint32_t>(n);
   g->flags = g->dalloc<int>(kMaxFlags);
   // level 0 without the degree-0 vertices (rep_a holds the id map meanwhile)
   if (g->non_isolated == n) {
@@ -1650,6 +1852,11 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
         fail(KC_ERR_UNSUPPORTED, "pre-contraction applies to the contraction variants only");
       if (o.pre_blocks < 1) fail(KC_ERR_INVALID, "pre_blocks must be >= 1 with pre_contract %d", o.pre_contract);
     }
+    if (o.replicate) {
+      if (o.variant != KC_VARIANT_EXP) fail(KC_ERR_UNSUPPORTED, "replicate applies to exponential contraction only");
+      if (o.pre_contract != KC_PRE_NONE) fail(KC_ERR_UNSUPPORTED, "replicate does not combine with pre_contract");
+      if (!(o.hd_threshold >= 0.0) || !(o.hd_threshold < 1e12)) fail(KC_ERR_INVALID, "hd_threshold must be in [0, 1e12)");
+    }
     if (o.learned) {
       if (o.variant != KC_VARIANT_EXP) fail(KC_ERR_UNSUPPORTED, "learned centre proposals apply to exponential contraction only");
       if (!g->mlp) fail(KC_ERR_INVALID, "learned centre proposals need a model (kc_graph_set_mlp)");
@@ -1688,10 +1895,84 @@ int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out,
       stats->components = g->components;
       stats->sweeps = out.sweeps;
       stats->radius_sum = out.radius_sum;
+      stats->replicas = out.replicas;
+      stats->hubs = out.hubs;
+      const BinCfg cfg = bin_cfg(g, &o);
+      stats->bin_small = cfg.tmax;
+      stats->bin_mid = cfg.smax;
+      stats->bin_large = cfg.wmax;
       g->timer.finish(stats);
       uint64_t tot = 0;
       for (int i = 0; i < KC_KERNEL_CLASSES; ++i) tot += stats->kernel_bytes[i];
       stats->alg_bytes = tot;
+    }
+  });
+}
+
+int kc_graph_set_bins(kc_graph *g, const kc_bin_tuning *t) {
+  return guarded([&] {
+    if (!g) fail(KC_ERR_INVALID, "null graph");
+    if (!t) {
+      g->tuned = BinCfg{0, 0, 0};
+      return;
+    }
+    if (t->small <= 0 || t->mid <= 0 || t->large <= 0) fail(KC_ERR_INVALID, "bin thresholds must be positive");
+    g->tuned = bin_config(t->small, t->mid, t->large);
+  });
+}
+
+int kc_graph_autotune(kc_graph *g, const kc_options *opt, kc_bin_tuning *chosen) {
+  return guarded([&] {
+    if (!g) fail(KC_ERR_INVALID, "null graph");
+    kc_options o;
+    if (opt)
+      o = *opt;
+    else
+      kc_options_init(&o);
+    o.time_kernels = 0;
+    o.record_trace = 0;
+    // one lane, 16 lanes, one wave (and the hub chunk size): the defaults and
+    // their neighbours by factors of 2 to 4
+    const int32_t smalls[] = {8, 16, 32}, mids[] = {128, 256, 512}, larges[] = {1024, 4096, 16384};
+    auto time_of = [&](int32_t a, int32_t b, int32_t c) {
+      kc_options t = o;
+      t.sweep_group_small = a;
+      t.sweep_group_mid = b;
+      t.sweep_group_large = c;
+      kc_stats st;
+      double sum = 0;
+      for (int rep = 0; rep < 3; ++rep) {
+        t.seed = o.seed + rep;
+        const int rc = kc_find_components(g, &t, nullptr, &st);
+        if (rc != KC_OK) throw Failure{rc};
+        if (rep) sum += st.time_ms;  // the first call warms the bins and buffers
+      }
+      return sum / 2;
+    };
+    const double def_ms = time_of(16, 256, 4096);
+    double best = def_ms;
+    int32_t bs = 16, bm = 256, bl = 4096, n = 1;
+    for (int32_t a : smalls)
+      for (int32_t b : mids)
+        for (int32_t c : larges) {
+          if (a == 16 && b == 256 && c == 4096) continue;
+          const double ms = time_of(a, b, c);
+          ++n;
+          if (ms < best) {
+            best = ms;
+            bs = a;
+            bm = b;
+            bl = c;
+          }
+        }
+    g->tuned = bin_config(bs, bm, bl);
+    if (chosen) {
+      chosen->small = static_cast<int32_t>(g->tuned.tmax);
+      chosen->mid = static_cast<int32_t>(g->tuned.smax);
+      chosen->large = static_cast<int32_t>(g->tuned.wmax);
+      chosen->ms = best;
+      chosen->default_ms = def_ms;
+      chosen->candidates = n;
     }
   });
 }

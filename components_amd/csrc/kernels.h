@@ -258,6 +258,42 @@ void component_histogram(uint64_t n, const uint32_t *labels, uint32_t *count, ui
 void labels_from_keys(uint64_t n, const uint64_t *key, uint32_t *labels, uint32_t *roots, uint32_t *partial,
                       hipStream_t s);
 
+// Hub replication (kc_options.replicate).  Hub h (vertex hubs[h], H of them in
+// increasing order) has k[h] proxy slots at slot_base[h]; cnt[s] counts the
+// hub edges a slot receives, rep_of_slot[s] is the proxy's level index (n0 +
+// its rank among the slots with an edge) or kNone, nexist[h] / first_rep[h]
+// the number and first index of h's proxies, slot_of_rep / root_of_rep the
+// slot and hub of each proxy; hubidx (n0) = the hub index of a vertex or kNone.
+struct ReplicaArgs {
+  const uint64_t *off;
+  const uint32_t *col;
+  const uint32_t *gid;  // null: identity
+  int32_t seed;
+  const uint32_t *hubs, *k;
+  const uint64_t *slot_base;
+  uint32_t *hubidx, *cnt;
+  const uint32_t *rep_of_slot, *nexist, *first_rep, *slot_of_rep, *root_of_rep;
+};
+// A slice of a hub's row: edges [e0, e1) of hub h.
+struct HubEdgeChunk {
+  uint32_t h, pad;
+  uint64_t e0, e1;
+};
+// bounds[2i], bounds[2i+1] = off[rows[i]], off[rows[i] + 1].
+void row_bounds(uint32_t count, const uint32_t *rows, const uint64_t *off, uint64_t *bounds, hipStream_t s);
+// list[0 .. *count) = the vertices of degree >= limit (in no fixed order).
+void find_hubs(uint64_t n, const uint64_t *off, uint64_t limit, uint32_t *list, uint32_t *count, hipStream_t s);
+// hubidx from hubs, then cnt (pre-zeroed) from the hub edges.
+void replica_count(const ReplicaArgs &r, uint64_t n0, uint32_t H, const HubEdgeChunk *chunks, uint32_t n_chunks,
+                   hipStream_t s);
+// Row lengths of the replicated level (n0 + R rows).
+void replica_lengths(const ReplicaArgs &r, uint64_t n0, uint32_t R, uint32_t *len, hipStream_t s);
+// Columns (row order of the proxies' hub edges not fixed) and ids of the
+// replicated level; cursor (one per slot) pre-zeroed.
+void replica_fill(const ReplicaArgs &r, uint64_t n0, uint32_t H, uint32_t R, const HubEdgeChunk *chunks,
+                  uint32_t n_chunks, const uint64_t *off_r, uint32_t *cursor, uint32_t *col_r, uint32_t *gid_r,
+                  hipStream_t s);
+
 // Device generators (gen_kernels.hip): the edge streams of the host generators.
 // 1-based (u, v) u64 pairs of a binary graph file -> 0-based u32 pairs
 // (*bad = 1 when a pair lies outside [1, n]).

@@ -2401,4 +2401,154 @@ void edge_text(const uint8_t *b, uint64_t bytes, const uint64_t *nlpos, uint64_t
   KC_LAUNCH_CHECK();
 }
 
+
+// ---------------------------------------------------------------------------
+// Hub replication (kc_options.replicate): a level-0 graph in which every hub
+// is split into proxies that take over its edges.  Ids through gid (null:
+// identity) feed the per-edge proxy pick, so the split does not depend on the
+// storage order.
+
+__global__ void k_find_hubs(uint64_t n, const uint64_t *off, uint64_t limit, uint32_t *list, uint32_t *count) {
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v < n && off[v + 1] - off[v] >= limit) list[atomicAdd(count, 1u)] = static_cast<uint32_t>(v);
+}
+
+__global__ void k_hub_index(uint32_t H, const uint32_t *hubs, uint32_t *hubidx) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < H) hubidx[hubs[h]] = h;
+}
+
+__device__ __forceinline__ uint32_t gid_of(const uint32_t *gid, uint32_t v) { return gid ? gid[v] : v; }
+
+__global__ __launch_bounds__(kBlock) void k_replica_count(ReplicaArgs r, const HubEdgeChunk *chunks) {
+  const HubEdgeChunk c = chunks[blockIdx.x];
+  const uint32_t v = r.hubs[c.h], gv = gid_of(r.gid, v), k = r.k[c.h];
+  const uint64_t base = r.slot_base[c.h];
+  for (uint64_t e = c.e0 + threadIdx.x; e < c.e1; e += kBlock)
+    atomicAdd(&r.cnt[base + replica_pick(gv, gid_of(r.gid, r.col[e]), r.seed, k)], 1u);
+}
+
+__global__ void k_replica_lengths(ReplicaArgs r, uint64_t n0, uint32_t R, uint32_t *len) {
+  const uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (x < n0) {
+    const uint32_t h = r.hubidx[x];
+    len[x] = h == kNone ? static_cast<uint32_t>(r.off[x + 1] - r.off[x]) : r.nexist[h];
+  } else if (x < n0 + R) {
+    len[x] = 1 + r.cnt[r.slot_of_rep[x - n0]];
+  }
+}
+
+// The proxy of hub w that holds w's edge to u.
+__device__ __forceinline__ uint32_t proxy_of(const ReplicaArgs &r, uint32_t w, uint32_t hw, uint32_t gu) {
+  return r.rep_of_slot[r.slot_base[hw] + replica_pick(gid_of(r.gid, w), gu, r.seed, r.k[hw])];
+}
+
+// Rows of the other vertices, one wave per row: the neighbours in place, a hub
+// replaced by its proxy for this edge.
+__global__ __launch_bounds__(kBlock) void k_replica_rows(ReplicaArgs r, uint64_t n0, const uint64_t *off_r,
+                                                          uint32_t *col_r) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock / kWave) + threadIdx.x / kWave; v < n0; v += waves) {
+    if (r.hubidx[v] != kNone) continue;
+    const uint64_t e0 = r.off[v], e1 = r.off[v + 1], o = off_r[v];
+    const uint32_t gv = gid_of(r.gid, static_cast<uint32_t>(v));
+    for (uint64_t e = e0 + lane_id(); e < e1; e += kWave) {
+      const uint32_t w = r.col[e], hw = r.hubidx[w];
+      col_r[o + (e - e0)] = hw == kNone ? w : proxy_of(r, w, hw, gv);
+    }
+  }
+}
+
+// Hub rows list their proxies; a proxy row starts with its hub; ids of the
+// level (a proxy takes its hub's).
+__global__ void k_replica_fixed(ReplicaArgs r, uint64_t n0, uint32_t H, uint32_t R, const uint64_t *off_r,
+                                uint32_t *col_r, uint32_t *gid_r) {
+  const uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (x < n0) gid_r[x] = gid_of(r.gid, static_cast<uint32_t>(x));
+  if (x < R) {
+    const uint32_t root = r.root_of_rep[x];
+    col_r[off_r[n0 + x]] = root;
+    gid_r[n0 + x] = gid_of(r.gid, root);
+  }
+  if (x < H) {
+    const uint64_t o = off_r[r.hubs[x]];
+    for (uint32_t j = 0; j < r.nexist[x]; ++j) col_r[o + j] = r.first_rep[x] + j;
+  }
+}
+
+// The hub edges, each appended to the row of its proxy (after the hub entry).
+__global__ __launch_bounds__(kBlock) void k_replica_edges(ReplicaArgs r, const HubEdgeChunk *chunks,
+                                                          const uint64_t *off_r, uint32_t *cursor, uint32_t *col_r) {
+  const HubEdgeChunk c = chunks[blockIdx.x];
+  const uint32_t v = r.hubs[c.h], gv = gid_of(r.gid, v), k = r.k[c.h];
+  const uint64_t base = r.slot_base[c.h];
+  for (uint64_t e = c.e0 + threadIdx.x; e < c.e1; e += kBlock) {
+    const uint32_t w = r.col[e], hw = r.hubidx[w];
+    const uint64_t slot = base + replica_pick(gv, gid_of(r.gid, w), r.seed, k);
+    const uint32_t p = r.rep_of_slot[slot];
+    col_r[off_r[p] + 1 + atomicAdd(&cursor[slot], 1u)] = hw == kNone ? w : proxy_of(r, w, hw, gv);
+  }
+}
+
+__global__ void k_row_bounds(uint32_t count, const uint32_t *rows, const uint64_t *off, uint64_t *bounds) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    bounds[2ull * i] = off[rows[i]];
+    bounds[2ull * i + 1] = off[rows[i] + 1];
+  }
+}
+
+void row_bounds(uint32_t count, const uint32_t *rows, const uint64_t *off, uint64_t *bounds, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_row_bounds, dim3(static_cast<unsigned>(ceil_div(count, kBlock))), dim3(kBlock), 0, s, count,
+                     rows, off, bounds);
+  KC_LAUNCH_CHECK();
+}
+
+void find_hubs(uint64_t n, const uint64_t *off, uint64_t limit, uint32_t *list, uint32_t *count, hipStream_t s) {
+  fill_u32(count, 1, 0, s);
+  if (!n) return;
+  hipLaunchKernelGGL(k_find_hubs, dim3(grid_for(n)), dim3(kBlock), 0, s, n, off, limit, list, count);
+  KC_LAUNCH_CHECK();
+}
+
+void replica_count(const ReplicaArgs &r, uint64_t n0, uint32_t H, const HubEdgeChunk *chunks, uint32_t n_chunks,
+                   hipStream_t s) {
+  fill_u32(r.hubidx, n0, kNone, s);
+  if (H) {
+    hipLaunchKernelGGL(k_hub_index, dim3(static_cast<unsigned>(ceil_div(H, kBlock))), dim3(kBlock), 0, s, H, r.hubs,
+                       r.hubidx);
+    KC_LAUNCH_CHECK();
+  }
+  if (n_chunks) {
+    hipLaunchKernelGGL(k_replica_count, dim3(n_chunks), dim3(kBlock), 0, s, r, chunks);
+    KC_LAUNCH_CHECK();
+  }
+}
+
+void replica_lengths(const ReplicaArgs &r, uint64_t n0, uint32_t R, uint32_t *len, hipStream_t s) {
+  if (!(n0 + R)) return;
+  hipLaunchKernelGGL(k_replica_lengths, dim3(grid_for(n0 + R)), dim3(kBlock), 0, s, r, n0, R, len);
+  KC_LAUNCH_CHECK();
+}
+
+void replica_fill(const ReplicaArgs &r, uint64_t n0, uint32_t H, uint32_t R, const HubEdgeChunk *chunks,
+                  uint32_t n_chunks, const uint64_t *off_r, uint32_t *cursor, uint32_t *col_r, uint32_t *gid_r,
+                  hipStream_t s) {
+  if (n0) {
+    const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n0, kBlock / kWave), 1u << 16));
+    hipLaunchKernelGGL(k_replica_rows, dim3(g), dim3(kBlock), 0, s, r, n0, off_r, col_r);
+    KC_LAUNCH_CHECK();
+  }
+  const uint64_t m = std::max<uint64_t>({n0, static_cast<uint64_t>(H), static_cast<uint64_t>(R)});
+  if (m) {
+    hipLaunchKernelGGL(k_replica_fixed, dim3(grid_for(m)), dim3(kBlock), 0, s, r, n0, H, R, off_r, col_r, gid_r);
+    KC_LAUNCH_CHECK();
+  }
+  if (n_chunks) {
+    hipLaunchKernelGGL(k_replica_edges, dim3(n_chunks), dim3(kBlock), 0, s, r, chunks, off_r, cursor, col_r);
+    KC_LAUNCH_CHECK();
+  }
+}
+
 }  // namespace kc

@@ -199,8 +199,6 @@ inline std::vector<uint32_t> HostCanonicalLabels(const kc_csr &g) {
 inline int RunApp(int argc, char **argv, int variant, const char *runner, bool whole_graph_base_case = false) {
   Config c = ParseParameters(argc, argv);
   PrintParameters(c);
-  if (c.replicate_high_degree)
-    std::printf("[INFO] -replicate is served by degree-binned sweeps (hub rows are split across waves)\n");
   const int initial_seed = c.seed;
   Statistics t_stats, dev_stats, gteps_stats, level_stats, sweep_stats, radius_stats;
   uint64_t last_alg_bytes = 0;
@@ -263,6 +261,12 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner, bool w
     o.seq_limit = c.fixed_limit ? c.sequential_limit : (rel > 4096 ? rel : 4096);
     if (whole_graph_base_case) o.seq_limit = ~0ull;
     o.deviate_stream = c.hashed ? KC_STREAM_HASHED : KC_STREAM_REFERENCE;
+    // -replicate / -hd-thres (exponential_contraction.h:235-237, parse_parameters.h:67-71): the
+    // level-0 hubs split into proxies; the reference does it on the default branch only
+    if (variant == KC_VARIANT_EXP && c.replicate_high_degree && !c.single_level_contraction && !c.use_contraction) {
+      o.replicate = 1;
+      o.hd_threshold = static_cast<double>(c.degree_threshold);
+    }
     // -single takes precedence over -contraction (exponential_contraction.h:76,134): the
     // components inside -pes blocks of consecutive ids are contracted first, once or twice
     if (variant == KC_VARIANT_EXP && !whole_graph_base_case &&
@@ -289,6 +293,8 @@ inline int RunApp(int argc, char **argv, int variant, const char *runner, bool w
     if (c.print_verbose)
       std::printf("ITERATION i=%u seed=%d time=%g device_ms=%g levels=%u sweeps=%llu components=%u\n", i, c.seed, secs,
                   st.time_ms, st.levels, static_cast<unsigned long long>(st.sweeps), st.components);
+    if (o.replicate && i == 0)
+      std::printf("REPLICATE hubs=%u proxies=%llu\n", st.hubs, static_cast<unsigned long long>(st.replicas));
     if (variant == KC_VARIANT_LABELS || variant == KC_VARIANT_LOCAL) {
       // OutputComponents (lib/data_structures/static_graph.h:504-591), called by the
       // labels and local_contraction programs after every run; its vertex loop

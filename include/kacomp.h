@@ -145,6 +145,15 @@ typedef struct kc_options {
                              consecutive ids: KC_PRE_NONE (default: every vertex its own PE),
                              KC_PRE_LOCAL, KC_PRE_SINGLE or KC_PRE_DOUBLE (below) */
   uint32_t pre_blocks;    /* PEs (blocks) of the pre-contraction, >= 1 */
+  int32_t replicate;      /* -replicate (exponential contraction): every level-0 hub, a vertex of
+                             degree >= hd_threshold * sqrt(vertices of degree > 0), is split into
+                             floor(sqrt(degree)) proxies that take over its edges (each edge to the
+                             proxy a seeded hash of the edge picks; an edge between two hubs joins
+                             their proxies) and are joined to it; proxies draw their hub's deviate,
+                             rank after every vertex in ties, and are dropped from the labels
+                             (DistributeHighDegreeVertices, exponential_contraction.h:935-1253,
+                             with every vertex its own PE; deviates :483-503) */
+  double hd_threshold;    /* -hd-thres (default 16) */
 } kc_options;
 
 /* Pre-contraction (kc_options.pre_contract), each a level of the contraction
@@ -210,6 +219,9 @@ typedef struct kc_stats {
   double kernel_ms[KC_KERNEL_CLASSES];        /* valid with time_kernels */
   uint64_t kernel_launches[KC_KERNEL_CLASSES];
   uint64_t kernel_bytes[KC_KERNEL_CLASSES];   /* algorithmic bytes per class */
+  uint64_t replicas;    /* proxies added to level 0 by kc_options.replicate */
+  uint32_t bin_small, bin_mid, bin_large; /* degree-bin thresholds the call used */
+  uint32_t hubs;        /* hubs split by kc_options.replicate */
 } kc_stats;
 
 typedef struct kc_graph kc_graph;
@@ -256,6 +268,24 @@ int kc_graph_download(const kc_graph *g, kc_csr *out);
 /* Runs the selected variant.  labels_out (host, n entries) may be NULL: the
  * labels then stay on the device (kc_copy_labels fetches them). */
 int kc_find_components(kc_graph *g, const kc_options *opt, uint32_t *labels_out, kc_stats *stats);
+
+/* Degree-bin thresholds of the plain path: rows of at most `small` entries
+ * take one lane, at most `mid` 16 lanes, at most `large` a wave; longer rows
+ * are cut into chunks of `large` slots, one wave each. */
+typedef struct kc_bin_tuning {
+  int32_t small, mid, large;
+  double ms;          /* mean device time of the chosen thresholds' calls */
+  double default_ms;  /* the same with the built-in defaults (16, 256, 4096) */
+  int32_t candidates; /* threshold triples timed */
+} kc_bin_tuning;
+/* Autotuner of the degree bins (no reference counterpart; the reference has no
+ * bins): times every candidate triple with the options opt (their thresholds
+ * ignored) on g and keeps the fastest in g, where later kc_find_components
+ * calls whose sweep_group_* fields are 0 use it.  Results do not depend on the
+ * thresholds; only the time does.  chosen may be NULL. */
+int kc_graph_autotune(kc_graph *g, const kc_options *opt, kc_bin_tuning *chosen);
+/* Sets (or, with NULL, clears) the thresholds kept in g. */
+int kc_graph_set_bins(kc_graph *g, const kc_bin_tuning *t);
 int kc_copy_labels(kc_graph *g, uint32_t *labels_out);
 /* Component-size histogram of the last kc_find_components call, computed on
  * the device (StaticGraph::OutputComponents, lib/data_structures/static_graph.h:504-591):

@@ -32,7 +32,10 @@ def lib():
             "or_std_fma_compiled": (i32, []),
             "or_lp_fixpoint": (u64, [u64, _P(u64), _P(u32), _P(u64), u64, _P(u64)]),
             "or_lp_closed_form": (None, [u64, _P(u64), _P(u32), _P(u64), u64, _P(u64)]),
-            "or_run": (ctypes.c_void_p, [u64, _P(u64), _P(u32), i32, i32, dbl, i32, i32, u64, i32, i32, u64]),
+            "or_run": (ctypes.c_void_p, [u64, _P(u64), _P(u32), i32, i32, dbl, i32, i32, u64, i32, i32, u64, i32,
+                                         dbl]),
+            "or_replication": (None, [ctypes.c_void_p, _P(u64)]),
+            "or_replicate": (u64, [u64, _P(u64), _P(u32), i32, dbl, _P(u64), _P(u32), _P(u32)]),
             "or_hashed_deviate": (u32, [u32, i32, u32, dbl]),
             "or_free": (None, [ctypes.c_void_p]),
             "or_labels": (None, [ctypes.c_void_p, _P(u32)]),
@@ -113,6 +116,24 @@ def splitmix64_mix(z):
     return int(lib().or_splitmix64_mix(z))
 
 
+def replicate(offsets, targets, seed, hd_threshold=16.0):
+    """Level 0 of the graph with its hubs split into proxies: (offsets, targets, ids), a proxy's id
+    being its hub's."""
+    off, tgt = _arrs(offsets, targets)
+    n = len(off) - 1
+    nr = int(lib().or_replicate(n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
+                                seed, float(hd_threshold), None, None, None))
+    n0 = int(np.count_nonzero(np.diff(off.astype(np.int64))))
+    m2 = int(off[-1]) + 2 * (nr - n0)
+    o = np.empty(nr + 1, np.uint64)
+    t = np.empty(max(m2, 1), np.uint32)
+    gid = np.empty(max(nr, 1), np.uint32)
+    lib().or_replicate(n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)), seed,
+                       float(hd_threshold), o.ctypes.data_as(_P(ctypes.c_uint64)), t.ctypes.data_as(_P(ctypes.c_uint32)),
+                       gid.ctypes.data_as(_P(ctypes.c_uint32)))
+    return o, t[:m2], gid[:nr]
+
+
 class Run:
     """Result of or_run: labels and per-level trace."""
 
@@ -120,14 +141,14 @@ class Run:
     PRE_MODES = {"none": 0, "local": 1, "single": 2, "double": 3}
 
     def __init__(self, offsets, targets, variant="exp", seed=1, beta=0.1, fma=True, seq_limit=4096, keep_keys=False,
-                 stream=0, pre_contract="none", pre_blocks=1):
+                 stream=0, pre_contract="none", pre_blocks=1, replicate=False, hd_threshold=16.0):
         off, tgt = _arrs(offsets, targets)
         self.n = len(off) - 1
         v = self.VARIANTS[variant] if isinstance(variant, str) else int(variant)
         self.h = lib().or_run(self.n, off.ctypes.data_as(_P(ctypes.c_uint64)), tgt.ctypes.data_as(_P(ctypes.c_uint32)),
                               v, seed, beta, 1 if fma else 0, stream, seq_limit, 1 if keep_keys else 0,
                               self.PRE_MODES[pre_contract] if isinstance(pre_contract, str) else int(pre_contract),
-                              pre_blocks)
+                              pre_blocks, 1 if replicate else 0, float(hd_threshold))
 
     def labels(self):
         out = np.empty(max(self.n, 1), np.uint32)
@@ -152,6 +173,12 @@ class Run:
 
     def sweeps(self):
         return int(lib().or_total_sweeps(self.h))
+
+    def replication(self):
+        """(hubs split, proxies added) by replicate."""
+        out = (ctypes.c_uint64 * 2)()
+        lib().or_replication(self.h, out)
+        return int(out[0]), int(out[1])
 
     def seconds(self):
         return float(lib().or_seconds(self.h))

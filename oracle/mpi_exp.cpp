@@ -99,35 +99,41 @@ u32 exp_deviate(u32 gid, int32_t seed, u32 iteration, u64 rng_offset, double bet
   return static_cast<u32>(static_cast<u64>(-std::log(1.0 - u) / beta));
 }
 
-// All-to-all of per-peer vectors of a trivially copyable type.
+// All-to-all of per-peer vectors of a trivially copyable type.  Counts and
+// displacements are in elements of a contiguous datatype of sizeof(T) bytes
+// (MPI's int counts would cap a rank's bytes at 2 GB; the rank that owns the
+// hubs of a skewed graph receives more than that).
 template <class T>
 std::vector<T> all_to_all(const std::vector<std::vector<T>> &out, std::vector<u64> *from = nullptr) {
   const int P = g_size;
+  MPI_Datatype elem;
+  MPI_Type_contiguous(static_cast<int>(sizeof(T)), MPI_BYTE, &elem);
+  MPI_Type_commit(&elem);
   std::vector<int> sc(P), rc(P), sd(P), rd(P);
   u64 stot = 0;
   for (int q = 0; q < P; ++q) {
-    const u64 b = out[q].size() * sizeof(T);
-    if (b > static_cast<u64>(INT_MAX)) die("exchange of more than 2 GB with one peer");
-    sc[q] = static_cast<int>(b);
-    stot += b;
+    if (stot + out[q].size() > static_cast<u64>(INT_MAX)) die("exchange of more than 2^31 elements per rank");
+    sc[q] = static_cast<int>(out[q].size());
+    sd[q] = static_cast<int>(stot);
+    stot += out[q].size();
   }
   MPI_Alltoall(sc.data(), 1, MPI_INT, rc.data(), 1, MPI_INT, MPI_COMM_WORLD);
   u64 rtot = 0;
   for (int q = 0; q < P; ++q) {
-    if (stot > static_cast<u64>(INT_MAX) || rtot > static_cast<u64>(INT_MAX)) die("exchange above 2 GB per rank");
-    sd[q] = static_cast<int>(q ? sd[q - 1] + sc[q - 1] : 0);
+    if (rtot + static_cast<u64>(rc[q]) > static_cast<u64>(INT_MAX)) die("exchange of more than 2^31 elements per rank");
     rd[q] = static_cast<int>(rtot);
     rtot += static_cast<u64>(rc[q]);
   }
-  std::vector<char> sbuf(std::max<u64>(stot, 1));
+  std::vector<T> sbuf(std::max<u64>(stot, 1));
   for (int q = 0; q < P; ++q)
     if (!out[q].empty()) std::memcpy(sbuf.data() + sd[q], out[q].data(), out[q].size() * sizeof(T));
-  std::vector<T> in(rtot / sizeof(T));
-  MPI_Alltoallv(sbuf.data(), sc.data(), sd.data(), MPI_BYTE, in.data(), rc.data(), rd.data(), MPI_BYTE,
-                MPI_COMM_WORLD);
+  std::vector<T> in(std::max<u64>(rtot, 1));
+  MPI_Alltoallv(sbuf.data(), sc.data(), sd.data(), elem, in.data(), rc.data(), rd.data(), elem, MPI_COMM_WORLD);
+  MPI_Type_free(&elem);
+  in.resize(rtot);
   if (from) {
     from->assign(P + 1, 0);
-    for (int q = 0; q < P; ++q) (*from)[q + 1] = (*from)[q] + static_cast<u64>(rc[q]) / sizeof(T);
+    for (int q = 0; q < P; ++q) (*from)[q + 1] = (*from)[q] + static_cast<u64>(rc[q]);
   }
   return in;
 }

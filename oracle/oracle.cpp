@@ -177,6 +177,7 @@ struct or_result {
   std::vector<LevelTrace> levels;
   uint64_t sweeps = 0;
   double seconds = 0.0;
+  uint64_t hubs = 0, replicas = 0;
 };
 
 namespace {
@@ -243,8 +244,92 @@ std::vector<uint32_t> block_components(const Csr &g, uint64_t blocks, bool only_
   return lab;
 }
 
+// Proxy slot in [0, k) of the edge (v, w) of a hub v: SplitMix64 over the two
+// ids and the seed, multiply-shift reduced (the library's replica_pick).
+uint32_t proxy_slot(uint32_t gv, uint32_t gw, int32_t seed, uint32_t k) {
+  const uint64_t z = mix64((static_cast<uint64_t>(gv) << 32 | gw) ^
+                           (0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(static_cast<uint32_t>(seed)) + 1)));
+  return static_cast<uint32_t>(((z >> 32) * k) >> 32);
+}
+
+// DistributeHighDegreeVertices (exponential_contraction.h:935-1253) with every
+// vertex its own PE, so that every edge of a hub is remote: a hub (degree >=
+// threshold * sqrt(n), utils.h:108-117) of degree d has floor(sqrt(d)) proxy
+// slots (:1040, min(P - 1, sqrt(d)) with P = n); each of its edges moves to the
+// slot proxy_slot names, an edge between two hubs joins the two proxies, and
+// the slots that receive an edge become proxies n, n + 1, ... (hub order, then
+// slot order), each joined to its hub, which keeps only its proxies.  A proxy
+// takes its hub's id (its deviate is the hub's, :483-503).
+void replicate_level0(Csr &cur, std::vector<uint32_t> &gid, int32_t seed, double threshold, or_result &res) {
+  const uint64_t n = cur.n;
+  if (!n) return;
+  const uint64_t limit = std::max<uint64_t>(1, static_cast<uint64_t>(threshold * std::sqrt(static_cast<double>(n))));
+  std::vector<uint32_t> hub_of(n, kNone), hubs, k;
+  std::vector<uint64_t> base(1, 0);
+  for (uint64_t v = 0; v < n; ++v)
+    if (cur.off[v + 1] - cur.off[v] >= limit) {
+      hub_of[v] = static_cast<uint32_t>(hubs.size());
+      hubs.push_back(static_cast<uint32_t>(v));
+      const uint64_t d = cur.off[v + 1] - cur.off[v];
+      uint32_t kk = static_cast<uint32_t>(std::min<uint64_t>(n - 1, static_cast<uint64_t>(std::sqrt(static_cast<double>(d)))));
+      k.push_back(kk ? kk : 1);
+      base.push_back(base.back() + k.back());
+    }
+  if (hubs.empty()) return;
+  std::vector<uint32_t> cnt(base.back(), 0);
+  for (size_t h = 0; h < hubs.size(); ++h) {
+    const uint32_t v = hubs[h];
+    for (uint64_t e = cur.off[v]; e < cur.off[v + 1]; ++e) cnt[base[h] + proxy_slot(gid[v], gid[cur.tgt[e]], seed, k[h])]++;
+  }
+  std::vector<uint32_t> proxy(base.back(), kNone), first(hubs.size()), count(hubs.size(), 0), root;
+  for (size_t h = 0; h < hubs.size(); ++h) {
+    first[h] = static_cast<uint32_t>(n + root.size());
+    for (uint64_t sl = base[h]; sl < base[h + 1]; ++sl)
+      if (cnt[sl]) {
+        proxy[sl] = static_cast<uint32_t>(n + root.size());
+        root.push_back(hubs[h]);
+        ++count[h];
+      }
+  }
+  auto far_end = [&](uint32_t u, uint32_t w) {  // w as seen from u's edge
+    const uint32_t hw = hub_of[w];
+    return hw == kNone ? w : proxy[base[hw] + proxy_slot(gid[w], gid[u], seed, k[hw])];
+  };
+  const uint64_t R = root.size();
+  std::vector<std::vector<uint32_t>> prow(R);
+  for (uint64_t p = 0; p < R; ++p) prow[p].push_back(root[p]);
+  for (size_t h = 0; h < hubs.size(); ++h) {
+    const uint32_t v = hubs[h];
+    for (uint64_t e = cur.off[v]; e < cur.off[v + 1]; ++e) {
+      const uint32_t w = cur.tgt[e];
+      prow[proxy[base[h] + proxy_slot(gid[v], gid[w], seed, k[h])] - n].push_back(far_end(v, w));
+    }
+  }
+  Csr out;
+  out.n = n + R;
+  out.off.assign(out.n + 1, 0);
+  for (uint64_t v = 0; v < n; ++v) {
+    const uint32_t h = hub_of[v];
+    if (h != kNone) {
+      for (uint32_t j = 0; j < count[h]; ++j) out.tgt.push_back(first[h] + j);
+    } else {
+      for (uint64_t e = cur.off[v]; e < cur.off[v + 1]; ++e) out.tgt.push_back(far_end(static_cast<uint32_t>(v), cur.tgt[e]));
+    }
+    out.off[v + 1] = out.tgt.size();
+  }
+  for (uint64_t p = 0; p < R; ++p) {
+    out.tgt.insert(out.tgt.end(), prow[p].begin(), prow[p].end());
+    out.off[n + p + 1] = out.tgt.size();
+    gid.push_back(gid[root[p]]);
+  }
+  cur = std::move(out);
+  res.hubs = hubs.size();
+  res.replicas = R;
+}
+
 void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool fma, int stream,
-                     uint64_t seq_limit, bool keep_keys, int pre_mode, uint64_t pre_blocks, or_result &res) {
+                     uint64_t seq_limit, bool keep_keys, int pre_mode, uint64_t pre_blocks, bool replicate,
+                     double hd_threshold, or_result &res) {
   // Level 0 holds the vertices of degree > 0 in id order: the reference sets
   // degree-0 vertices inactive at load (lib/io/graph_io.h:380) and its vertex
   // loops skip inactive vertices (lib/data_structures/dynamic_graph.h:113-117).
@@ -264,6 +349,7 @@ void run_contraction(const Csr &g0, int variant, int32_t seed, double beta, bool
     for (uint64_t e = g0.off[gid[a]]; e < g0.off[gid[a] + 1]; ++e) cur.tgt.push_back(newid[g0.tgt[e]]);
     cur.off[a + 1] = cur.tgt.size();
   }
+  if (replicate && variant == 0) replicate_level0(cur, gid, seed, hd_threshold, res);
   uint32_t iteration = 1;
   std::vector<uint32_t> top_rep;
   bool have_top = false;
@@ -418,7 +504,7 @@ void or_lp_closed_form(uint64_t n, const uint64_t *off, const uint32_t *tgt, con
 
 or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t variant, int32_t seed, double beta,
                   int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys, int32_t pre_mode,
-                  uint64_t pre_blocks) {
+                  uint64_t pre_blocks, int32_t replicate, double hd_threshold) {
   Csr g;
   g.n = n;
   g.off.assign(off, off + n + 1);
@@ -427,7 +513,7 @@ or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t 
   const auto t0 = std::chrono::steady_clock::now();
   if (variant == 0 || variant == 1)
     run_contraction(g, variant, seed, beta, fma != 0, stream, seq_limit, keep_keys != 0, pre_mode,
-                    pre_blocks ? pre_blocks : 1, *r);
+                    pre_blocks ? pre_blocks : 1, replicate != 0, hd_threshold, *r);
   else
     run_propagation(g, variant == 2, *r);
   r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -450,6 +536,36 @@ void or_level_keys(const or_result *r, uint32_t level, uint64_t *keys, uint32_t 
   if (gid) std::copy(t.gid.begin(), t.gid.end(), gid);
 }
 uint64_t or_total_sweeps(const or_result *r) { return r->sweeps; }
+uint64_t or_replicate(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t seed, double hd_threshold,
+                      uint64_t *off_out, uint32_t *tgt_out, uint32_t *gid_out) {
+  // level 0 (the vertices of degree > 0 in id order), then its hubs split
+  std::vector<uint32_t> newid(n, kNone), gid;
+  for (uint64_t v = 0; v < n; ++v)
+    if (off[v + 1] > off[v]) {
+      newid[v] = static_cast<uint32_t>(gid.size());
+      gid.push_back(static_cast<uint32_t>(v));
+    }
+  Csr cur;
+  cur.n = gid.size();
+  cur.off.assign(cur.n + 1, 0);
+  for (uint64_t a = 0; a < cur.n; ++a) {
+    for (uint64_t e = off[gid[a]]; e < off[gid[a] + 1]; ++e) cur.tgt.push_back(newid[tgt[e]]);
+    cur.off[a + 1] = cur.tgt.size();
+  }
+  or_result res;
+  replicate_level0(cur, gid, seed, hd_threshold, res);
+  if (off_out) {
+    std::copy(cur.off.begin(), cur.off.end(), off_out);
+    std::copy(cur.tgt.begin(), cur.tgt.end(), tgt_out);
+    std::copy(gid.begin(), gid.end(), gid_out);
+  }
+  return cur.n;
+}
+
+void or_replication(const or_result *r, uint64_t out[2]) {
+  out[0] = r->hubs;
+  out[1] = r->replicas;
+}
 double or_seconds(const or_result *r) { return r->seconds; }
 
 }  // extern "C"

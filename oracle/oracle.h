@@ -33,6 +33,10 @@
  *                         pinned through the FP64 form (rounding a network whose
  *                         weights are already bf16 values changes nothing else)
  *   or_level_features     hand-computed features of small graphs
+ *   or_replicate          the split graph is symmetric and simple, every proxy
+ *                         hangs off its hub, hubs keep only their proxies, and
+ *                         merging every proxy into its hub gives back the input's
+ *                         edges exactly (tests/test_oracle_streams.py)
  */
 #ifndef KC_ORACLE_H_
 #define KC_ORACLE_H_
@@ -70,9 +74,19 @@ typedef struct or_result or_result;
  * pre_mode: 0 none, 1 local, 2 single, 3 double (the library's KC_PRE_*): the
  * components inside pre_blocks blocks of consecutive level indices contracted
  * first (levels with distance-0 keys, sweeps 0, radius 0). */
+/* replicate: level 0 with its hubs (degree >= hd_threshold * sqrt(n0)) split
+ * into proxies (the library's kc_options.replicate; exp only). */
 or_result *or_run(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t variant, int32_t seed, double beta,
                   int32_t fma, int32_t stream, uint64_t seq_limit, int32_t keep_keys, int32_t pre_mode,
-                  uint64_t pre_blocks);
+                  uint64_t pre_blocks, int32_t replicate, double hd_threshold);
+/* hubs split and proxies added by replicate */
+void or_replication(const or_result *r, uint64_t out[2]);
+/* Level 0 of the input with its hubs split: returns its vertex count n0 + R;
+ * with off_out non-null also writes its CSR (n0 + R + 1 offsets; the caller
+ * sizes tgt_out by off[n] + 2 * R) and the id of every vertex (a proxy has its
+ * hub's). */
+uint64_t or_replicate(uint64_t n, const uint64_t *off, const uint32_t *tgt, int32_t seed, double hd_threshold,
+                      uint64_t *off_out, uint32_t *tgt_out, uint32_t *gid_out);
 void or_free(or_result *r);
 void or_labels(const or_result *r, uint32_t *labels);
 uint32_t or_levels(const or_result *r);

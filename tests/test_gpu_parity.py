@@ -136,6 +136,98 @@ def test_level_trace_matches_oracle(case, stream, bfs):
     assert st["radius_sum"] == sum(w[3] for w in want_levels)
 
 
+def replicate_graphs():
+    star = ca.Graph.from_edges(3001, [(0, i) for i in range(1, 3001)])
+    hubs = ca.Graph.from_edges(2500, [(0, i) for i in range(1, 1200)] + [(1300, i) for i in range(1301, 2500)]
+                               + [(0, 1300), (5, 1305), (7, 8)])
+    return [("star", star, 16.0), ("hubs", hubs, 4.0), ("rmat", ca.generate("rmat", n=1 << 13, m=1 << 16, seed=3), 2.0),
+            ("rhg", ca.generate("rhg", n=1 << 13, gamma=2.5, d=12.0, seed=5), 1.0),
+            ("gnm", ca.generate("gnm_undirected", n=1 << 12, m=1 << 13, seed=2), 16.0)]
+
+
+@pytest.mark.parametrize("bfs", [False, True])
+@pytest.mark.parametrize("stream", [ca.STREAM_REFERENCE, ca.STREAM_HASHED])
+def test_replicate_trace_matches_oracle(stream, bfs):
+    """-replicate: level 0 with its hubs split into proxies that draw their hub's deviate; every
+    level's sizes, radius, ids and keys equal the oracle's, and so do the hub and proxy counts."""
+    for name, g, thr in replicate_graphs():
+        print(f"replicate {name}: n={g.n} m={g.m} threshold={thr}", flush=True)
+        for seq in (0, 64):
+            run = oracle.Run(g.offsets, g.targets, variant="exp", seed=1007, seq_limit=seq, keep_keys=True,
+                             stream=stream, replicate=True, hd_threshold=thr)
+            dg = ca.DeviceGraph(g)
+            try:
+                lab, st = dg.find_components(variant="exp", seed=1007, seq_limit=seq, record_trace=True, stream=stream,
+                                             use_bfs=bfs, replicate=True, hd_threshold=thr)
+                tr = dg.trace()
+            finally:
+                dg.close()
+            assert (st["hubs"], st["replicas"]) == run.replication(), name
+            want = run.levels()
+            assert len(tr) == len(want), name
+            for lvl, ((info, keys, gid), winfo) in enumerate(zip(tr, want)):
+                assert info[0] == winfo[0] and info[1] == winfo[1], (name, lvl)
+                assert info[3] == winfo[3], (name, lvl)
+                wkeys, wgid = run.level_keys(lvl)
+                assert np.array_equal(gid, wgid), (name, lvl)
+                assert np.array_equal(keys, wkeys), (name, lvl)
+            assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets)), name
+            assert st["components"] == len(np.unique(lab))
+
+
+def test_replicate_option_checks():
+    g = ca.generate("rmat", n=1 << 10, m=1 << 13, seed=3)
+    dg = ca.DeviceGraph(g)
+    try:
+        for bad in (dict(variant="local", replicate=True), dict(replicate=True, pre_contract="single", pre_blocks=4),
+                    dict(replicate=True, hd_threshold=-1.0)):
+            with pytest.raises(ca.KcError):
+                dg.find_components(**bad)
+        # a threshold no vertex reaches: the plain path, nothing split
+        lab, st = dg.find_components(replicate=True, hd_threshold=1e6, seed=3)
+        assert st["hubs"] == 0 and st["replicas"] == 0
+        assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets))
+        # the same handle after a replicated call (its work buffers may have grown)
+        lab2, st2 = dg.find_components(replicate=True, hd_threshold=0.5, seed=4)
+        assert st2["replicas"] > 0 and np.array_equal(lab2, lab)
+        lab3, _ = dg.find_components(seed=5)
+        assert np.array_equal(lab3, lab)
+    finally:
+        dg.close()
+
+
+def test_autotuned_bins_keep_results():
+    """The autotuner only picks how rows are spread over lanes: with its choice kept in the handle
+    every level's sizes, radius, ids and keys, and the labels, equal the defaults' (and the
+    oracle's labels); the stats report the thresholds used."""
+    for g in (ca.generate("rmat", n=1 << 14, m=1 << 18, seed=3), ca.generate("rgg_2d", n=1 << 14, r=0.012, seed=2)):
+        dg = ca.DeviceGraph(g)
+        try:
+            kw = dict(seed=1009, seq_limit=64, stream=ca.STREAM_HASHED, use_bfs=True)
+            lab0, st0 = dg.find_components(record_trace=True, **kw)
+            tr0 = dg.trace()
+            assert (st0["bin_small"], st0["bin_mid"], st0["bin_large"]) == (16, 256, 4096)
+            t = dg.autotune(**kw)
+            print(f"autotune n={g.n}: {t}", flush=True)
+            assert t["candidates"] == 27 and t["ms"] <= t["default_ms"]
+            lab, st = dg.find_components(record_trace=True, **kw)
+            assert (st["bin_small"], st["bin_mid"], st["bin_large"]) == (t["small"], t["mid"], t["large"])
+            tr = dg.trace()
+            assert np.array_equal(lab, lab0) and len(tr) == len(tr0)
+            for (info, keys, gid), (info0, keys0, gid0) in zip(tr, tr0):
+                assert info[0] == info0[0] and info[1] == info0[1] and info[3] == info0[3]
+                assert np.array_equal(keys, keys0) and np.array_equal(gid, gid0)
+            # explicit thresholds still win over the kept ones; clearing restores the defaults
+            _, st = dg.find_components(tmax=4, smax=64, wmax=256, **kw)
+            assert (st["bin_small"], st["bin_mid"], st["bin_large"]) == (4, 64, 256)
+            dg.set_bins()
+            _, st = dg.find_components(**kw)
+            assert (st["bin_small"], st["bin_mid"], st["bin_large"]) == (16, 256, 4096)
+            assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets))
+        finally:
+            dg.close()
+
+
 def test_local_contraction_trace_matches_oracle():
     g = ca.generate("gnm_undirected", n=1 << 13, m=1 << 14, seed=9)
     run = oracle.Run(g.offsets, g.targets, variant="local", seed=1001, seq_limit=64, keep_keys=True)

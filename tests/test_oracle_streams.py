@@ -234,3 +234,103 @@ def test_local_keys_equal_two_hop_closed_form():
             planted_caught |= not np.array_equal(passes_in_place(off, tgt, key0), two_hop_min(off, tgt, key0))
             assert not np.array_equal(passes_in_place(off, tgt, key0, passes=1), two_hop_min(off, tgt, key0))
     assert planted_caught
+
+
+# --- hub replication (-replicate) ----------------------------------------------------------
+
+def split_ok(g, off_r, tgt_r, gid_r, threshold):
+    """What a split of the hubs into proxies must satisfy (DistributeHighDegreeVertices,
+    exponential_contraction.h:935-1253, every vertex its own PE): the graph stays symmetric and
+    simple; proxies come after the n0 vertices, each joined to its hub, which keeps only its
+    proxies; a hub of degree d has at most floor(sqrt(d)) of them; and merging every proxy into
+    its hub gives back level 0's edges exactly, each once per direction."""
+    off0, tgt0, keep = level0(g)
+    n0, nr = len(off0) - 1, len(off_r) - 1
+    R = nr - n0
+    off_r = off_r.astype(np.int64)
+    if R < 0 or off_r[-1] != off0[-1] + 2 * R:
+        return False
+    src = np.repeat(np.arange(nr), np.diff(off_r))
+    dst = tgt_r.astype(np.int64)
+    if (src == dst).any() or len(np.unique(src * nr + dst)) != len(src):  # no loops, no duplicates
+        return False
+    if not np.array_equal(np.sort(src * nr + dst), np.sort(dst * nr + src)):  # symmetric
+        return False
+    # a proxy's hub: the level-0 vertex with its id
+    pos = np.full(int(g.n), -1, np.int64)
+    pos[keep] = np.arange(n0)
+    home = np.concatenate([np.arange(n0), pos[gid_r[n0:].astype(np.int64)]])
+    if not np.array_equal(gid_r[:n0], keep) or (home < 0).any():
+        return False
+    deg0 = np.diff(off0)
+    limit = max(1, int(threshold * math.sqrt(n0)))
+    hubs = np.flatnonzero(deg0 >= limit)
+    is_hub = np.zeros(n0, bool)
+    is_hub[hubs] = True
+    proxies_of = np.bincount(home[n0:], minlength=n0)
+    if (proxies_of[~is_hub] != 0).any() or (proxies_of[hubs] > np.floor(np.sqrt(deg0[hubs]))).any():
+        return False
+    if (is_hub & (proxies_of == 0) & (deg0 > 0)).any():
+        return False
+    for p in range(n0, nr):  # every proxy's row starts at its hub and holds it once
+        row = dst[off_r[p]:off_r[p + 1]]
+        if row[0] != home[p] or (row == home[p]).sum() != 1 or len(row) < 2:
+            return False
+    for h in hubs:  # a hub's row is its proxies
+        row = dst[off_r[h]:off_r[h + 1]]
+        if not (np.all(row >= n0) and np.all(home[row] == h)):
+            return False
+    merged = np.stack([home[src], home[dst]], 1)
+    merged = merged[merged[:, 0] != merged[:, 1]]  # the hub-proxy joins vanish
+    orig = np.stack([np.repeat(np.arange(n0), deg0), tgt0], 1)
+    a = np.sort(merged[:, 0] * n0 + merged[:, 1])
+    b = np.sort(orig[:, 0] * n0 + orig[:, 1])
+    return np.array_equal(a, b)
+
+
+def test_replicated_level0_pinned():
+    star = ca.Graph.from_edges(3001, [(0, i) for i in range(1, 3001)])
+    hubs = ca.Graph.from_edges(2500, [(0, i) for i in range(1, 1200)] + [(1300, i) for i in range(1301, 2500)]
+                               + [(0, 1300), (5, 1305), (7, 8)])
+    cases = [(star, 16.0), (hubs, 4.0), (ca.generate("rmat", n=1 << 12, m=1 << 16, seed=3), 2.0),
+             (ca.generate("rhg", n=1 << 12, gamma=2.5, d=12.0, seed=5), 1.0),
+             (ca.generate("gnm_undirected", n=1 << 10, m=1 << 12, seed=2), 16.0)]  # no hub at all
+    for g, thr in cases:
+        for seed in (1001, 7):
+            o, t, gid = oracle.replicate(g.offsets, g.targets, seed, thr)
+            assert split_ok(g, o, t, gid, thr), (g.n, thr, seed)
+    # the hub pair (0, 1300) joins two proxies; the split moved something
+    o, t, gid = oracle.replicate(hubs.offsets, hubs.targets, 3, 4.0)
+    assert len(o) - 1 > len(level0(hubs)[0]) - 1
+    # planted mistakes: a hub that keeps its own edges besides its proxies; one proxy edge
+    # redirected to the hub itself (the two ends disagree about the proxy)
+    off0, tgt0, _ = level0(hubs)
+    src = np.repeat(np.arange(len(o) - 1), np.diff(o.astype(np.int64)))
+    keep_hub = np.concatenate([np.stack([src, t.astype(np.int64)], 1),
+                               np.stack([np.zeros(int(off0[1]), np.int64), tgt0[:int(off0[1])]], 1),
+                               np.stack([tgt0[:int(off0[1])], np.zeros(int(off0[1]), np.int64)], 1)])
+    bad = ca.Graph.from_edges(len(o) - 1, keep_hub)
+    assert not split_ok(hubs, bad.offsets, bad.targets, gid, 4.0)
+    t2 = t.copy()
+    e = int(np.flatnonzero(t2 >= len(off0) - 1)[-1])  # some edge into a proxy
+    t2[e] = 0
+    assert not split_ok(hubs, o, t2, gid, 4.0)
+
+
+def test_replicated_runs_are_canonical():
+    for g, thr in [(ca.generate("rmat", n=1 << 12, m=1 << 16, seed=3), 2.0),
+                   (ca.generate("rhg", n=1 << 12, gamma=2.5, d=12.0, seed=5), 1.0)]:
+        ref = oracle.canonical_labels(g.offsets, g.targets)
+        for stream in (0, 1):
+            run = oracle.Run(g.offsets, g.targets, seed=1001, stream=stream, seq_limit=0, keep_keys=True,
+                             replicate=True, hd_threshold=thr)
+            assert np.array_equal(run.labels(), ref)
+            hubs, reps = run.replication()
+            assert hubs > 0 and reps > 0
+            keys, gid = run.level_keys(0)
+            n0 = int(np.count_nonzero(np.diff(g.offsets.astype(np.int64))))
+            assert run.levels()[0][0] == n0 + reps
+            # a proxy draws its hub's deviate: its key's starting distance is the hub's
+            dev = np.array([oracle.hashed_deviate(int(x), 1001, 1) if stream else
+                            oracle.exp_deviate(int(x), 1001, 1, n0, 0.1, True) for x in gid], np.uint64)
+            assert np.all((keys >> np.uint64(32)) <= dev)
