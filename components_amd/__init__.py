@@ -136,6 +136,8 @@ _SIGNATURES = {
     "kc_graph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "kc_graph_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
     "kc_find_components": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(ctypes.c_uint32), _P(Stats)]),
+    "kc_find_components_noalloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, _P(Options),
+                                                  ctypes.c_void_p, ctypes.c_void_p, _P(Stats)]),
     "kc_copy_labels": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint32)]),
     "kc_graph_autotune": (ctypes.c_int, [ctypes.c_void_p, _P(Options), _P(BinTuning)]),
     "kc_graph_set_bins": (ctypes.c_int, [ctypes.c_void_p, _P(BinTuning)]),
@@ -506,6 +508,21 @@ def deviates(gid0, count, seed, iteration, rng_offset, beta=0.1, fma=True, strea
     out = np.empty(max(count, 1), np.uint32)
     _check(lib().kc_deviates(gid0, count, seed, iteration, rng_offset, beta, 1 if fma else 0, stream, _u32p(out)))
     return out[:count]
+
+
+NOALLOC_MAX_VERTICES = 1 << 20
+NOALLOC_MAX_EDGE_SLOTS = 1 << 24
+
+
+def find_components_noalloc(n, offsets_ptr, targets_ptr, labels_ptr, stream=None, **kw):
+    """kc_find_components_noalloc: the labels or shortcut variant on a CSR already in device
+    memory (offsets, targets and labels are device addresses the caller owns; stream a
+    hipStream_t handle or None), with no allocation by the library.  Returns the stats dict."""
+    o = options(**kw)
+    st = Stats()
+    _check(lib().kc_find_components_noalloc(n, offsets_ptr, targets_ptr, ctypes.byref(o), labels_ptr, stream,
+                                            ctypes.byref(st)))
+    return stats_dict(st)
 
 
 def find_components(g: Graph, **kw):

@@ -312,6 +312,26 @@ int kc_trace_level_keys(const kc_graph *g, uint32_t level, uint64_t *keys_out, u
 int kc_deviates(uint32_t gid0, uint32_t count, int32_t seed, uint32_t iteration, uint64_t rng_offset,
                 double beta, int32_t fma, int32_t stream, uint32_t *out);
 
+/* Allocation-free form of kc_find_components for the KC_VARIANT_LABELS and
+ * KC_VARIANT_SHORTCUT variants.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  offsets (n+1 u64),
+ * targets (offsets[n] u32, both directions of every edge) and labels (n u32,
+ * written) are DEVICE pointers the caller owns; stream is a hipStream_t (NULL:
+ * the null stream).  Every work buffer comes from a fixed device array of the
+ * library (about 30 MB per GPU, held until the process ends), so graphs are
+ * limited to KC_NOALLOC_MAX_VERTICES vertices and KC_NOALLOC_MAX_EDGE_SLOTS
+ * edge slots (larger ones: KC_ERR_INVALID).  No device or pinned host memory
+ * is allocated; the call reads a few words back to the host and returns
+ * synchronised.  Labels equal kc_find_components' for the same variant (the
+ * sweep count may differ: in-place sweeps race benignly on the keys).  Calls that overlap on one GPU share its
+ * array and race: the caller serialises them.  stats (may be NULL) gets n, m,
+ * sweeps, components and total_ms. */
+#define KC_NOALLOC_MAX_VERTICES (1ull << 20)
+#define KC_NOALLOC_MAX_EDGE_SLOTS (1ull << 24)
+int kc_find_components_noalloc(uint64_t n, const uint64_t *offsets, const uint32_t *targets, const kc_options *opt,
+                               uint32_t *labels, void *stream, kc_stats *stats);
+
 /* ------------------------------------------------------------------------
  * Batched form: many independent graphs contracted in one persistent launch.
  * No reference counterpart as a batch; per graph it computes what
