@@ -670,7 +670,9 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       // Round 0 pushes from about a tenth of the rows; a hub gets about a
       // tenth of its degree in pushes, serialised on its key.  Once that
       // exceeds what the round's other pushes take (past about 1/512 of the
-      // level's edge slots), the largest hubs pull first.
+      // level's edge slots), the largest hubs pull first.  (Pulling round 0
+      // everywhere, every row scanned against flags of the distance-0
+      // vertices, was measured slower at R-MAT 24: 2.36 against 1.35 ms.)
       const uint64_t hub_pull_deg = t + i == 0 ? std::max<uint64_t>(kHubPullMin, m2 / 512) : 0;
       push_round(off, col, key, kOneHop, b, t + i, &g->flags[f], prev, hub_pull_deg, g->stream);
       g->timer.stop(slots[i], 0);
@@ -966,10 +968,11 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         }
         sweeps = 2;
       } else if (o->use_bfs) {
+        const bool choose = o->pull_rounds > 0 || (o->pull_rounds == 0 && n_l >= kPullRoundsMin &&
+                                                   m2_l >= kDirectionMinDegree * n_l);
         sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base,
                                     o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps,
-                                    o->pull_rounds > 0 || (o->pull_rounds == 0 && n_l >= kPullRoundsMin &&
-                                                           m2_l >= kDirectionMinDegree * n_l));
+                                    choose);
       } else {
         sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
       }
@@ -1240,6 +1243,19 @@ std::unique_ptr<kc_graph> new_handle() {
   return g;
 }
 
+// Temporary device buffer freed at scope exit.
+struct DevTemp {
+  void *p = nullptr;
+  explicit DevTemp(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 64) != hipSuccess) fail(KC_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  ~DevTemp() { (void)hipFree(p); }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
 // The level-sized work set for levels of up to cap_n vertices and cap_m2 edge
 // slots (the previous set, if any, is freed first).
 void alloc_work(kc_graph *g, uint64_t cap_n, uint64_t cap_m2) {
@@ -1328,23 +1344,18 @@ void init_work(kc_graph *g) {
     g->col0p = g->dalloc<uint32_t>(m2);
     level0_order(n0, g->off0c, g->col0c, g->gid0, g->idx0, g->pos0, g->gidp0, g->off0p, g->col0p, g->sk, g->sv, g->sk2,
                  g->sv2, g->sort_scratch, g->scan_scratch, g->stream);
-    sync(g);
+    // every row ascending: a row's gathers then walk up the (degree-ordered)
+    // storage, and neighbouring entries share cache lines more often
+    {
+      DevTemp k0(m2 * 4 + 64), v0(m2 * 4 + 64), k1(m2 * 4 + 64), v1(m2 * 4 + 64);
+      DevTemp scratch(radix_sort_scratch_words(m2) * 8 + 64);
+      sort_rows(n0, g->off0p, g->col0p, m2, k0.as<uint32_t>(), v0.as<uint32_t>(), k1.as<uint32_t>(), v1.as<uint32_t>(),
+                scratch.as<uint64_t>(), g->stream);
+      sync(g);
+    }
   }
   KC_HIP(hipDeviceSynchronize());
 }
-
-// Temporary device buffer freed at scope exit.
-struct DevTemp {
-  void *p = nullptr;
-  explicit DevTemp(size_t bytes) {
-    if (hipMalloc(&p, bytes ? bytes : 64) != hipSuccess) fail(KC_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
-  }
-  ~DevTemp() { (void)hipFree(p); }
-  template <class T>
-  T *as() const {
-    return static_cast<T *>(p);
-  }
-};
 
 // Symmetric, self-loop-free, duplicate-free CSR on the device from an edge
 // list (rows deduplicated in LDS hash tables, so the order inside a row is

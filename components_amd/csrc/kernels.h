@@ -250,6 +250,11 @@ void identity_u32(uint64_t n, uint32_t *p, hipStream_t s);
 void level0_order(uint64_t n, const uint64_t *off, const uint32_t *col, const uint32_t *gid, uint32_t *idx,
                   uint32_t *pos, uint32_t *gidp, uint64_t *offp, uint32_t *colp, uint32_t *k0, uint32_t *v0,
                   uint32_t *k1, uint32_t *v1, uint64_t *sort_scratch, uint64_t *scan_scratch, hipStream_t s);
+// Every row of a CSR (n rows, m2 slots) sorted ascending, in place: the
+// (column, row) pairs sorted by column, then stably by row.  k0, v0, k1, v1:
+// m2 words each; scratch: radix_sort_scratch_words(m2) words.
+void sort_rows(uint64_t n, const uint64_t *off, uint32_t *col, uint64_t m2, uint32_t *k0, uint32_t *v0, uint32_t *k1,
+               uint32_t *v1, uint64_t *scratch, hipStream_t s);
 // (size, number of components of that size) pairs of a canonical labelling in
 // increasing size; count and hist are n and n + 1 words of scratch, pairs
 // holds 2 * (distinct sizes) words, *num_pairs the number of pairs.

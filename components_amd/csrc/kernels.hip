@@ -2325,6 +2325,29 @@ void level0_order(uint64_t n, const uint64_t *off, const uint32_t *col, const ui
   KC_LAUNCH_CHECK();
 }
 
+// Row ids of a CSR's slots, one wave per row.
+__global__ __launch_bounds__(kBlock) void k_row_ids(uint64_t n, const uint64_t *off, uint32_t *row) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock / kWave) + threadIdx.x / kWave; v < n; v += waves)
+    for (uint64_t e = off[v] + lane_id(); e < off[v + 1]; e += kWave) row[e] = static_cast<uint32_t>(v);
+}
+
+void sort_rows(uint64_t n, const uint64_t *off, uint32_t *col, uint64_t m2, uint32_t *k0, uint32_t *v0, uint32_t *k1,
+               uint32_t *v1, uint64_t *scratch, hipStream_t s) {
+  if (m2 < 2 || n < 2) return;
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < n) ++bits;
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock / kWave), 1u << 16));
+  hipLaunchKernelGGL(k_row_ids, dim3(g), dim3(kBlock), 0, s, n, off, v0);
+  KC_LAUNCH_CHECK();
+  KC_HIP(hipMemcpyAsync(k0, col, m2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  // (column, row) pairs by column, then stably by row: rows in order, each ascending
+  const int w = radix_sort_pairs(m2, bits, k0, v0, k1, v1, scratch, s);
+  uint32_t *sc = w ? k1 : k0, *sr = w ? v1 : v0, *tc = w ? k0 : k1, *tr = w ? v0 : v1;
+  const int w2 = radix_sort_pairs(m2, bits, sr, sc, tr, tc, scratch, s);
+  KC_HIP(hipMemcpyAsync(col, w2 ? tc : sc, m2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+}
+
 void identity_u32(uint64_t n, uint32_t *p, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_identity, dim3(grid_for(n) > 65536 ? 65536 : grid_for(n)), dim3(kBlock), 0, s, n, p);
