@@ -8,6 +8,7 @@ import pytest
 
 import components_amd as ca
 import oracle
+from batch_cases import special_graphs
 
 pytestmark = pytest.mark.gpu
 
@@ -51,25 +52,6 @@ def test_mixed_batch_matches_oracle(stream):
     b = ca.batch_generate(graphs=96, log_n_min=1, log_n_max=11, deg_min=0, deg_max=16, seed=11)
     print(f"batch: {b.graphs} graphs, n={int(b.vtx_off[-1])}, 2m={int(b.edge_off[-1])}", flush=True)
     check_batch(b, seed=1001, stream=stream)
-
-
-def special_graphs():
-    rng = np.random.default_rng(5)
-    star = ca.Graph.from_edges(3001, [(0, i) for i in range(1, 3001)])
-    path = ca.Graph.from_edges(1000, [(i, i + 1) for i in range(999)])
-    clique = ca.Graph.from_edges(40, [(i, j) for i in range(40) for j in range(i + 1, 40)])
-    two = ca.Graph.from_edges(9, [(0, 1), (1, 2), (5, 6), (6, 7), (7, 8)])
-    hubs = ca.Graph.from_edges(2500, [(0, i) for i in range(1, 1200)] + [(1300, i) for i in range(1301, 2500)]
-                               + [(5, 1305)])
-    rnd = ca.Graph.from_edges(700, rng.integers(0, 700, size=(900, 2)))
-    return [
-        ca.Graph(np.zeros(1, np.uint64), np.zeros(0, np.uint32)),  # no vertex
-        ca.Graph(np.zeros(2, np.uint64), np.zeros(0, np.uint32)),  # one vertex
-        ca.Graph(np.zeros(6, np.uint64), np.zeros(0, np.uint32)),  # isolated vertices only
-        ca.Graph.from_edges(2, [(0, 1)]),
-        star, path, clique, two, hubs, rnd,
-        ca.Graph.from_edges(5, [(3, 4)]),  # isolated vertices beside an edge
-    ]
 
 
 @pytest.mark.skip(reason="this batch hung the GPU in its first run (with a since-removed wave-per-row path "
