@@ -275,6 +275,13 @@ struct kc_graph {
   float *feat = nullptr, *target = nullptr;
   uint32_t *ldeg = nullptr, *csize = nullptr;
   uint8_t *delay = nullptr;
+  // Scores of the input's level 0: its features (round index 0) and so its
+  // scores depend on the graph and the model alone, not on the seed or alpha;
+  // kept for the model and storage layout they were computed for.
+  float *l0_score = nullptr;
+  const kc_mlp *l0_score_mlp = nullptr;
+  uint64_t l0_score_version = 0;
+  const uint64_t *l0_score_off = nullptr;
   uint64_t learned_cap = 0;  // vertices the learned buffers hold
   BinCfg tuned{0, 0, 0};     // thresholds kept by kc_graph_autotune (tmax 0: none)
   // level-sized work buffers hold levels of up to cap_n vertices and cap_m2
@@ -857,6 +864,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     dev_gid_l = g->gidp0;
   }
   const uint32_t *canon_gid = ordered ? g->gidp0 : g->gid0;  // input id of each level-0 storage position
+  bool input_level0 = true;  // the level is the input's level 0 (not replicated, not contracted)
   if (o->replicate && !local && m2_l > 0) {
     // level 0 with its hubs split into proxies (input order); the canonical
     // labels below read the first n0 positions, the real vertices
@@ -868,6 +876,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       col_l = rp.col;
       gid_l = rp.gid;
       dev_gid_l = rp.gid;
+      input_level0 = false;
       out.hubs = rp.H;
       out.replicas = rp.R;
       if (o->learned || g->collect) ensure_learned_buffers(g, n_l);
@@ -939,11 +948,26 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       // deviates -> keys
       LearnedArgs la{nullptr};
       if (!local && o->learned) {
-        // scores of this level's vertices delay their deviates
-        TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
-        level_degrees(n_l, off_l, g->ldeg, g->stream);
-        level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
-        mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
+        // scores of this level's vertices delay their deviates; the input's
+        // level 0 reuses its scores from an earlier call with the same model
+        const bool l0 = input_level0 && iteration == 1;
+        if (l0 && g->l0_score && g->l0_score_mlp == g->mlp && g->l0_score_version == g->mlp->version &&
+            g->l0_score_off == off_l) {
+          TimedScope lt(g->timer, KC_K_LEARNED, 4 * n_l + n_l);
+          score_delays(n_l, g->l0_score, o->learned_alpha, g->delay, g->stream);
+        } else {
+          TimedScope lt(g->timer, KC_K_LEARNED, 8 * (n_l + 1) + 4 * n_l + 40 * n_l + 24 * n_l + n_l);
+          level_degrees(n_l, off_l, g->ldeg, g->stream);
+          level_features(n_l, off_l, col_l, g->ldeg, iteration - 1, g->feat, g->stream);
+          mlp_delays(g->mlp, n_l, g->feat, g->target, o->learned_alpha, g->delay, g->stream);
+          if (l0) {
+            if (!g->l0_score) g->l0_score = g->dalloc<float>(g->n0);
+            KC_HIP(hipMemcpyAsync(g->l0_score, g->target, n_l * sizeof(float), hipMemcpyDeviceToDevice, g->stream));
+            g->l0_score_mlp = g->mlp;
+            g->l0_score_version = g->mlp->version;
+            g->l0_score_off = off_l;
+          }
+        }
         la.delay = g->delay;
       }
       {
@@ -1137,6 +1161,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     pos_l = nullptr;
     pp ^= 1;
     bins_valid = false;
+    input_level0 = false;
     if (!pre) ++iteration;
   }
   out.levels = static_cast<uint32_t>(g->levels.size());
@@ -2016,6 +2041,7 @@ int kc_graph_set_mlp(kc_graph *g, kc_mlp *m) {
     if (!g) fail(KC_ERR_INVALID, "null graph");
     if (m) KC_HIP(hipStreamSynchronize(m->stream));
     g->mlp = m;
+    g->l0_score_mlp = nullptr;  // level 0's scores are computed again for the new model
   });
 }
 

@@ -82,6 +82,8 @@ void cluster_targets(uint64_t n, const uint32_t *lab, uint32_t *size, float *y, 
 void mlp_infer(const kc_mlp *m, uint64_t n, const float *X, float *score, hipStream_t s);
 // Scores, then delay[v] = min(255, round(alpha * max(score[v], 0))).
 void mlp_delays(const kc_mlp *m, uint64_t n, const float *X, float *score, float alpha, uint8_t *delay, hipStream_t s);
+// The delays alone, from scores computed earlier.
+void score_delays(uint64_t n, const float *score, float alpha, uint8_t *delay, hipStream_t s);
 // Appends rows (every stride-th of n) of X / y to the model's dataset.
 void mlp_append(kc_mlp *m, uint64_t n, const float *X, const float *y, uint64_t stride, hipStream_t s);
 

@@ -567,6 +567,12 @@ void mlp_delays(const kc_mlp *m, uint64_t n, const float *X, float *score, float
   KC_LAUNCH_CHECK();
 }
 
+void score_delays(uint64_t n, const float *score, float alpha, uint8_t *delay, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_delays, dim3(grid_of(n)), dim3(kBlock), 0, s, n, score, alpha, delay);
+  KC_LAUNCH_CHECK();
+}
+
 void mlp_append(kc_mlp *m, uint64_t n, const float *X, const float *y, uint64_t stride, hipStream_t s) {
   if (!n) return;
   if (stride < 1) stride = 1;

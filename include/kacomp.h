@@ -476,7 +476,11 @@ int kc_mlp_add_dataset(kc_mlp *m, const float *features, const float *targets, u
 void kc_train_params_init(kc_train_params *p);
 /* Adam on the dataset with the project's MFMA forward and backward kernels. */
 int kc_mlp_train(kc_mlp *m, const kc_train_params *p, kc_train_stats *stats);
-/* Model used by kc_find_components on g when kc_options.learned is set (NULL detaches). */
+/* Model used by kc_find_components on g when kc_options.learned is set (NULL detaches).
+ * The features of g's level 0 (round index 0), hence its scores, depend on g
+ * and the model only: the first learned call computes them and later calls
+ * reuse them (each call still derives its delays with its own learned_alpha)
+ * until the model is attached again or its weights change. */
 int kc_graph_set_mlp(kc_graph *g, kc_mlp *m);
 /* Features of the level-0 vertices of g (its vertices of degree > 0 in id
  * order), non_isolated x 6 floats. */

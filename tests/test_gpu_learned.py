@@ -146,6 +146,45 @@ def test_learned_labels_exact():
         m.close()
 
 
+def test_level0_scores_reused():
+    """Level 0's features (round index 0), hence its scores, depend on the graph and the model only: a
+    second call reuses them (the learned class moves far fewer bytes) and gives the same levels and
+    keys; new weights or a re-attached model compute them again."""
+    m, _ = trained_model(steps=50)
+    try:
+        g = ca.generate("rmat", n=1 << 15, m=1 << 18, seed=7)
+        dg = ca.DeviceGraph(g)
+        dg.set_mlp(m)
+        try:
+            kw = dict(seed=5, learned=True, alpha=1.0, seq_limit=0, record_trace=True, stream=ca.STREAM_HASHED,
+                      use_bfs=True, time_kernels=True)
+            lab1, st1 = dg.find_components(**kw)
+            tr1 = dg.trace()
+            lab2, st2 = dg.find_components(**kw)
+            tr2 = dg.trace()
+            assert st2["kernel_bytes"]["learned"] < st1["kernel_bytes"]["learned"]
+            assert np.array_equal(lab1, lab2) and len(tr1) == len(tr2)
+            for (i1, k1, g1), (i2, k2, g2) in zip(tr1, tr2):
+                assert i1 == i2 and np.array_equal(k1, k2) and np.array_equal(g1, g2)
+            # another alpha from the same scores: the delays change, level 0's scores are reused (5 bytes
+            # per level-0 vertex instead of about 77; later levels are far smaller)
+            _, st3 = dg.find_components(**dict(kw, alpha=2.0))
+            assert st3["kernel_bytes"]["learned"] < st1["kernel_bytes"]["learned"]
+            dg.set_mlp(m)  # re-attached: computed again
+            lab4, st4 = dg.find_components(**kw)
+            assert st4["kernel_bytes"]["learned"] == st1["kernel_bytes"]["learned"]
+            w = m.get_weights()
+            w[-1] += 0.5  # the output bias: new weights, a new version
+            m.set_weights(w)
+            _, st5 = dg.find_components(**kw)
+            assert st5["kernel_bytes"]["learned"] > st2["kernel_bytes"]["learned"]
+            assert np.array_equal(lab4, oracle.canonical_labels(g.offsets, g.targets))
+        finally:
+            dg.close()
+    finally:
+        m.close()
+
+
 def level_sizes(dg, **kw):
     _, st = dg.find_components(record_trace=True, **kw)
     return [info[0] for info, _, _ in dg.trace()], st
