@@ -170,6 +170,10 @@ def main():
     bytes_per_launch = sum(s["kernel_bytes"][dom] for s in stats) / launches
     avg_launch_ms = sum(s["kernel_ms"][dom] for s in stats) / launches
     achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9 if avg_launch_ms > 0 else 0.0
+    # the first level's launch alone (the levels' launches differ in shape: level 0 is the large one)
+    first_ms = float(np.mean([s["kernel_first_ms"][dom] for s in stats]))
+    first_bytes = float(np.mean([s["kernel_first_bytes"][dom] for s in stats]))
+    first_gbs = first_bytes / (first_ms / 1e3) / 1e9 if first_ms > 0 else 0.0
     total_kernel_ms = sum(cls_ms.values())
     all_bytes = sum(cls_bytes.values())
     workload = f"rmat{a.scale}_{a.schedule}"
@@ -203,7 +207,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": MI355X_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / MI355X_HBM_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_launch_ms, 5),
-                     "launches_per_step": round(launches / a.steps, 2)},
+                     "launches_per_step": round(launches / a.steps, 2),
+                     "level0_launch": {"achieved": round(first_gbs, 2), "frac": round(first_gbs / MI355X_HBM_GBS, 4),
+                                       "alg_bytes": int(first_bytes), "ms": round(first_ms, 5)}},
         "whole_step": {"alg_bytes": int(all_bytes / nb), "kernel_ms": round(total_kernel_ms / nb, 4),
                        "alg_gbs": round(all_bytes / (total_kernel_ms / 1e3) / 1e9, 2) if total_kernel_ms else 0.0,
                        "breakdown_steps": len(brk)},

@@ -66,7 +66,8 @@ class Stats(ctypes.Structure):
                 ("kernel_ms", ctypes.c_double * _NK), ("kernel_launches", ctypes.c_uint64 * _NK),
                 ("kernel_bytes", ctypes.c_uint64 * _NK), ("replicas", ctypes.c_uint64),
                 ("bin_small", ctypes.c_uint32), ("bin_mid", ctypes.c_uint32), ("bin_large", ctypes.c_uint32),
-                ("hubs", ctypes.c_uint32)]
+                ("hubs", ctypes.c_uint32), ("kernel_first_ms", ctypes.c_double * _NK),
+                ("kernel_first_bytes", ctypes.c_uint64 * _NK)]
 
 
 class BinTuning(ctypes.Structure):
@@ -369,6 +370,8 @@ def stats_dict(st):
     d["kernel_ms"] = dict(zip(KERNEL_CLASSES, list(st.kernel_ms)))
     d["kernel_launches"] = dict(zip(KERNEL_CLASSES, list(st.kernel_launches)))
     d["kernel_bytes"] = dict(zip(KERNEL_CLASSES, list(st.kernel_bytes)))
+    d["kernel_first_ms"] = dict(zip(KERNEL_CLASSES, list(st.kernel_first_ms)))
+    d["kernel_first_bytes"] = dict(zip(KERNEL_CLASSES, list(st.kernel_first_bytes)))
     return d
 
 

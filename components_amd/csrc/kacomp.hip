@@ -131,10 +131,14 @@ void KernelTimer::retag(int slot, int cls, uint64_t alg_bytes) {
 
 void KernelTimer::finish(kc_stats *st) {
   if (!st) return;
+  bool seen[KC_KERNEL_CLASSES];
   for (int i = 0; i < KC_KERNEL_CLASSES; ++i) {
     st->kernel_ms[i] = 0.0;
     st->kernel_launches[i] = 0;
     st->kernel_bytes[i] = 0;
+    st->kernel_first_ms[i] = 0.0;
+    st->kernel_first_bytes[i] = 0;
+    seen[i] = false;
   }
   if (enabled_) KC_HIP(hipStreamSynchronize(stream_));
   for (auto &r : recs_) {  // launches without events add no time
@@ -144,6 +148,11 @@ void KernelTimer::finish(kc_stats *st) {
       float ms = 0.f;
       KC_HIP(hipEventElapsedTime(&ms, pool_[r.ev], pool_[r.ev + 1]));
       st->kernel_ms[r.cls] += ms;
+      if (!seen[r.cls]) {
+        seen[r.cls] = true;
+        st->kernel_first_ms[r.cls] = ms;
+        st->kernel_first_bytes[r.cls] = r.bytes;
+      }
     }
   }
 }
@@ -2001,6 +2010,13 @@ int kc_graph_autotune(kc_graph *g, const kc_options *opt, kc_bin_tuning *chosen)
             bl = c;
           }
         }
+    // a choice within 2% of the defaults is run-to-run noise: keep the defaults
+    if (best > 0.98 * def_ms) {
+      best = def_ms;
+      bs = 16;
+      bm = 256;
+      bl = 4096;
+    }
     g->tuned = bin_config(bs, bm, bl);
     if (chosen) {
       chosen->small = static_cast<int32_t>(g->tuned.tmax);

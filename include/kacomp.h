@@ -222,6 +222,9 @@ typedef struct kc_stats {
   uint64_t replicas;    /* proxies added to level 0 by kc_options.replicate */
   uint32_t bin_small, bin_mid, bin_large; /* degree-bin thresholds the call used */
   uint32_t hubs;        /* hubs split by kc_options.replicate */
+  double kernel_first_ms[KC_KERNEL_CLASSES];    /* with time_kernels: the class's first timed launch group
+                                                   (the first level's, for the per-level classes) */
+  uint64_t kernel_first_bytes[KC_KERNEL_CLASSES]; /* its algorithmic bytes */
 } kc_stats;
 
 typedef struct kc_graph kc_graph;
@@ -280,7 +283,8 @@ typedef struct kc_bin_tuning {
 } kc_bin_tuning;
 /* Autotuner of the degree bins (no reference counterpart; the reference has no
  * bins): times every candidate triple with the options opt (their thresholds
- * ignored) on g and keeps the fastest in g, where later kc_find_components
+ * ignored) on g and keeps the fastest in g (the defaults unless a triple is
+ * more than 2% faster), where later kc_find_components
  * calls whose sweep_group_* fields are 0 use it.  Results do not depend on the
  * thresholds; only the time does.  chosen may be NULL. */
 int kc_graph_autotune(kc_graph *g, const kc_options *opt, kc_bin_tuning *chosen);
