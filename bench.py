@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--smax", type=int, default=0)
     ap.add_argument("--wmax", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in degree-bin thresholds")
+    ap.add_argument("--relabel-split", default="default",
+                    help="owner-split relabel of the degree-ordered level 0: on, off (one pass over whole rows), "
+                         "default, or the row length above which rows are split")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other plain workloads")
@@ -85,9 +88,14 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_per_launch": 1 << 30, "launches": 3}), flush=True)
 
 
-def read_traffic(kernel, workload):
+SPLIT_MODES = {"default": 0, "on": 1, "off": -1}
+
+
+def read_traffic(kernel, workload, key="hbm_bytes_per_launch"):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/traffic_<workload>.txt, lines `key: value`), or None."""
+    (profiles/traffic_<workload>.txt, lines `key: value`), or None: `key` is
+    hbm_bytes_per_launch (the average over the kernel's launches in a call) or
+    level0_hbm_bytes_per_launch (its first-level launch)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
@@ -100,7 +108,7 @@ def read_traffic(kernel, workload):
     if kv.get("kernel") != kernel:
         return None
     try:
-        return int(float(kv["hbm_bytes_per_launch"]))
+        return int(float(kv[key]))
     except (KeyError, ValueError):
         return None
 
@@ -131,7 +139,8 @@ def main():
     seq_limit = max(m // 256, 4096)  # ComputeSequentialLimit with one PE per compute unit
     run_kw = dict(variant="exp", beta=0.1, seq_limit=seq_limit, stream=stream, tmax=a.tmax, smax=a.smax, wmax=a.wmax,
                   use_bfs=a.schedule == "rounds", tail_sweeps=a.tail_sweeps,
-                  pull_rounds=a.pull_rounds, dedup_huge=a.dedup_huge)
+                  pull_rounds=a.pull_rounds, dedup_huge=a.dedup_huge,
+                  relabel_split=SPLIT_MODES[a.relabel_split] if a.relabel_split in SPLIT_MODES else int(a.relabel_split))
     print(f"[bench] rmat scale {a.scale}: n={n} m={m} non_isolated={dg.non_isolated} built in {gen_s:.1f}s",
           file=sys.stderr, flush=True)
 
@@ -178,6 +187,7 @@ def main():
     all_bytes = sum(cls_bytes.values())
     workload = f"rmat{a.scale}_{a.schedule}"
     traffic = read_traffic(kernel, workload)
+    traffic0 = read_traffic(kernel, workload, "level0_hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -195,6 +205,7 @@ def main():
         "config": {"workload": f"rmat-scale{a.scale}-ef{a.edge_factor}-abcd(0.57,0.19,0.19,0.05)-seed3",
                    "n": n, "m": m, "non_isolated": dg.non_isolated, "form": "plain",
                    "deviate_stream": a.stream, "schedule": a.schedule, "beta": 0.1, "seq_limit": seq_limit,
+                   "relabel": "one-pass" if a.relabel_split == "off" else "owner-split",
                    "bins": {"small": stats[-1]["bin_small"], "mid": stats[-1]["bin_mid"],
                             "large": stats[-1]["bin_large"],
                             "tuned": None if bins is None else {k: (round(v, 4) if isinstance(v, float) else v)
@@ -209,7 +220,8 @@ def main():
                      "alg_bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_launch_ms, 5),
                      "launches_per_step": round(launches / a.steps, 2),
                      "level0_launch": {"achieved": round(first_gbs, 2), "frac": round(first_gbs / MI355X_HBM_GBS, 4),
-                                       "alg_bytes": int(first_bytes), "ms": round(first_ms, 5)}},
+                                       "alg_bytes": int(first_bytes), "ms": round(first_ms, 5),
+                                       "traffic": traffic0}},
         "whole_step": {"alg_bytes": int(all_bytes / nb), "kernel_ms": round(total_kernel_ms / nb, 4),
                        "alg_gbs": round(all_bytes / (total_kernel_ms / 1e3) / 1e9, 2) if total_kernel_ms else 0.0,
                        "breakdown_steps": len(brk)},

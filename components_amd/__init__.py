@@ -55,7 +55,8 @@ class Options(ctypes.Structure):
                 ("pull_rounds", ctypes.c_int32),
                 ("dedup_huge", ctypes.c_int32), ("time_class", ctypes.c_int32),
                 ("pre_contract", ctypes.c_int32), ("pre_blocks", ctypes.c_uint32),
-                ("replicate", ctypes.c_int32), ("hd_threshold", ctypes.c_double)]
+                ("replicate", ctypes.c_int32), ("hd_threshold", ctypes.c_double),
+                ("relabel_split", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -335,7 +336,7 @@ PRE_MODES = {"none": 0, "local": 1, "single": 2, "double": 3}
 def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_trace=False, time_kernels=False,
             tmax=0, smax=0, wmax=0, use_bfs=False, max_levels=4096, stream=STREAM_REFERENCE, learned=False,
             alpha=1.0, input_order=False, tail_sweeps=0, pull_rounds=0, dedup_huge="auto", time_class=None,
-            pre_contract="none", pre_blocks=1, replicate=False, hd_threshold=16.0):
+            pre_contract="none", pre_blocks=1, replicate=False, hd_threshold=16.0, relabel_split=0):
     o = Options()
     lib().kc_options_init(ctypes.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -362,6 +363,8 @@ def options(variant="exp", seed=1, beta=0.1, seq_limit=4096, fma=True, record_tr
     o.pre_blocks = int(pre_blocks)
     o.replicate = 1 if replicate else 0
     o.hd_threshold = float(hd_threshold)
+    # relabel of the degree-ordered level 0: 1 owner-split, -1 one pass, 0 default
+    o.relabel_split = int(relabel_split)
     return o
 
 

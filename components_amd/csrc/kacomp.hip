@@ -293,6 +293,19 @@ struct kc_graph {
   const uint64_t *l0_score_off = nullptr;
   uint64_t learned_cap = 0;  // vertices the learned buffers hold
   BinCfg tuned{0, 0, 0};     // thresholds kept by kc_graph_autotune (tmax 0: none)
+  // Pieces of the degree-ordered level 0's rows above the 16-lane bin for the
+  // owner-split relabel (one per owner group of a row, cut every wmax slots),
+  // their counts, offsets in their row's run and prefix, the piece indices of
+  // each owner part (part p: l0_part_list[l0_part_at[p] .. l0_part_at[p+1]));
+  // built by the first call that needs them, for the bins they were cut with.
+  HubChunk *l0_pieces = nullptr;
+  uint32_t *l0_part_list = nullptr, *l0_piece_inter = nullptr, *l0_piece_pos = nullptr;
+  uint64_t *l0_piece_prefix = nullptr;
+  uint64_t l0_npieces = 0;
+  uint32_t l0_part_at[kParts + 2] = {};  // [kParts .. kParts+1): the shared pieces
+  BinCfg l0_piece_cfg{0, 0, 0};
+  uint64_t l0_piece_split = 0;  // rows above this many slots cut by owner part
+  bool l0_pieces_valid = false;
   // level-sized work buffers hold levels of up to cap_n vertices and cap_m2
   // edge slots (the input's level 0, or a larger replicated one)
   uint64_t cap_n = 0, cap_m2 = 0;
@@ -349,6 +362,10 @@ namespace {
 
 constexpr uint32_t kMaxFlags = 1u << 16;
 constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep the input order
+// Owner-split relabel: the degree-ordered level 0's rows of more slots than
+// this are cut by the owner part of their targets (shorter rows above the
+// 16-lane bin stay whole)
+constexpr uint64_t kSplitRowsAbove = 4096;
 constexpr int kTailShift = 5;  // -bfs: pull sweeps finish a level once at most n/32 rows are pending
 constexpr uint64_t kPullRoundsMin = 1u << 16;  // -bfs: levels at least this large choose round directions
 constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulled
@@ -718,6 +735,7 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
 
 void ensure_work(kc_graph *g, uint64_t n, uint64_t m2);
 void ensure_learned_buffers(kc_graph *g, uint64_t n = 0);
+void ensure_pieces(kc_graph *g, const BinCfg &cfg, uint64_t split);
 
 struct RunOut {
   uint64_t sweeps = 0, radius_sum = 0;
@@ -1076,16 +1094,38 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     // runs of the dedup.  Its bytes (offsets, neighbour ids and their ids, the
     // row's id and count, the packed writes) are booked once P is known.
     uint32_t *relab = g->slots;
+    // the degree-ordered level 0 relabels the rows above the 16-lane bin in
+    // owner pieces, which then stand in for its wave rows and hub chunks in
+    // the count fold and the bucket copy
+    const bool split = pos_l && off_l == g->off0p && n_l == g->n0 && o->relabel_split >= 0;
+    Bins rb = b;
+    uint32_t *r_inter = g->hub_inter, *r_pos = g->hub_pos;
+    uint64_t *r_prefix = g->hub_prefix;
+    if (split) {
+      ensure_pieces(g, cfg, o->relabel_split > 1 ? static_cast<uint64_t>(o->relabel_split) : kSplitRowsAbove);
+      rb.listW = nullptr;
+      rb.nW = 0;
+      rb.hubs = g->l0_pieces;
+      rb.nH = static_cast<uint32_t>(g->l0_npieces);
+      finish_bins(rb);
+      r_inter = g->l0_piece_inter;
+      r_pos = g->l0_piece_pos;
+      r_prefix = g->l0_piece_prefix;
+    }
     int relabel_slot;
     {
       TimedScope ts(g->timer, KC_K_RELABEL, 0);
       relabel_slot = ts.slot;
       // the hot-label launch pays off on the degree-ordered level 0
-      relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l ? n_l : 0, g->stream);
+      if (split)
+        relabel_split(off_l, col_l, g->cid, n_l, b, g->l0_pieces, g->l0_part_list, g->l0_part_at, relab,
+                      g->deg_inter, r_inter, n_l, g->stream);
+      else
+        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l ? n_l : 0, g->stream);
     }
-    if (b.nH) {
-      TimedScope ts(g->timer, KC_K_COUNT, 12ull * b.nH + 4ull * b.nH);
-      hub_fold(b, g->hub_inter, g->deg_inter, g->hub_pos, g->hub_prefix, g->scan_scratch, g->stream);
+    if (rb.nH) {
+      TimedScope ts(g->timer, KC_K_COUNT, 12ull * rb.nH + 4ull * rb.nH);
+      hub_fold(rb, r_inter, g->deg_inter, r_pos, r_prefix, g->scan_scratch, g->stream);
     }
     // clusters as segments of the (label, vertex) list sorted by label
     {
@@ -1139,7 +1179,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     {
       g->timer.retag(relabel_slot, KC_K_RELABEL, 8 * (n_l + 1) + 8 * m2_l + 8 * n_l + 4 * P);
       TimedScope ts(g->timer, KC_K_SCATTER, 8 * (n_l + 1) + 8 * n_l + 8 * P);
-      scatter(off_l, relab, b, g->deg_inter, g->hub_inter, g->hub_pos, g->pos_of, bucket, g->stream);
+      scatter(off_l, relab, rb, g->deg_inter, r_inter, r_pos, g->pos_of, bucket, g->stream);
     }
     const int dedup_slot = dedup_buckets(g, n_next, g->seg_off, bucket, g->slots, g->uniq, g->dl_medium, g->dl_huge,
                                          n_medium, n_huge, P, clusters, g->scan_scratch, o->dedup_huge);
@@ -1337,6 +1377,56 @@ void alloc_work(kc_graph *g, uint64_t cap_n, uint64_t cap_m2) {
   g->l0_off = nullptr;
 }
 
+// The pieces of the degree-ordered level 0 for the bins cfg, the rows above
+// `split` slots cut by owner part (kept while the mid and large thresholds and
+// the bound stay the same).
+void ensure_pieces(kc_graph *g, const BinCfg &cfg, uint64_t split) {
+  if (g->l0_pieces_valid && g->l0_piece_cfg.smax == cfg.smax && g->l0_piece_cfg.wmax == cfg.wmax &&
+      g->l0_piece_split == split)
+    return;
+  g->l0_pieces_valid = false;
+  const uint64_t n = g->n0;
+  DevTemp cnt(n * 4 + 64), start((n + 1) * 8 + 64);
+  count_pieces(n, g->off0p, g->col0p, cfg.smax, cfg.wmax, split, cnt.as<uint32_t>(), start.as<uint64_t>(),
+               g->scan_scratch, g->stream);
+  uint64_t total = 0;
+  KC_HIP(hipMemcpyAsync(&total, start.as<uint64_t>() + n, sizeof(total), hipMemcpyDeviceToHost, g->stream));
+  sync(g);
+  if (total >= (1ull << 32)) fail(KC_ERR_INVALID, "internal: %llu relabel pieces", (unsigned long long)total);
+  g->drealloc(g->l0_pieces, total);
+  g->drealloc(g->l0_part_list, total);
+  g->drealloc(g->l0_piece_inter, total);
+  g->drealloc(g->l0_piece_pos, total);
+  g->drealloc(g->l0_piece_prefix, total + 1);
+  std::vector<uint8_t> part(total);
+  if (total) {
+    DevTemp dpart(total + 64);
+    write_pieces(n, g->off0p, g->col0p, cfg.smax, cfg.wmax, split, start.as<uint64_t>(), g->l0_pieces,
+                 dpart.as<uint8_t>(), g->stream);
+    KC_HIP(hipMemcpyAsync(part.data(), dpart.as<uint8_t>(), total, hipMemcpyDeviceToHost, g->stream));
+    sync(g);
+  }
+  // piece indices grouped by part (the shared ones last), ascending inside a part
+  uint32_t at[kParts + 2] = {};
+  for (uint8_t p : part) {
+    if (p > kParts) fail(KC_ERR_INVALID, "internal: relabel piece of part %u", p);
+    ++at[p + 1];
+  }
+  for (uint32_t p = 0; p <= kParts; ++p) at[p + 1] += at[p];
+  std::vector<uint32_t> list(total);
+  uint32_t next[kParts + 1];
+  for (uint32_t p = 0; p <= kParts; ++p) next[p] = at[p];
+  for (uint64_t i = 0; i < total; ++i) list[next[part[i]]++] = static_cast<uint32_t>(i);
+  if (total)
+    KC_HIP(hipMemcpyAsync(g->l0_part_list, list.data(), total * 4, hipMemcpyHostToDevice, g->stream));
+  sync(g);
+  for (uint32_t p = 0; p <= kParts + 1; ++p) g->l0_part_at[p] = at[p];
+  g->l0_npieces = total;
+  g->l0_piece_cfg = cfg;
+  g->l0_piece_split = split;
+  g->l0_pieces_valid = true;
+}
+
 // Grows the work set to hold a level of n vertices and m2 edge slots.
 void ensure_work(kc_graph *g, uint64_t n, uint64_t m2) {
   if (n <= g->cap_n && m2 <= g->cap_m2) return;
@@ -1378,8 +1468,10 @@ void init_work(kc_graph *g) {
     g->col0p = g->dalloc<uint32_t>(m2);
     level0_order(n0, g->off0c, g->col0c, g->gid0, g->idx0, g->pos0, g->gidp0, g->off0p, g->col0p, g->sk, g->sv, g->sk2,
                  g->sv2, g->sort_scratch, g->scan_scratch, g->stream);
-    // every row ascending: a row's gathers then walk up the (degree-ordered)
-    // storage, and neighbouring entries share cache lines more often
+    // every row grouped by the owner part of its targets, ascending inside a
+    // group: the owner-split relabel cuts rows at the group bounds, and a
+    // row's gathers walk up the (degree-ordered) storage, neighbouring entries
+    // sharing cache lines more often
     {
       DevTemp k0(m2 * 4 + 64), v0(m2 * 4 + 64), k1(m2 * 4 + 64), v1(m2 * 4 + 64);
       DevTemp scratch(radix_sort_scratch_words(m2) * 8 + 64);

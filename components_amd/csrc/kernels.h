@@ -40,6 +40,12 @@ inline BinCfg bin_config(int32_t small, int32_t mid, int32_t large) {
 // chunks of wmax slots, so at most m2 / (kMinWaveBin / 2) + 2 of them.
 inline uint64_t hub_chunk_cap(uint64_t m2) { return m2 / (kMinWaveBin / 2) + 2; }
 
+// Owner parts of a degree-ordered level 0: vertex t belongs to part
+// owner_of(t), the 128-byte line of its 4-byte label taken modulo kParts, so
+// each part's labels are one line in every kParts (see relabel_split).
+constexpr uint32_t kParts = 8;
+__host__ __device__ inline uint32_t owner_of(uint32_t t) { return (t >> 5) & (kParts - 1); }
+
 // Degree bins of one level; lists and chunks live on the device.
 struct Bins {
   uint32_t *listT = nullptr, *listS = nullptr, *listW = nullptr;
@@ -154,6 +160,27 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
 // degree-ordered levels, whose first vertices take most endpoints).
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
              uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s);
+// Owner-split relabel of a degree-ordered level 0 whose rows are grouped by
+// owner (sort_rows): the one-lane and 16-lane bins of b as relabel does, and
+// the pieces of the longer rows (write_pieces): the shared ones by every
+// block with the first n_hot labels in LDS, the others by the blocks of their
+// owner part with that part's labels in LDS (part_at: kParts + 2 bounds into
+// part_list); piece counts go to piece_inter.  Same cluster edges per row as relabel, split over the row's
+// pieces (hub_fold with the pieces as chunks gives the rows' counts).
+void relabel_split(const uint64_t *off, const uint32_t *col, const uint32_t *lab, uint64_t n, const Bins &b,
+                   const HubChunk *pieces, const uint32_t *part_list, const uint32_t *part_at, uint32_t *relab,
+                   uint32_t *deg_inter, uint32_t *piece_inter, uint64_t n_hot, hipStream_t s);
+// Pieces of the rows of more than smax slots of an owner-grouped CSR: cnt[v]
+// = the row's pieces (rows of at most `split` slots: the whole row, cut every
+// wmax slots, shared by all blocks; longer rows: one per owner group they
+// have, cut every wmax slots), start (n + 1 words) = their exclusive prefix,
+// start[n] = the total.
+void count_pieces(uint64_t n, const uint64_t *off, const uint32_t *col, uint32_t smax, uint32_t wmax, uint64_t split,
+                  uint32_t *cnt, uint64_t *start, uint64_t *scratch, hipStream_t s);
+// pieces[start[v] ...] = row v's pieces (HubChunk, k = index in the row),
+// part[...] = their owner parts (kParts: shared).
+void write_pieces(uint64_t n, const uint64_t *off, const uint32_t *col, uint32_t smax, uint32_t wmax, uint64_t split,
+                  const uint64_t *start, HubChunk *pieces, uint8_t *part, hipStream_t s);
 // Copies the deg_inter[v] packed relab slots of every row (hub chunks:
 // hub_inter) to bucket[pos_of[v] + ...].
 void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,
@@ -250,8 +277,9 @@ void identity_u32(uint64_t n, uint32_t *p, hipStream_t s);
 void level0_order(uint64_t n, const uint64_t *off, const uint32_t *col, const uint32_t *gid, uint32_t *idx,
                   uint32_t *pos, uint32_t *gidp, uint64_t *offp, uint32_t *colp, uint32_t *k0, uint32_t *v0,
                   uint32_t *k1, uint32_t *v1, uint64_t *sort_scratch, uint64_t *scan_scratch, hipStream_t s);
-// Every row of a CSR (n rows, m2 slots) sorted ascending, in place: the
-// (column, row) pairs sorted by column, then stably by row.  k0, v0, k1, v1:
+// Every row of a CSR (n rows, m2 slots) sorted in place by owner part, then
+// ascending inside a part: the (owner key, row) pairs sorted by key, then
+// stably by row.  k0, v0, k1, v1:
 // m2 words each; scratch: radix_sort_scratch_words(m2) words.
 void sort_rows(uint64_t n, const uint64_t *off, uint32_t *col, uint64_t m2, uint32_t *k0, uint32_t *v0, uint32_t *k1,
                uint32_t *v1, uint64_t *scratch, hipStream_t s);

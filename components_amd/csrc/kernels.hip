@@ -415,6 +415,22 @@ struct LabHot {
   const uint32_t *hot;
   uint32_t n_hot;
   __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return c < n_hot ? hot[c] : lab[c]; }
+  __device__ __forceinline__ uint32_t row(uint32_t v) const { return (*this)(v); }
+};
+
+// Label lookups of one owner part p (the vertices t with owner_of(t) == p, a
+// 128-byte line of labels in every kParts lines): the part's share of the
+// first kSplitHot vertices from LDS, packed line by line, the others from the
+// array.  Only targets of part p are looked up through it; the row's own
+// label (any part) comes from the array.
+constexpr uint32_t kRelabelHot = 28672;
+constexpr uint32_t kSplitHot = kParts * kRelabelHot;
+__device__ __forceinline__ uint32_t part_slot(uint32_t t) { return ((t >> 8) << 5) | (t & 31); }
+struct LabPart {
+  const uint32_t *__restrict__ lab;
+  const uint32_t *hot;
+  __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return c < kSplitHot ? hot[part_slot(c)] : lab[c]; }
+  __device__ __forceinline__ uint32_t row(uint32_t v) const { return lab[v]; }
 };
 
 template <int W, class Lab>
@@ -428,7 +444,7 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const Lab &lab
   }
   const int gbase = static_cast<int>(lane_id()) & ~(W - 1);
   const uint64_t gmask = (W >= 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << gbase);
-  const uint32_t lv = active ? lab(v) : 0u;
+  const uint32_t lv = active ? lab.row(v) : 0u;
   // every lane of the wave takes the same number of W-edge steps (the most any
   // group needs), so the ballots below see every lane of each group
   uint64_t rounds = active ? ceil_div(e1 - e0, static_cast<uint64_t>(W)) : 0;
@@ -509,7 +525,6 @@ __device__ __forceinline__ void relabel_block(const SweepArgs &a, const Lab &lab
 // virtual 256-thread blocks of the binned layout in a grid stride, each
 // quarter with its own LDS filter (the quarters are whole waves and need no
 // barrier between them).
-constexpr int kRelabelHot = 28672;
 constexpr int kRelabelQuarters = 4;
 
 __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(SweepArgs a, const uint32_t *lab,
@@ -522,6 +537,155 @@ __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(Sweep
   const LabHot lk{lab, hot, n_hot};
   for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.blocks; blk += gridDim.x * kRelabelQuarters)
     relabel_block(a, lk, relab, blk, tid, filt[q]);
+}
+
+// Owner-split relabel of a degree-ordered level 0.  The rows above the 16-lane
+// bin are cut into pieces, one per owner group of the row (cut again every
+// wmax slots), which the launch spreads over kParts parts of its blocks: the
+// blocks of part p (blockIdx % kParts == p; the device hands consecutive
+// workgroups to its kParts chiplets in turn, each with its own L2) look up
+// only labels of part p, so each chiplet's L2 serves one part's labels and its
+// LDS holds that part's share of the kSplitHot highest-degree vertices.  Every
+// block first runs its share of the one-lane and 16-lane bins (rows kept
+// whole) with the first n_hot labels in LDS, as k_relabel_hot, then reloads
+// LDS with its part's share and runs its part's pieces, one wave per piece.
+// Results do not depend on which chiplet runs a block.
+struct PartRange {
+  // pieces of part p: part_list[at[p] .. at[p + 1]); p = kParts: the shared
+  // pieces (whole rows up to the split bound), run by every block
+  uint32_t at[kParts + 2];
+};
+
+__global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_split(SweepArgs a, const uint32_t *lab,
+                                                                          uint32_t *relab, uint32_t n_hot, uint64_t n,
+                                                                          const HubChunk *pieces,
+                                                                          const uint32_t *part_list, PartRange pr) {
+  __shared__ uint32_t hot[kRelabelHot];
+  __shared__ uint32_t filt[kRelabelQuarters][kFilterWords];
+  const int q = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
+  for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) hot[i] = lab[i];
+  __syncthreads();
+  constexpr uint32_t kWavesPerBlock = kBlock * kRelabelQuarters / kWave;
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t *wfilt = filt[q] + (tid / kWave) * 512;
+  {
+    const LabHot lk{lab, hot, n_hot};
+    for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.bW; blk += gridDim.x * kRelabelQuarters)
+      relabel_block(a, lk, relab, blk, tid, filt[q]);
+    const uint32_t s0 = pr.at[kParts], shared = pr.at[kParts + 1] - s0;
+    for (uint32_t i = blockIdx.x * kWavesPerBlock + wave; i < shared; i += gridDim.x * kWavesPerBlock) {
+      const uint32_t c = part_list[s0 + i];
+      const HubChunk h = pieces[c];
+      relabel_group<kWave>(a, lk, relab, true, h.v, h.e0, h.e1, tid & 63, true, c, wfilt, 511);
+    }
+  }
+  __syncthreads();
+  const uint32_t part = blockIdx.x % kParts;
+  for (uint32_t i = threadIdx.x; i < kRelabelHot; i += kBlock * kRelabelQuarters) {
+    const uint64_t t = (static_cast<uint64_t>(i >> 5) << 8) | (part << 5) | (i & 31);
+    hot[i] = t < n ? lab[t] : kNone;
+  }
+  __syncthreads();
+  const LabPart lp{lab, hot};
+  const uint32_t p0 = pr.at[part], count = pr.at[part + 1] - p0;
+  const uint32_t stride = (gridDim.x / kParts) * kWavesPerBlock;
+  for (uint32_t i = (blockIdx.x / kParts) * kWavesPerBlock + wave; i < count; i += stride) {
+    const uint32_t c = part_list[p0 + i];
+    const HubChunk h = pieces[c];
+    relabel_group<kWave>(a, lp, relab, true, h.v, h.e0, h.e1, tid & 63, true, c, wfilt, 511);
+  }
+}
+
+// Pieces of the rows above smax: bounds of owner group x inside a row whose
+// targets are grouped by owner (binary search for the first slot of part >= x).
+__device__ __forceinline__ uint64_t owner_bound(const uint32_t *col, uint64_t lo, uint64_t hi, uint32_t x) {
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (owner_of(col[mid]) < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// cnt[v] = pieces of row v: 0 up to smax slots; up to split slots the whole
+// row cut every wmax slots (shared pieces); above, the sum over its owner
+// groups of ceil(group slots / wmax).
+__global__ __launch_bounds__(kBlock) void k_piece_count(uint64_t n, const uint64_t *off, const uint32_t *col,
+                                                        uint32_t smax, uint32_t wmax, uint64_t split, uint32_t *cnt) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint64_t e0 = off[v], e1 = off[v + 1];
+    uint64_t c = 0;
+    if (e1 - e0 > smax && e1 - e0 <= split) {
+      c = ceil_div(e1 - e0, wmax);
+    } else if (e1 - e0 > smax) {
+      uint64_t lo = e0;
+      for (uint32_t x = 1; x <= kParts; ++x) {
+        const uint64_t hi = x < kParts ? owner_bound(col, lo, e1, x) : e1;
+        c += ceil_div(hi - lo, wmax);
+        lo = hi;
+      }
+    }
+    cnt[v] = static_cast<uint32_t>(c);
+  }
+}
+
+// The pieces of every row from start[v] (exclusive prefix of the counts), in
+// row order and within a row by owner group, then slot: the chunks of one row
+// are consecutive with k = 0, 1, ... as hub chunks are.
+__global__ __launch_bounds__(kBlock) void k_piece_write(uint64_t n, const uint64_t *off, const uint32_t *col,
+                                                        uint32_t smax, uint32_t wmax, uint64_t split,
+                                                        const uint64_t *start, HubChunk *pieces, uint8_t *part) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint64_t e0 = off[v], e1 = off[v + 1];
+    if (e1 - e0 <= smax) continue;
+    uint64_t p = start[v];
+    uint32_t k = 0;
+    if (e1 - e0 <= split) {
+      for (uint64_t e = e0; e < e1; e += wmax) {
+        pieces[p] = HubChunk{static_cast<uint32_t>(v), k, e, e + wmax < e1 ? e + wmax : e1};
+        part[p] = static_cast<uint8_t>(kParts);
+        ++p;
+        ++k;
+      }
+      continue;
+    }
+    uint64_t lo = e0;
+    for (uint32_t x = 1; x <= kParts; ++x) {
+      const uint64_t hi = x < kParts ? owner_bound(col, lo, e1, x) : e1;
+      for (uint64_t e = lo; e < hi; e += wmax) {
+        pieces[p] = HubChunk{static_cast<uint32_t>(v), k, e, e + wmax < hi ? e + wmax : hi};
+        part[p] = static_cast<uint8_t>(x - 1);
+        ++p;
+        ++k;
+      }
+      lo = hi;
+    }
+  }
+}
+
+// Owner-grouped sort keys over B >= 8 bits (t < 2^B): the owner in the top
+// three bits, then the target without its owner bits, so that ascending keys
+// list a row's targets by owner group, ascending inside a group.
+__device__ __forceinline__ uint32_t owner_key(uint32_t t, int B) {
+  return (owner_of(t) << (B - 3)) | ((t >> 8) << 5) | (t & 31);
+}
+__device__ __forceinline__ uint32_t owner_unkey(uint32_t k, int B) {
+  const uint32_t x = k >> (B - 3), r = k & ((1u << (B - 3)) - 1u);
+  return ((r >> 5) << 8) | (x << 5) | (r & 31);
+}
+__global__ __launch_bounds__(kBlock) void k_owner_keys(uint64_t m2, const uint32_t *col, uint32_t *key, int B) {
+  for (uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; e < m2;
+       e += static_cast<uint64_t>(gridDim.x) * kBlock)
+    key[e] = owner_key(col[e], B);
+}
+__global__ __launch_bounds__(kBlock) void k_owner_cols(uint64_t m2, const uint32_t *key, uint32_t *col, int B) {
+  for (uint64_t e = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; e < m2;
+       e += static_cast<uint64_t>(gridDim.x) * kBlock)
+    col[e] = owner_unkey(key[e], B);
 }
 
 // From the exclusive prefix of all chunk counts: a chunk's offset in its hub's
@@ -2054,6 +2218,41 @@ void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, cons
   KC_LAUNCH_CHECK();
 }
 
+void relabel_split(const uint64_t *off, const uint32_t *col, const uint32_t *lab, uint64_t n, const Bins &b,
+                   const HubChunk *pieces, const uint32_t *part_list, const uint32_t *part_at, uint32_t *relab,
+                   uint32_t *deg_inter, uint32_t *piece_inter, uint64_t n_hot, hipStream_t s) {
+  if (b.bW == 0 && part_at[kParts + 1] == 0) return;
+  SweepArgs a{off, col, nullptr, nullptr, 0, b, deg_inter, piece_inter, nullptr, nullptr};
+  PartRange pr;
+  for (uint32_t p = 0; p <= kParts + 1; ++p) pr.at[p] = part_at[p];
+  int dev = 0, cus = 0;
+  KC_HIP(hipGetDevice(&dev));
+  KC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // one block per CU, a multiple of kParts (each part needs at least one)
+  const uint32_t grid = std::max<uint32_t>(kParts, static_cast<uint32_t>(cus > 0 ? cus : 1) / kParts * kParts);
+  hipLaunchKernelGGL(k_relabel_split, dim3(grid), dim3(kBlock * kRelabelQuarters), 0, s, a, lab, relab,
+                     static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot)), n, pieces, part_list, pr);
+  KC_LAUNCH_CHECK();
+}
+
+void count_pieces(uint64_t n, const uint64_t *off, const uint32_t *col, uint32_t smax, uint32_t wmax, uint64_t split,
+                  uint32_t *cnt, uint64_t *start, uint64_t *scratch, hipStream_t s) {
+  if (!n) return;
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(grid_for(n), 65536));
+  hipLaunchKernelGGL(k_piece_count, dim3(g), dim3(kBlock), 0, s, n, off, col, smax, wmax, split, cnt);
+  KC_LAUNCH_CHECK();
+  scan_exclusive<uint64_t>(n, ArrLoadU32{cnt}, ArrayStore<uint64_t>{start}, start + n, scratch, s);
+}
+
+void write_pieces(uint64_t n, const uint64_t *off, const uint32_t *col, uint32_t smax, uint32_t wmax, uint64_t split,
+                  const uint64_t *start, HubChunk *pieces, uint8_t *part, hipStream_t s) {
+  if (!n) return;
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(grid_for(n), 65536));
+  hipLaunchKernelGGL(k_piece_write, dim3(g), dim3(kBlock), 0, s, n, off, col, smax, wmax, split, start, pieces,
+                     part);
+  KC_LAUNCH_CHECK();
+}
+
 void scatter(const uint64_t *off, const uint32_t *relab, const Bins &b, const uint32_t *deg_inter,
              const uint32_t *hub_inter, const uint32_t *hub_pos, const uint32_t *pos_of, uint32_t *bucket,
              hipStream_t s) {
@@ -2340,12 +2539,17 @@ void sort_rows(uint64_t n, const uint64_t *off, uint32_t *col, uint64_t m2, uint
   const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock / kWave), 1u << 16));
   hipLaunchKernelGGL(k_row_ids, dim3(g), dim3(kBlock), 0, s, n, off, v0);
   KC_LAUNCH_CHECK();
-  KC_HIP(hipMemcpyAsync(k0, col, m2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  // (column, row) pairs by column, then stably by row: rows in order, each ascending
-  const int w = radix_sort_pairs(m2, bits, k0, v0, k1, v1, scratch, s);
+  // (owner key, row) pairs by key, then stably by row: rows in order, each
+  // grouped by owner part and ascending inside a group
+  const int B = bits < 8 ? 8 : bits;
+  const unsigned ge = static_cast<unsigned>(std::min<uint64_t>(grid_for(m2), 65536));
+  hipLaunchKernelGGL(k_owner_keys, dim3(ge), dim3(kBlock), 0, s, m2, col, k0, B);
+  KC_LAUNCH_CHECK();
+  const int w = radix_sort_pairs(m2, B, k0, v0, k1, v1, scratch, s);
   uint32_t *sc = w ? k1 : k0, *sr = w ? v1 : v0, *tc = w ? k0 : k1, *tr = w ? v0 : v1;
   const int w2 = radix_sort_pairs(m2, bits, sr, sc, tr, tc, scratch, s);
-  KC_HIP(hipMemcpyAsync(col, w2 ? tc : sc, m2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_owner_cols, dim3(ge), dim3(kBlock), 0, s, m2, w2 ? tc : sc, col, B);
+  KC_LAUNCH_CHECK();
 }
 
 void identity_u32(uint64_t n, uint32_t *p, hipStream_t s) {

@@ -154,6 +154,11 @@ typedef struct kc_options {
                              (DistributeHighDegreeVertices, exponential_contraction.h:935-1253,
                              with every vertex its own PE; deviates :483-503) */
   double hd_threshold;    /* -hd-thres (default 16) */
+  int32_t relabel_split;  /* relabel of the degree-ordered level 0: 1 = owner-split (its longest rows
+                             cut by the owner part of their targets, each part looked up by its own
+                             share of the workgroups), k > 1 = owner-split of the rows of more than k
+                             slots, -1 = one pass over whole rows, 0 = default (owner-split)
+                             (same results) */
 } kc_options;
 
 /* Pre-contraction (kc_options.pre_contract), each a level of the contraction

@@ -460,6 +460,39 @@ def test_degree_order_matches_input_order(stream, bfs):
             assert np.array_equal(gid, wgid), (inorder, lvl)
 
 
+def test_owner_split_relabel_matches_one_pass():
+    """The owner-split relabel of the degree-ordered level 0 (its long rows cut into pieces by the
+    owner part of their targets, each part looked up by its own blocks; the shorter rows above the
+    16-lane bin kept whole as shared pieces) gives the same levels, keys, ids and labels as the
+    one-pass relabel over whole rows, for split bounds of 64 and 1024 slots and the default.  The
+    bin thresholds change between calls on one handle, so the pieces are cut again (short pieces
+    with smax 16, rows cut every 256 slots with wmax 256), and every run equals the oracle's
+    labels."""
+    g = ca.generate("rmat", n=1 << 17, m=1 << 21, seed=7)
+    ref = oracle.canonical_labels(g.offsets, g.targets)
+    dg = ca.DeviceGraph(g)
+    try:
+        for bins in ((0, 0, 0), (4, 16, 256), (16, 128, 1024), (0, 0, 0)):
+            tmax, smax, wmax = bins
+            runs = {}
+            for split in (1, 64, 1024, -1):
+                for bfs in (True, False):
+                    lab, st = dg.find_components(seed=11, stream=ca.STREAM_HASHED, use_bfs=bfs, seq_limit=1024,
+                                                 record_trace=True, tmax=tmax, smax=smax, wmax=wmax,
+                                                 relabel_split=split)
+                    runs[split, bfs] = (lab, dg.trace())
+            print(f"bins {bins}: levels {len(runs[1, True][1])}", flush=True)
+            for split, bfs in itertools.product((1, 64, 1024), (True, False)):
+                (lab, tr), (lab1, tr1) = runs[split, bfs], runs[-1, bfs]
+                assert np.array_equal(lab, ref) and np.array_equal(lab1, ref), (bins, split, bfs)
+                assert len(tr) == len(tr1) >= 2, (bins, split, bfs)
+                for lvl, ((info, keys, gid), (info1, keys1, gid1)) in enumerate(zip(tr, tr1)):
+                    assert tuple(info) == tuple(info1), (bins, split, bfs, lvl)
+                    assert np.array_equal(keys, keys1) and np.array_equal(gid, gid1), (bins, split, bfs, lvl)
+    finally:
+        dg.close()
+
+
 def test_option_checks_and_timed_class():
     g = ca.generate("rmat", n=1 << 16, m=1 << 20, seed=4)
     ref = oracle.canonical_labels(g.offsets, g.targets)
