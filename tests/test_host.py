@@ -44,6 +44,39 @@ def test_kernel_classes_match_header():
         assert getattr(ca.Stats, f).size == count * 8
 
 
+# ctypes mirrors of the header's structs (the Python side of the C ABI)
+STRUCTS = {"kc_csr_host": "CsrHost", "kc_csr": "Csr", "kc_gen_params": "GenParams", "kc_options": "Options",
+           "kc_stats": "Stats", "kc_bin_tuning": "BinTuning", "kc_batch_host": "BatchHost",
+           "kc_batch_csr": "BatchCsr", "kc_batch_gen_params": "BatchGenParams", "kc_batch_tuning": "BatchTuning",
+           "kc_batch_stats": "BatchStats", "kc_train_params": "TrainParams", "kc_train_stats": "TrainStats"}
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """Every ctypes struct has the size of its C struct and every field its C offset, as a C program
+    compiled against include/kacomp.h prints them (a field added on one side only fails here)."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "kacomp.h"', "int main(void) {"]
+    for cname, pname in STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in getattr(ca, pname)._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            cname, what, val = line.split()
+            got[cname, what] = int(val)
+    for cname, pname in STRUCTS.items():
+        cls = getattr(ca, pname)
+        assert got[cname, "size"] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[cname, field] == getattr(cls, field).offset, (cname, field)
+
+
 def test_read_metis_simple():
     g = ca.read_metis(os.path.join(GOLDEN, "simple.graph"))
     assert g.n == 6
