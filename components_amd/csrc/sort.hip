@@ -4,7 +4,8 @@
 // (2) an exclusive scan over the digit-major histogram table gives each
 // (digit, tile) its output offset; (3) every tile ranks its items in index
 // order -- peers with the same digit inside a wave are found with 8 ballots,
-// waves are ordered through LDS counts -- and writes them out.  No global
+// waves are ordered through LDS counts -- places them in LDS in digit order
+// and writes each digit's items out as one contiguous run.  No global
 // atomics; traffic per pass is 8 B read twice and 8 B written per item.
 #include <hip/hip_runtime.h>
 
@@ -35,23 +36,53 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(uint64_t n, const uint32_
   hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Exclusive prefix of one value per thread over the block (four waves).
+__device__ __forceinline__ uint32_t block_exclusive_u32(uint32_t x, uint32_t *wave_tot) {
+  const uint32_t lane = lane_id();
+  uint32_t inc = x;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o));
+    if (lane >= static_cast<uint32_t>(o)) inc += y;
+  }
+  if (lane == kWave - 1) wave_tot[threadIdx.x / kWave] = inc;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < static_cast<int>(threadIdx.x / kWave); ++w) before += wave_tot[w];
+  __syncthreads();
+  return before + inc - x;
+}
+
+// Scatter pass, staged in LDS: the tile's items are ranked in index order --
+// peers with the same digit inside a wave found with 8 ballots, waves ordered
+// through LDS counts, rounds through running per-digit counts -- and placed
+// in LDS in digit order; then consecutive threads write consecutive items, so
+// each digit's items leave the tile as one contiguous run per array instead
+// of one scattered 4-byte store each.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(uint64_t n, const uint32_t *keys, const uint32_t *vals,
-                                                         int shift, const uint64_t *offsets, uint32_t tiles,
-                                                         uint32_t *okeys, uint32_t *ovals) {
-  __shared__ uint64_t digit_base[kDigits];
+                                                         int shift, const uint32_t *hist, const uint64_t *offsets,
+                                                         uint32_t tiles, uint32_t *okeys, uint32_t *ovals) {
+  __shared__ uint32_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
+  __shared__ uint32_t local_start[kDigits];  // first tile position of each digit
+  __shared__ uint32_t run[kDigits];          // items of each digit placed so far
   __shared__ uint32_t wave_cnt[kWaves][kDigits];
+  __shared__ uint32_t wave_tot[kWaves];
+  __shared__ uint64_t digit_base[kDigits];  // output offset of the tile's first item of each digit
   const int tid = threadIdx.x;
   const int w = tid / kWave;
   const uint32_t lane = lane_id();
-  digit_base[tid] = offsets[static_cast<uint64_t>(tid) * tiles + blockIdx.x];
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
+  const uint32_t count = static_cast<uint32_t>(n - base < static_cast<uint64_t>(kTile) ? n - base : kTile);
+  local_start[tid] = block_exclusive_u32(hist[static_cast<uint64_t>(tid) * tiles + blockIdx.x], wave_tot);
+  run[tid] = 0;
+  digit_base[tid] = offsets[static_cast<uint64_t>(tid) * tiles + blockIdx.x];
   for (int r = 0; r < kItemsPerThread; ++r) {
     for (int d = lane; d < kDigits; d += kWave) wave_cnt[w][d] = 0;
     __syncthreads();
-    const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + tid;
-    const bool valid = i < n;
-    const uint32_t k = valid ? keys[i] : 0u;
-    const uint32_t v = valid ? vals[i] : 0u;
+    const uint32_t i = static_cast<uint32_t>(r) * kBlock + tid;
+    const bool valid = i < count;
+    const uint32_t k = valid ? keys[base + i] : 0u;
+    const uint32_t v = valid ? vals[base + i] : 0u;
     const uint32_t d = (k >> shift) & (kDigits - 1);
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < kRadixBits; ++b) {
@@ -63,14 +94,21 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(uint64_t n, const uint
     if (valid && rank == 0) wave_cnt[w][d] = static_cast<uint32_t>(__popcll(peers));
     __syncthreads();
     if (valid) {
-      uint64_t off = digit_base[d] + rank;
-      for (int w2 = 0; w2 < w; ++w2) off += wave_cnt[w2][d];
-      okeys[off] = k;
-      ovals[off] = v;
+      uint32_t pos = local_start[d] + run[d] + rank;
+      for (int w2 = 0; w2 < w; ++w2) pos += wave_cnt[w2][d];
+      skey[pos] = k;
+      sval[pos] = v;
     }
     __syncthreads();
-    digit_base[tid] += wave_cnt[0][tid] + wave_cnt[1][tid] + wave_cnt[2][tid] + wave_cnt[3][tid];
-    __syncthreads();
+    run[tid] += wave_cnt[0][tid] + wave_cnt[1][tid] + wave_cnt[2][tid] + wave_cnt[3][tid];
+    __syncthreads();  // the counts are read before the next round clears them
+  }
+  for (uint32_t j = tid; j < count; j += kBlock) {
+    const uint32_t k = skey[j];
+    const uint32_t d = (k >> shift) & (kDigits - 1);
+    const uint64_t g = digit_base[d] + (j - local_start[d]);
+    okeys[g] = k;
+    ovals[g] = sval[j];
   }
 }
 
@@ -109,7 +147,8 @@ int radix_sort_pairs(uint64_t n, int bits, uint32_t *keys, uint32_t *vals, uint3
     hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, s, n, ik, shift, hist, tiles);
     KC_LAUNCH_CHECK();
     scan_exclusive_u32_to_u64(cells, hist, offs, total, rest, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, s, n, ik, iv, shift, offs, tiles, ok, ov);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, s, n, ik, iv, shift, hist, offs, tiles, ok,
+                       ov);
     KC_LAUNCH_CHECK();
     cur ^= 1;
   }
