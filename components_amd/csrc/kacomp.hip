@@ -1081,9 +1081,17 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l);
       mark_labels(n_l, rec.lab, g->cid, g->stream);
     }
+    // on the degree-ordered level 0 the clusters of the first kHotClusters
+    // positions take the ids below 65536, so the relabel keeps those
+    // positions' ids in LDS at 16 bits (twice as many as at 32)
+    const bool hot_ids = pos_l != nullptr;
     {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l);
-      cluster_ranks(n_l, g->cid, g->crank, g->sizes + 9, g->scan_scratch, g->stream);
+      if (hot_ids)
+        cluster_ranks_hot(n_l, std::min<uint64_t>(n_l, kHotClusters), rec.lab, g->cid, g->crank, g->sizes + 9,
+                          g->scan_scratch, g->stream);
+      else
+        cluster_ranks(n_l, g->cid, g->crank, g->sizes + 9, g->scan_scratch, g->stream);
     }
     {
       TimedScope ts(g->timer, KC_K_COUNT, 12 * n_l);
@@ -1097,7 +1105,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
     // the degree-ordered level 0 relabels the rows above the 16-lane bin in
     // owner pieces, which then stand in for its wave rows and hub chunks in
     // the count fold and the bucket copy
-    const bool split = pos_l && off_l == g->off0p && n_l == g->n0 && o->relabel_split >= 0;
+    const bool split = hot_ids && off_l == g->off0p && n_l == g->n0 && o->relabel_split >= 0;
     Bins rb = b;
     uint32_t *r_inter = g->hub_inter, *r_pos = g->hub_pos;
     uint64_t *r_prefix = g->hub_prefix;
@@ -1121,7 +1129,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         relabel_split(off_l, col_l, g->cid, n_l, b, g->l0_pieces, g->l0_part_list, g->l0_part_at, relab,
                       g->deg_inter, r_inter, n_l, g->stream);
       else
-        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l ? n_l : 0, g->stream);
+        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l ? n_l : 0, g->stream, hot_ids);
     }
     if (rb.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * rb.nH + 4ull * rb.nH);

@@ -141,6 +141,16 @@ void mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used, hipStream_t s)
 void cluster_ranks(uint64_t n, const uint32_t *used, uint32_t *rank, uint64_t *count, uint64_t *scratch,
                    hipStream_t s);
 void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid, hipStream_t s);
+// Positions whose cluster ids the relabel of a degree-ordered level 0 keeps in
+// LDS as 16-bit values.
+constexpr uint64_t kHotClusters = 57344;
+// cluster_ranks with the clusters of the first k positions (k <= 65536) numbered
+// first, so that their ids are below 65536 (used: mark_labels's flags, changed
+// in place; scratch: 8 words + scan scratch).  Same count and the same set of
+// ids as cluster_ranks: only the order of the ids differs, and nothing after
+// the relabel depends on it.
+void cluster_ranks_hot(uint64_t n, uint64_t k, const uint32_t *lab, uint32_t *used, uint32_t *rank, uint64_t *count,
+                       uint64_t *scratch, hipStream_t s);
 // (label, vertex) of every vertex with a cluster edge; *count (device) = N.
 void select_inter(uint64_t n, const uint64_t *key, const uint32_t *deg_inter, uint32_t *sk, uint32_t *sv,
                   uint64_t *count, uint64_t *scratch, hipStream_t s);
@@ -157,13 +167,16 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
 // LDS filter of repeats), packed at relab[off[v] ...] (hub chunks: at the
 // chunk start); deg_inter[v] (hub chunks: hub_inter) = their number.
 // One persistent launch; lab[0 .. min(n_hot, 28672)) is served from LDS (for
-// degree-ordered levels, whose first vertices take most endpoints).
+// degree-ordered levels, whose first vertices take most endpoints); with
+// hot16, lab[0 .. min(n_hot, kHotClusters)) as 16-bit values, which requires
+// those positions' cluster ids to fit 16 bits (cluster_ranks_hot).
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
-             uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s);
+             uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s, bool hot16 = false);
 // Owner-split relabel of a degree-ordered level 0 whose rows are grouped by
 // owner (sort_rows): the one-lane and 16-lane bins of b as relabel does, and
 // the pieces of the longer rows (write_pieces): the shared ones by every
-// block with the first n_hot labels in LDS, the others by the blocks of their
+// block with the first n_hot labels in LDS as 16-bit values (their ids must
+// fit, cluster_ranks_hot), the others by the blocks of their
 // owner part with that part's labels in LDS (part_at: kParts + 2 bounds into
 // part_list); piece counts go to piece_inter.  Same cluster edges per row as relabel, split over the row's
 // pieces (hub_fold with the pieces as chunks gives the rows' counts).

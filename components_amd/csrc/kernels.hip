@@ -418,12 +418,25 @@ struct LabHot {
   __device__ __forceinline__ uint32_t row(uint32_t v) const { return (*this)(v); }
 };
 
+// The same with 16-bit entries: the first n_hot (up to 2 kRelabelHot)
+// positions of a degree-ordered level 0, whose clusters were numbered first
+// (cluster_ranks_hot), so that their ids fit 16 bits.
+struct LabHot16 {
+  const uint32_t *__restrict__ lab;
+  const uint16_t *hot;
+  uint32_t n_hot;
+  __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return c < n_hot ? hot[c] : lab[c]; }
+  __device__ __forceinline__ uint32_t row(uint32_t v) const { return (*this)(v); }
+};
+
 // Label lookups of one owner part p (the vertices t with owner_of(t) == p, a
 // 128-byte line of labels in every kParts lines): the part's share of the
 // first kSplitHot vertices from LDS, packed line by line, the others from the
 // array.  Only targets of part p are looked up through it; the row's own
 // label (any part) comes from the array.
 constexpr uint32_t kRelabelHot = 28672;
+constexpr uint32_t kRelabelHot16 = 2 * kRelabelHot;  // 16-bit entries in the same LDS
+static_assert(kRelabelHot16 <= kHotClusters, "16-bit LDS labels need the hot positions' clusters numbered first");
 constexpr uint32_t kSplitHot = kParts * kRelabelHot;
 __device__ __forceinline__ uint32_t part_slot(uint32_t t) { return ((t >> 8) << 5) | (t & 31); }
 struct LabPart {
@@ -527,16 +540,27 @@ __device__ __forceinline__ void relabel_block(const SweepArgs &a, const Lab &lab
 // barrier between them).
 constexpr int kRelabelQuarters = 4;
 
+// H16: 16-bit entries for twice the positions (their cluster ids fit 16 bits).
+template <bool H16>
 __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(SweepArgs a, const uint32_t *lab,
                                                                         uint32_t *relab, uint32_t n_hot) {
   __shared__ uint32_t hot[kRelabelHot];
   __shared__ uint32_t filt[kRelabelQuarters][kFilterWords];
-  for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) hot[i] = lab[i];
-  __syncthreads();
   const int q = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
-  const LabHot lk{lab, hot, n_hot};
-  for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.blocks; blk += gridDim.x * kRelabelQuarters)
-    relabel_block(a, lk, relab, blk, tid, filt[q]);
+  if (H16) {
+    uint16_t *h16 = reinterpret_cast<uint16_t *>(hot);
+    for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) h16[i] = static_cast<uint16_t>(lab[i]);
+    __syncthreads();
+    const LabHot16 lk{lab, h16, n_hot};
+    for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.blocks; blk += gridDim.x * kRelabelQuarters)
+      relabel_block(a, lk, relab, blk, tid, filt[q]);
+  } else {
+    for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) hot[i] = lab[i];
+    __syncthreads();
+    const LabHot lk{lab, hot, n_hot};
+    for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.blocks; blk += gridDim.x * kRelabelQuarters)
+      relabel_block(a, lk, relab, blk, tid, filt[q]);
+  }
 }
 
 // Owner-split relabel of a degree-ordered level 0.  The rows above the 16-lane
@@ -549,7 +573,8 @@ __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_hot(Sweep
 // block first runs its share of the one-lane and 16-lane bins (rows kept
 // whole) with the first n_hot labels in LDS, as k_relabel_hot, then reloads
 // LDS with its part's share and runs its part's pieces, one wave per piece.
-// Results do not depend on which chiplet runs a block.
+// Results do not depend on which chiplet runs a block.  The first n_hot
+// positions' cluster ids must fit 16 bits (cluster_ranks_hot).
 struct PartRange {
   // pieces of part p: part_list[at[p] .. at[p + 1]); p = kParts: the shared
   // pieces (whole rows up to the split bound), run by every block
@@ -563,13 +588,14 @@ __global__ __launch_bounds__(kBlock * kRelabelQuarters) void k_relabel_split(Swe
   __shared__ uint32_t hot[kRelabelHot];
   __shared__ uint32_t filt[kRelabelQuarters][kFilterWords];
   const int q = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
-  for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) hot[i] = lab[i];
+  uint16_t *h16 = reinterpret_cast<uint16_t *>(hot);
+  for (uint32_t i = threadIdx.x; i < n_hot; i += kBlock * kRelabelQuarters) h16[i] = static_cast<uint16_t>(lab[i]);
   __syncthreads();
   constexpr uint32_t kWavesPerBlock = kBlock * kRelabelQuarters / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   uint32_t *wfilt = filt[q] + (tid / kWave) * 512;
   {
-    const LabHot lk{lab, hot, n_hot};
+    const LabHot16 lk{lab, h16, n_hot};
     for (uint32_t blk = blockIdx.x * kRelabelQuarters + q; blk < a.b.bW; blk += gridDim.x * kRelabelQuarters)
       relabel_block(a, lk, relab, blk, tid, filt[q]);
     const uint32_t s0 = pr.at[kParts], shared = pr.at[kParts + 1] - s0;
@@ -2158,9 +2184,51 @@ void cluster_ranks(uint64_t n, const uint32_t *used, uint32_t *rank, uint64_t *c
   scan_exclusive<uint64_t>(n, ArrLoadU32{used}, RankStore{rank}, count, scratch, s);
 }
 
+// Cluster numbering with the clusters of the first k positions first: used[a]
+// = 3 for their labels (after mark_labels set 1 for every used label); a
+// two-counter scan ranks those labels 0 .. H-1 and the others from H on.
+__global__ void k_mark_hot(uint64_t k, const uint32_t *lab, uint32_t *used) {
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v < k) used[lab[v]] = 3;
+}
+struct HotRankLoad {
+  const uint32_t *used;
+  __device__ Pair64 operator()(uint64_t i) const {
+    const uint32_t u = used[i];
+    return {u == 3 ? 1ull : 0ull, u == 1 ? 1ull : 0ull};
+  }
+};
+struct HotRankStore {
+  const uint32_t *used;
+  uint32_t *rank;
+  __device__ void operator()(uint64_t i, Pair64 prefix, Pair64 v) const {
+    if (v.a) rank[i] = static_cast<uint32_t>(prefix.a);
+    else if (v.b) rank[i] = static_cast<uint32_t>(prefix.b);  // H added by k_rank_fix
+  }
+};
+__global__ void k_rank_fix(uint64_t n, const uint32_t *used, uint32_t *rank, const Pair64 *total, uint64_t *count) {
+  const uint64_t a = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (a < n && used[a] == 1) rank[a] += static_cast<uint32_t>(total->a);
+  if (a == 0) *count = total->a + total->b;
+}
+
 __global__ void k_cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid) {
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (v < n) cid[v] = rank[lab[v]];
+}
+
+void cluster_ranks_hot(uint64_t n, uint64_t k, const uint32_t *lab, uint32_t *used, uint32_t *rank, uint64_t *count,
+                       uint64_t *scratch, hipStream_t s) {
+  if (!n) return;
+  if (k > n) k = n;
+  if (k) {
+    hipLaunchKernelGGL(k_mark_hot, dim3(grid_for(k)), dim3(kBlock), 0, s, k, lab, used);
+    KC_LAUNCH_CHECK();
+  }
+  Pair64 *total = reinterpret_cast<Pair64 *>(scratch);
+  scan_exclusive<Pair64>(n, HotRankLoad{used}, HotRankStore{used, rank}, total, scratch + 4, s);
+  hipLaunchKernelGGL(k_rank_fix, dim3(grid_for(n)), dim3(kBlock), 0, s, n, used, rank, total, count);
+  KC_LAUNCH_CHECK();
 }
 
 void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid, hipStream_t s) {
@@ -2205,7 +2273,7 @@ void scan_exclusive_u32_to_u64(uint64_t n, const uint32_t *in, uint64_t *out, ui
 }
 
 void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, const Bins &b, uint32_t *relab,
-             uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s) {
+             uint32_t *deg_inter, uint32_t *hub_inter, uint64_t n_hot, hipStream_t s, bool hot16) {
   if (b.blocks == 0) return;
   SweepArgs a{off, col, nullptr, nullptr, 0, b, deg_inter, hub_inter, nullptr, nullptr};
   int dev = 0, cus = 0;
@@ -2213,8 +2281,12 @@ void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, cons
   KC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const uint32_t blocks = static_cast<uint32_t>(ceil_div(b.blocks, kRelabelQuarters));
   const uint32_t grid = std::min<uint32_t>(blocks, static_cast<uint32_t>(cus > 0 ? cus : 1));
-  hipLaunchKernelGGL(k_relabel_hot, dim3(grid), dim3(kBlock * kRelabelQuarters), 0, s, a, lab, relab,
-                     static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot)));
+  if (hot16)
+    hipLaunchKernelGGL(k_relabel_hot<true>, dim3(grid), dim3(kBlock * kRelabelQuarters), 0, s, a, lab, relab,
+                       static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot16)));
+  else
+    hipLaunchKernelGGL(k_relabel_hot<false>, dim3(grid), dim3(kBlock * kRelabelQuarters), 0, s, a, lab, relab,
+                       static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot)));
   KC_LAUNCH_CHECK();
 }
 
@@ -2231,7 +2303,7 @@ void relabel_split(const uint64_t *off, const uint32_t *col, const uint32_t *lab
   // one block per CU, a multiple of kParts (each part needs at least one)
   const uint32_t grid = std::max<uint32_t>(kParts, static_cast<uint32_t>(cus > 0 ? cus : 1) / kParts * kParts);
   hipLaunchKernelGGL(k_relabel_split, dim3(grid), dim3(kBlock * kRelabelQuarters), 0, s, a, lab, relab,
-                     static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot)), n, pieces, part_list, pr);
+                     static_cast<uint32_t>(std::min<uint64_t>(n_hot, kRelabelHot16)), n, pieces, part_list, pr);
   KC_LAUNCH_CHECK();
 }
 
