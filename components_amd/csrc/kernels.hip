@@ -404,9 +404,11 @@ __global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
 // of the chunk), in ballot order; the count goes to deg_inter (hub chunks:
 // hub_inter).  The slots after the count are left as they were.
 // Groups of 16 or 64 lanes also drop repeats of a label inside their row
-// through a small direct-mapped LDS filter (best effort: a label is dropped
-// only when the filter holds it, so every kept set still covers the row's
-// distinct labels; the bucket dedup removes the rest).
+// through a small direct-mapped LDS filter whose slots are claimed by
+// compare-and-swap (best effort: a label is dropped only when the filter
+// already holds it or another lane's swap put it there in the same step, so
+// every kept set still covers the row's distinct labels; the bucket dedup
+// removes the rest).
 // Label lookups: the first n_hot entries (the highest-degree vertices of a
 // degree-ordered level, which receive most edge endpoints) from an LDS copy,
 // the others from the array.
@@ -481,11 +483,12 @@ __device__ __forceinline__ void relabel_group(const SweepArgs &a, const Lab &lab
     for (int j = 0; j < kRelabelGather; ++j) {
       bool inter = l[j] != lv;
       if (W > 1 && inter) {
+        // the slot is claimed by compare-and-swap, so of the group's lanes
+        // that meet one label in this step only the one whose swap lands
+        // keeps it (a lane whose swap meets another label keeps its own)
         const uint32_t slot = hash32(l[j]) & fmask;
-        if (filt[slot] == l[j])
-          inter = false;
-        else
-          filt[slot] = l[j];
+        const uint32_t cur = filt[slot];
+        if (cur == l[j] || atomicCAS(&filt[slot], cur, l[j]) == l[j]) inter = false;
       }
       const uint64_t m = __ballot(inter) & gmask;
       if (inter) relab[e0 + base + __popcll(m & lanes_below())] = l[j];
