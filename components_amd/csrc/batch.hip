@@ -438,6 +438,16 @@ __device__ __forceinline__ uint32_t union_part(const uint64_t *start, uint32_t p
   return lo;
 }
 
+// The same for the lanes of a wave holding consecutive x from `first` (the
+// wave's first lane) on: one search for `first`, then each lane steps forward
+// over the parts it passes (the large graphs exceed the split size, so a wave
+// spans one or two of them).
+__device__ __forceinline__ uint32_t union_part_wave(const uint64_t *start, uint32_t parts, uint64_t x, uint64_t first) {
+  uint32_t k = union_part(start, parts, first);
+  while (k + 1 < parts && start[k + 1] <= x) ++k;
+  return k;
+}
+
 __global__ void k_union_rows(uint64_t n_union, uint32_t parts, const uint64_t *uoff, const uint64_t *ueoff,
                              const uint32_t *big, const uint64_t *vtx_off, const uint64_t *edge_off,
                              const uint64_t *off, uint64_t *uoff_rows) {
@@ -447,7 +457,7 @@ __global__ void k_union_rows(uint64_t n_union, uint32_t parts, const uint64_t *u
       uoff_rows[u] = ueoff[parts];
       continue;
     }
-    const uint32_t k = union_part(uoff, parts, u);
+    const uint32_t k = union_part_wave(uoff, parts, u, u - lane_id());
     const uint32_t g = big[k];
     uoff_rows[u] = off[vtx_off[g] + (u - uoff[k])] - edge_off[g] + ueoff[k];
   }
@@ -457,7 +467,7 @@ __global__ void k_union_cols(uint64_t m2_union, uint32_t parts, const uint64_t *
                              const uint32_t *big, const uint64_t *edge_off, const uint32_t *col, uint32_t *ucol) {
   for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < m2_union;
        j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t k = union_part(ueoff, parts, j);
+    const uint32_t k = union_part_wave(ueoff, parts, j, j - lane_id());
     ucol[j] = col[edge_off[big[k]] + (j - ueoff[k])] + static_cast<uint32_t>(uoff[k]);
   }
 }
@@ -487,7 +497,7 @@ __global__ void k_union_labels(uint64_t n_union, uint32_t parts, const uint64_t 
   uint32_t g = 0;
   bool root = false;
   if (act) {
-    const uint32_t k = union_part(uoff, parts, u);
+    const uint32_t k = union_part_wave(uoff, parts, u, u - lane_id());
     g = big[k];
     const uint32_t l = ulab[u];
     labels[vtx_off[g] + (u - uoff[k])] = l - static_cast<uint32_t>(uoff[k]);
