@@ -293,6 +293,9 @@ struct kc_graph {
   const uint64_t *l0_score_off = nullptr;
   uint64_t learned_cap = 0;  // vertices the learned buffers hold
   BinCfg tuned{0, 0, 0};     // thresholds kept by kc_graph_autotune (tmax 0: none)
+  // the degree-ordered level 0's first kHotClusters positions hold at least
+  // kHotShareFactor times their proportional share of its edge slots
+  bool l0_hot_skewed = false;
   // Pieces of the degree-ordered level 0's rows above the 16-lane bin for the
   // owner-split relabel (one per owner group of a row, cut every wmax slots),
   // their counts, offsets in their row's run and prefix, the piece indices of
@@ -366,6 +369,11 @@ constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep
 // this are cut by the owner part of their targets (shorter rows above the
 // 16-lane bin stay whole)
 constexpr uint64_t kSplitRowsAbove = 4096;
+// Hot-first cluster ids (and the owner-split relabel, which needs them) when the
+// first kHotClusters positions of the degree-ordered level 0 hold at least
+// this many times their proportional share of the edge slots (R-MAT 24: 68
+// times; G(n,m), 3D grids and 2D geometric graphs: under 2)
+constexpr double kHotShareFactor = 4.0;
 constexpr int kTailShift = 5;  // -bfs: pull sweeps finish a level once at most n/32 rows are pending
 constexpr uint64_t kPullRoundsMin = 1u << 16;  // -bfs: levels at least this large choose round directions
 constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulled
@@ -1063,9 +1071,13 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       KC_HIP(hipMemcpyAsync(rec.keys_copy, g->key, n_l * 8, hipMemcpyDeviceToDevice, g->stream));
     }
     fill_u32(g->small + 4, 1, 0, g->stream);
+    // dense cluster ids in label order: the buckets below hold them, so the
+    // dedup works over the C clusters instead of the n_l labels; the labels'
+    // pass marks the used ones
+    fill_u32(g->cid, n_l, 0, g->stream);  // the used-label flags, then the ids
     {
-      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
-      labels_radius(n_l, g->key, pos_l, rec.lab, g->small + 4, !local && !pre, g->stream);
+      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l + 4 * n_l + 4 * n_l + (local ? 0 : 8 * n_l));
+      labels_radius(n_l, g->key, pos_l, rec.lab, g->small + 4, !local && !pre, g->stream, g->cid);
     }
     if (g->collect && !pre) {
       // trace of the plain method: features and the log2 size of each vertex's cluster
@@ -1074,17 +1086,11 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       cluster_targets(n_l, rec.lab, g->csize, g->target, g->stream);
       mlp_append(g->collect, n_l, g->feat, g->target, g->collect_stride, g->stream);
     }
-    // dense cluster ids in label order: the buckets below hold them, so the
-    // dedup works over the C clusters instead of the n_l labels
-    fill_u32(g->cid, n_l, 0, g->stream);  // the used-label flags, then the ids
-    {
-      TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l);
-      mark_labels(n_l, rec.lab, g->cid, g->stream);
-    }
-    // on the degree-ordered level 0 the clusters of the first kHotClusters
-    // positions take the ids below 65536, so the relabel keeps those
-    // positions' ids in LDS at 16 bits (twice as many as at 32)
-    const bool hot_ids = pos_l != nullptr;
+    // on a degree-ordered level 0 whose first kHotClusters positions hold a
+    // large share of the edge slots, the clusters of those positions take the
+    // ids below 65536, so the relabel keeps those positions' ids in LDS at 16
+    // bits (twice as many as at 32)
+    const bool hot_ids = pos_l != nullptr && g->l0_hot_skewed;
     {
       TimedScope ts(g->timer, KC_K_COUNT, 8 * n_l);
       if (hot_ids)
@@ -1486,6 +1492,12 @@ void init_work(kc_graph *g) {
       sort_rows(n0, g->off0p, g->col0p, m2, k0.as<uint32_t>(), v0.as<uint32_t>(), k1.as<uint32_t>(), v1.as<uint32_t>(),
                 scratch.as<uint64_t>(), g->stream);
       sync(g);
+    }
+    {
+      const uint64_t k = std::min<uint64_t>(n0, kHotClusters);
+      uint64_t slots_k = 0;
+      KC_HIP(hipMemcpy(&slots_k, g->off0p + k, sizeof(slots_k), hipMemcpyDeviceToHost));
+      g->l0_hot_skewed = m2 > 0 && static_cast<double>(slots_k) * n0 >= kHotShareFactor * static_cast<double>(k) * m2;
     }
   }
   KC_HIP(hipDeviceSynchronize());

@@ -130,14 +130,14 @@ void jump(uint64_t n, uint64_t *key, int *flag, hipStream_t s);
 void block_interface(const uint64_t *off, const uint32_t *col, uint64_t n, uint64_t bsize, uint64_t *key,
                      hipStream_t s);
 
-// lab[v] = low half of key[v]; max cluster radius (atomicMax into *radius).
+// lab[v] = low half of key[v]; max cluster radius (atomicMax into *radius);
+// used (may be null, pre-zeroed): used[lab[v]] = 1, the used-label flags of
+// cluster_ranks.
 void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
-                   bool with_radius,
-                   hipStream_t s);
-// Dense cluster ids in label order: used[lab[v]] = 1 (used pre-zeroed);
+                   bool with_radius, hipStream_t s, uint32_t *used = nullptr);
+// Dense cluster ids in label order from the used-label flags (labels_radius):
 // rank[a] = number of used labels below a, written for the used ones, and
 // *count (device) = number of clusters C; then cid[v] = rank[lab[v]] in [0, C).
-void mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used, hipStream_t s);
 void cluster_ranks(uint64_t n, const uint32_t *used, uint32_t *rank, uint64_t *count, uint64_t *scratch,
                    hipStream_t s);
 void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid, hipStream_t s);
@@ -145,7 +145,7 @@ void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t
 // LDS as 16-bit values.
 constexpr uint64_t kHotClusters = 57344;
 // cluster_ranks with the clusters of the first k positions (k <= 65536) numbered
-// first, so that their ids are below 65536 (used: mark_labels's flags, changed
+// first, so that their ids are below 65536 (used: labels_radius's flags, changed
 // in place; scratch: 8 words + scan scratch).  Same count and the same set of
 // ids as cluster_ranks: only the order of the ids differs, and nothing after
 // the relabel depends on it.

@@ -772,7 +772,8 @@ __global__ __launch_bounds__(kBlock) void k_block_interface(const uint64_t *off,
 // Grid-stride, with one atomic per workgroup: atomics on one address are
 // serialised by the device, so one per wave cost more than the pass itself.
 __global__ __launch_bounds__(kBlock) void k_labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos,
-                                                         uint32_t *lab, uint32_t *radius, int with_radius) {
+                                                         uint32_t *lab, uint32_t *used, uint32_t *radius,
+                                                         int with_radius) {
   __shared__ uint32_t wave_max[kBlock / kWave];
   uint32_t r = 0;
   for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
@@ -780,6 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_labels_radius(uint64_t n, const uint
     const uint64_t k = key[v];
     const uint32_t l = static_cast<uint32_t>(k);
     lab[v] = l;
+    if (used) used[l] = 1;
     if (with_radius) r = max(r, static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[pos ? pos[l] : l] >> 32));
   }
   if (with_radius) {
@@ -2155,25 +2157,16 @@ void huge_sizes(uint32_t n_huge, const uint32_t *huge, const uint64_t *seg_off, 
 }
 
 void labels_radius(uint64_t n, const uint64_t *key, const uint32_t *pos, uint32_t *lab, uint32_t *radius,
-                   bool with_radius, hipStream_t s) {
+                   bool with_radius, hipStream_t s, uint32_t *used) {
   if (!n) return;
   const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock), 2048));
-  hipLaunchKernelGGL(k_labels_radius, dim3(g), dim3(kBlock), 0, s, n, key, pos, lab, radius, with_radius ? 1 : 0);
+  hipLaunchKernelGGL(k_labels_radius, dim3(g), dim3(kBlock), 0, s, n, key, pos, lab, used, radius,
+                     with_radius ? 1 : 0);
   KC_LAUNCH_CHECK();
 }
 
 uint64_t scan_scratch_words(uint64_t n) { return scan_words_for(n, 2) + 64; }
 
-__global__ void k_mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used) {
-  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (v < n) used[lab[v]] = 1;
-}
-
-void mark_labels(uint64_t n, const uint32_t *lab, uint32_t *used, hipStream_t s) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_mark_labels, dim3(grid_for(n)), dim3(kBlock), 0, s, n, lab, used);
-  KC_LAUNCH_CHECK();
-}
 
 struct RankStore {
   uint32_t *rank;
@@ -2188,7 +2181,7 @@ void cluster_ranks(uint64_t n, const uint32_t *used, uint32_t *rank, uint64_t *c
 }
 
 // Cluster numbering with the clusters of the first k positions first: used[a]
-// = 3 for their labels (after mark_labels set 1 for every used label); a
+// = 3 for their labels (after labels_radius set 1 for every used label); a
 // two-counter scan ranks those labels 0 .. H-1 and the others from H on.
 __global__ void k_mark_hot(uint64_t k, const uint32_t *lab, uint32_t *used) {
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
