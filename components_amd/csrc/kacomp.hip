@@ -371,8 +371,9 @@ constexpr uint64_t kOrderMinVertices = 1u << 16;  // smaller level-0 graphs keep
 constexpr uint64_t kSplitRowsAbove = 4096;
 // Hot-first cluster ids (and the owner-split relabel, which needs them) when the
 // first kHotClusters positions of the degree-ordered level 0 hold at least
-// this many times their proportional share of the edge slots (R-MAT 24: 68
-// times; G(n,m), 3D grids and 2D geometric graphs: under 2)
+// this many times their proportional share of the edge slots (R-MAT 24: 70
+// times, the hyperbolic graph of 2^24 vertices: 23; G(n,m) 2^20, the 3D grid
+// 256^3 and the 2D geometric graph of 2^24 vertices: 1.8 to 2.6)
 constexpr double kHotShareFactor = 4.0;
 constexpr int kTailShift = 5;  // -bfs: pull sweeps finish a level once at most n/32 rows are pending
 constexpr uint64_t kPullRoundsMin = 1u << 16;  // -bfs: levels at least this large choose round directions
