@@ -232,6 +232,12 @@ struct kc_graph {
   uint32_t *l0_lists = nullptr;
   HubChunk *l0_hubs = nullptr;
   Bins l0_bins;
+  // the same bins with every list in level-index order, for round 0 of the
+  // degree-ordered level 0 (built when first used, dropped with l0_bins)
+  uint32_t *l0i_lists = nullptr;
+  HubChunk *l0i_hubs = nullptr;
+  Bins l0i_bins;
+  bool l0i_valid = false;
   BinCfg l0_cfg{0, 0, 0};
   const uint64_t *l0_off = nullptr;
   uint64_t hubs_cap = 0;
@@ -428,6 +434,7 @@ Bins make_bins(kc_graph *g, uint64_t n, const uint64_t *off, const BinCfg &cfg, 
     g->l0_lists = g->walloc<uint32_t>(3 * g->cap_n);
     g->l0_hubs = g->walloc<HubChunk>(g->hubs_cap);
   }
+  if (level0) g->l0i_valid = false;
   // three sections of cap_n entries each: a level holds at most cap_n vertices
   // (a replicated level 0 can hold more than the input's n)
   if (n > g->cap_n) fail(KC_ERR_INVALID, "internal: level of %llu vertices above the work set's %llu",
@@ -637,9 +644,12 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
 // Time-stepped schedule until no vertex lies above the current round, then
 // one counting pass for the cluster-edge counts.  Returns the rounds that did
 // work (the reported "sweeps" of this schedule).
+// Level 0's cached bins in level-index order (see l0i_bins).
+Bins round0_bins(kc_graph *g, const Bins &b);
+
 uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                             uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base,
-                            int tail_shift, bool choose_direction) {
+                            int tail_shift, bool choose_direction, bool index_round0) {
   uint64_t done = 0;
   uint32_t t = 0;
   bool deciding = choose_direction;
@@ -716,7 +726,13 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       // everywhere, every row scanned against flags of the distance-0
       // vertices, was measured slower at R-MAT 24: 2.36 against 1.35 ms.)
       const uint64_t hub_pull_deg = t + i == 0 ? std::max<uint64_t>(kHubPullMin, m2 / 512) : 0;
-      push_round(off, col, key, kOneHop, b, t + i, &g->flags[f], prev, hub_pull_deg, g->stream);
+      // Round 0 of the degree-ordered level 0 walks each bin in level-index
+      // order: a target ends at the smallest index among its distance-0
+      // neighbours, which then tends to land first, so that the later
+      // pushes to it read a settled key and skip their atomic.  (The
+      // result does not depend on the order.)
+      const Bins &rb = t + i == 0 && index_round0 ? round0_bins(g, b) : b;
+      push_round(off, col, key, kOneHop, rb, t + i, &g->flags[f], prev, hub_pull_deg, g->stream);
       g->timer.stop(slots[i], 0);
       if (first_slot < 0) first_slot = slots[i];
     }
@@ -1032,7 +1048,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
                                                    m2_l >= kDirectionMinDegree * n_l);
         sweeps = rounds_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, b, cfg, flag_base,
                                     o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps,
-                                    choose);
+                                    choose, pos_l != nullptr && off_l == g->off0p && g->l0_hot_skewed);
       } else {
         sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
       }
@@ -1390,6 +1406,33 @@ void alloc_work(kc_graph *g, uint64_t cap_n, uint64_t cap_m2) {
   g->l0_lists = nullptr;
   g->l0_hubs = nullptr;
   g->l0_off = nullptr;
+  g->l0i_lists = nullptr;
+  g->l0i_hubs = nullptr;
+  g->l0i_valid = false;
+}
+
+Bins round0_bins(kc_graph *g, const Bins &b) {
+  if (g->l0i_valid) return g->l0i_bins;
+  if (!g->l0i_lists) {
+    g->l0i_lists = g->walloc<uint32_t>(3 * g->cap_n);
+    g->l0i_hubs = g->walloc<HubChunk>(g->hubs_cap);
+  }
+  Bins out = b;
+  out.listT = g->l0i_lists;
+  out.listS = g->l0i_lists + g->cap_n;
+  out.listW = g->l0i_lists + 2 * g->cap_n;
+  out.hubs = g->l0i_hubs;
+  const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(b.nT, b.nS), std::max<uint64_t>(b.nW, b.nH));
+  {
+    DevTemp k0(c * 4 + 64), v0(c * 4 + 64), k1(c * 4 + 64), v1(c * 4 + 64);
+    DevTemp scratch(radix_sort_scratch_words(c) * 8 + 64);
+    order_bins_by_index(b, g->idx0, g->n0, out, k0.as<uint32_t>(), v0.as<uint32_t>(), k1.as<uint32_t>(),
+                        v1.as<uint32_t>(), scratch.as<uint64_t>(), g->stream);
+    sync(g);
+  }
+  g->l0i_bins = out;
+  g->l0i_valid = true;
+  return out;
 }
 
 // The pieces of the degree-ordered level 0 for the bins cfg, the rows above

@@ -183,6 +183,12 @@ void relabel(const uint64_t *off, const uint32_t *col, const uint32_t *lab, cons
 void relabel_split(const uint64_t *off, const uint32_t *col, const uint32_t *lab, uint64_t n, const Bins &b,
                    const HubChunk *pieces, const uint32_t *part_list, const uint32_t *part_at, uint32_t *relab,
                    uint32_t *deg_inter, uint32_t *piece_inter, uint64_t n_hot, hipStream_t s);
+// out = b with every list (and the hub chunks, by their row) in increasing
+// idx[position] order (stable; out's lists and chunks are the caller's
+// buffers, its counts and block offsets b's).  k0, v0, k1, v1: max(nT, nS,
+// nW, nH) words each; scratch: radix_sort_scratch_words of that many.
+void order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &out, uint32_t *k0, uint32_t *v0,
+                         uint32_t *k1, uint32_t *v1, uint64_t *scratch, hipStream_t s);
 // Pieces of the rows of more than smax slots of an owner-grouped CSR: cnt[v]
 // = the row's pieces (rows of at most `split` slots: the whole row, cut every
 // wmax slots, shared by all blocks; longer rows: one per owner group they

@@ -2303,6 +2303,53 @@ void relabel_split(const uint64_t *off, const uint32_t *col, const uint32_t *lab
   KC_LAUNCH_CHECK();
 }
 
+// Bin lists reordered by level index (the key's low half: idx of the storage
+// position), for round 0 of a degree-ordered level.
+__global__ void k_list_keys(uint32_t c, const uint32_t *list, const uint32_t *idx, uint32_t *keys, uint32_t *vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c) return;
+  const uint32_t p = list[i];
+  keys[i] = idx[p];
+  vals[i] = p;
+}
+__global__ void k_hub_keys(uint32_t c, const HubChunk *hubs, const uint32_t *idx, uint32_t *keys, uint32_t *vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c) return;
+  keys[i] = idx[hubs[i].v];
+  vals[i] = i;
+}
+__global__ void k_hub_permute(uint32_t c, const HubChunk *hubs, const uint32_t *order, HubChunk *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c) out[i] = hubs[order[i]];
+}
+
+void order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &out, uint32_t *k0, uint32_t *v0,
+                         uint32_t *k1, uint32_t *v1, uint64_t *scratch, hipStream_t s) {
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < n) ++bits;
+  const uint32_t *in[3] = {b.listT, b.listS, b.listW};
+  uint32_t *dst[3] = {out.listT, out.listS, out.listW};
+  const uint32_t cnt[3] = {b.nT, b.nS, b.nW};
+  for (int l = 0; l < 3; ++l) {
+    if (!cnt[l]) continue;
+    hipLaunchKernelGGL(k_list_keys, dim3(static_cast<unsigned>(grid_for(cnt[l]))), dim3(kBlock), 0, s, cnt[l], in[l],
+                       idx, k0, v0);
+    KC_LAUNCH_CHECK();
+    const int w = radix_sort_pairs(cnt[l], bits, k0, v0, k1, v1, scratch, s);
+    KC_HIP(hipMemcpyAsync(dst[l], w ? v1 : v0, 4ull * cnt[l], hipMemcpyDeviceToDevice, s));
+  }
+  if (b.nH) {
+    // chunks of one hub share their key, so the stable sort keeps them in order
+    hipLaunchKernelGGL(k_hub_keys, dim3(static_cast<unsigned>(grid_for(b.nH))), dim3(kBlock), 0, s, b.nH, b.hubs, idx,
+                       k0, v0);
+    KC_LAUNCH_CHECK();
+    const int w = radix_sort_pairs(b.nH, bits, k0, v0, k1, v1, scratch, s);
+    hipLaunchKernelGGL(k_hub_permute, dim3(static_cast<unsigned>(grid_for(b.nH))), dim3(kBlock), 0, s, b.nH, b.hubs,
+                       w ? v1 : v0, out.hubs);
+    KC_LAUNCH_CHECK();
+  }
+}
+
 void count_pieces(uint64_t n, const uint64_t *off, const uint32_t *col, uint32_t smax, uint32_t wmax, uint64_t split,
                   uint32_t *cnt, uint64_t *start, uint64_t *scratch, hipStream_t s) {
   if (!n) return;
