@@ -238,6 +238,7 @@ struct kc_graph {
   HubChunk *l0i_hubs = nullptr;
   Bins l0i_bins;
   bool l0i_valid = false;
+  bool l0i_hubs_first = false;  // the hub chunks' rows carry small indices: their section goes first
   BinCfg l0_cfg{0, 0, 0};
   const uint64_t *l0_off = nullptr;
   uint64_t hubs_cap = 0;
@@ -732,7 +733,8 @@ uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t
       // pushes to it read a settled key and skip their atomic.  (The
       // result does not depend on the order.)
       const Bins &rb = t + i == 0 && index_round0 ? round0_bins(g, b) : b;
-      push_round(off, col, key, kOneHop, rb, t + i, &g->flags[f], prev, hub_pull_deg, g->stream);
+      push_round(off, col, key, kOneHop, rb, t + i, &g->flags[f], prev, hub_pull_deg, g->stream,
+                 t + i == 0 && index_round0 && g->l0i_hubs_first);
       g->timer.stop(slots[i], 0);
       if (first_slot < 0) first_slot = slots[i];
     }
@@ -1426,9 +1428,12 @@ Bins round0_bins(kc_graph *g, const Bins &b) {
   {
     DevTemp k0(c * 4 + 64), v0(c * 4 + 64), k1(c * 4 + 64), v1(c * 4 + 64);
     DevTemp scratch(radix_sort_scratch_words(c) * 8 + 64);
-    order_bins_by_index(b, g->idx0, g->n0, out, k0.as<uint32_t>(), v0.as<uint32_t>(), k1.as<uint32_t>(),
-                        v1.as<uint32_t>(), scratch.as<uint64_t>(), g->stream);
+    const uint32_t median = order_bins_by_index(b, g->idx0, g->n0, out, k0.as<uint32_t>(), v0.as<uint32_t>(),
+                                                k1.as<uint32_t>(), v1.as<uint32_t>(), scratch.as<uint64_t>(), g->stream);
     sync(g);
+    // a target ends at its smallest distance-0 label: when the hubs' labels
+    // are small (R-MAT's hubs have the lowest ids), their pushes go first
+    g->l0i_hubs_first = median != kNone && static_cast<uint64_t>(median) < g->n0 / 4;
   }
   g->l0i_bins = out;
   g->l0i_valid = true;

@@ -115,8 +115,10 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
 // prev_pending (may be null) reading 0 makes the launch return at once.
 // hub_pull_deg > 0: the hubs of at least that degree first pull round t's
 // candidates, so that the pushes find them settled.
+// hubs_first: the bins' sections dispatched from the hub chunks down to the
+// one-lane rows (the same result).
 void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
-                int *pending, const int *prev_pending, uint64_t hub_pull_deg, hipStream_t s);
+                int *pending, const int *prev_pending, uint64_t hub_pull_deg, hipStream_t s, bool hubs_first = false);
 // deg_inter of hubs from their chunk counts; hub_pos[c] = offset of chunk c in its hub's run
 // (prefix: b.nH + 1 words; scratch: scan scratch).
 void hub_fold(const Bins &b, const uint32_t *hub_inter, uint32_t *deg_inter, uint32_t *hub_pos, uint64_t *prefix,
@@ -187,8 +189,9 @@ void relabel_split(const uint64_t *off, const uint32_t *col, const uint32_t *lab
 // idx[position] order (stable; out's lists and chunks are the caller's
 // buffers, its counts and block offsets b's).  k0, v0, k1, v1: max(nT, nS,
 // nW, nH) words each; scratch: radix_sort_scratch_words of that many.
-void order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &out, uint32_t *k0, uint32_t *v0,
-                         uint32_t *k1, uint32_t *v1, uint64_t *scratch, hipStream_t s);
+// Returns the median idx of the hub chunks' rows (kNone without hubs).
+uint32_t order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &out, uint32_t *k0, uint32_t *v0,
+                             uint32_t *k1, uint32_t *v1, uint64_t *scratch, hipStream_t s);
 // Pieces of the rows of more than smax slots of an owner-grouped CSR: cnt[v]
 // = the row's pieces (rows of at most `split` slots: the whole row, cut every
 // wmax slots, shared by all blocks; longer rows: one per owner group they

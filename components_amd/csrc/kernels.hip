@@ -373,9 +373,18 @@ __global__ __launch_bounds__(kBlock) void k_hub_pull(SweepArgs a, uint32_t t, ui
     atomicMin(reinterpret_cast<unsigned long long *>(a.kout + h.v), static_cast<unsigned long long>(best));
 }
 
-__global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t) {
+// hubs_first: the sections are dispatched in the order hub chunks, wave rows,
+// 16-lane rows, one-lane rows (each section's order kept), so that on a
+// level stored in index order whose hubs have the smallest indices, as R-MAT's
+// do, the pushes with the smallest labels go first.
+__global__ __launch_bounds__(kBlock) void k_round(SweepArgs a, uint32_t t, int hubs_first) {
   if (a.prev_flag && *a.prev_flag == 0) return;
-  const uint32_t blk = blockIdx.x;
+  uint32_t blk = blockIdx.x;
+  if (hubs_first) {
+    const uint32_t nh = a.b.blocks - a.b.bH, nw = a.b.bH - a.b.bW, ns = a.b.bW - a.b.bS;
+    blk = blk < nh ? a.b.bH + blk
+                   : blk < nh + nw ? a.b.bW + (blk - nh) : blk < nh + nw + ns ? a.b.bS + (blk - nh - nw) : blk - nh - nw - ns;
+  }
   const int tid = threadIdx.x;
   bool pending = false;
   if (blk < a.b.bS) {
@@ -2107,7 +2116,7 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
 }
 
 void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_t hop, const Bins &b, uint32_t t,
-                int *pending, const int *prev_pending, uint64_t hub_pull_deg, hipStream_t s) {
+                int *pending, const int *prev_pending, uint64_t hub_pull_deg, hipStream_t s, bool hubs_first) {
   if (b.blocks == 0) return;
   SweepArgs a{off, col, key, key, hop, b, nullptr, nullptr, pending, prev_pending};
   if (hub_pull_deg && b.nH) {
@@ -2115,7 +2124,7 @@ void push_round(const uint64_t *off, const uint32_t *col, uint64_t *key, uint64_
                        dim3(kBlock), 0, s, a, t, hub_pull_deg);
     KC_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_round, dim3(b.blocks), dim3(kBlock), 0, s, a, t);
+  hipLaunchKernelGGL(k_round, dim3(b.blocks), dim3(kBlock), 0, s, a, t, hubs_first ? 1 : 0);
   KC_LAUNCH_CHECK();
 }
 
@@ -2323,8 +2332,9 @@ __global__ void k_hub_permute(uint32_t c, const HubChunk *hubs, const uint32_t *
   if (i < c) out[i] = hubs[order[i]];
 }
 
-void order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &out, uint32_t *k0, uint32_t *v0,
-                         uint32_t *k1, uint32_t *v1, uint64_t *scratch, hipStream_t s) {
+uint32_t order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &out, uint32_t *k0, uint32_t *v0,
+                             uint32_t *k1, uint32_t *v1, uint64_t *scratch, hipStream_t s) {
+  uint32_t median = kNone;
   int bits = 1;
   while (bits < 32 && (1ull << bits) < n) ++bits;
   const uint32_t *in[3] = {b.listT, b.listS, b.listW};
@@ -2347,7 +2357,10 @@ void order_bins_by_index(const Bins &b, const uint32_t *idx, uint64_t n, Bins &o
     hipLaunchKernelGGL(k_hub_permute, dim3(static_cast<unsigned>(grid_for(b.nH))), dim3(kBlock), 0, s, b.nH, b.hubs,
                        w ? v1 : v0, out.hubs);
     KC_LAUNCH_CHECK();
+    KC_HIP(hipMemcpyAsync(&median, (w ? k1 : k0) + b.nH / 2, sizeof(median), hipMemcpyDeviceToHost, s));
+    KC_HIP(hipStreamSynchronize(s));
   }
+  return median;
 }
 
 void count_pieces(uint64_t n, const uint64_t *off, const uint32_t *col, uint32_t smax, uint32_t wmax, uint64_t split,
