@@ -642,11 +642,12 @@ int dedup_buckets(kc_graph *g, uint64_t nb, const uint64_t *seg_off, const uint3
   return first;
 }
 
+// Level 0's cached bins in level-index order (see l0i_bins).
+Bins round0_bins(kc_graph *g, const Bins &b);
+
 // Time-stepped schedule until no vertex lies above the current round, then
 // one counting pass for the cluster-edge counts.  Returns the rounds that did
 // work (the reported "sweeps" of this schedule).
-// Level 0's cached bins in level-index order (see l0i_bins).
-Bins round0_bins(kc_graph *g, const Bins &b);
 
 uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                             uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base,
