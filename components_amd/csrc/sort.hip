@@ -73,16 +73,25 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(uint64_t n, const uint
   const uint32_t lane = lane_id();
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
   const uint32_t count = static_cast<uint32_t>(n - base < static_cast<uint64_t>(kTile) ? n - base : kTile);
+  // the tile's items come into registers first, all loads in flight at once
+  uint32_t kr[kItemsPerThread], vr[kItemsPerThread];
+#pragma unroll
+  for (int r = 0; r < kItemsPerThread; ++r) {
+    const uint32_t i = static_cast<uint32_t>(r) * kBlock + tid;
+    kr[r] = i < count ? keys[base + i] : 0u;
+    vr[r] = i < count ? vals[base + i] : 0u;
+  }
   local_start[tid] = block_exclusive_u32(hist[static_cast<uint64_t>(tid) * tiles + blockIdx.x], wave_tot);
   run[tid] = 0;
   digit_base[tid] = offsets[static_cast<uint64_t>(tid) * tiles + blockIdx.x];
+#pragma unroll
   for (int r = 0; r < kItemsPerThread; ++r) {
     for (int d = lane; d < kDigits; d += kWave) wave_cnt[w][d] = 0;
     __syncthreads();
     const uint32_t i = static_cast<uint32_t>(r) * kBlock + tid;
     const bool valid = i < count;
-    const uint32_t k = valid ? keys[base + i] : 0u;
-    const uint32_t v = valid ? vals[base + i] : 0u;
+    const uint32_t k = kr[r];
+    const uint32_t v = vr[r];
     const uint32_t d = (k >> shift) & (kDigits - 1);
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < kRadixBits; ++b) {
