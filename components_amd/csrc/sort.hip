@@ -52,20 +52,23 @@ __device__ __forceinline__ uint32_t block_exclusive_u32(uint32_t x, uint32_t *wa
   return before + inc - x;
 }
 
-// Scatter pass, staged in LDS: the tile's items are ranked in index order --
-// peers with the same digit inside a wave found with 8 ballots, waves ordered
-// through LDS counts, rounds through running per-digit counts -- and placed
-// in LDS in digit order; then consecutive threads write consecutive items, so
-// each digit's items leave the tile as one contiguous run per array instead
-// of one scattered 4-byte store each.
+// Scatter pass, staged in LDS: every wave owns a contiguous quarter of the
+// tile and ranks its items on its own -- peers with the same digit inside a
+// wave step found with 8 ballots, earlier steps through the wave's running
+// per-digit counts in LDS -- so that one barrier orders the waves (per digit,
+// the counts of the waves before) instead of barriers at every step.  Then
+// the items are placed in LDS in digit order and consecutive threads write
+// consecutive items, so each digit's items leave the tile as one contiguous
+// run per array instead of one scattered 4-byte store each.
+constexpr int kWaveItems = kWave * kItemsPerThread;  // a wave's share of the tile
+
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(uint64_t n, const uint32_t *keys, const uint32_t *vals,
                                                          int shift, const uint32_t *hist, const uint64_t *offsets,
                                                          uint32_t tiles, uint32_t *okeys, uint32_t *ovals) {
   __shared__ uint32_t skey[kTile];
   __shared__ uint32_t sval[kTile];
   __shared__ uint32_t local_start[kDigits];  // first tile position of each digit
-  __shared__ uint32_t run[kDigits];          // items of each digit placed so far
-  __shared__ uint32_t wave_cnt[kWaves][kDigits];
+  __shared__ uint32_t wave_cnt[kWaves][kDigits];  // per wave: running, then exclusive over waves
   __shared__ uint32_t wave_tot[kWaves];
   __shared__ uint64_t digit_base[kDigits];  // output offset of the tile's first item of each digit
   const int tid = threadIdx.x;
@@ -73,45 +76,60 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(uint64_t n, const uint
   const uint32_t lane = lane_id();
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
   const uint32_t count = static_cast<uint32_t>(n - base < static_cast<uint64_t>(kTile) ? n - base : kTile);
-  // the tile's items come into registers first, all loads in flight at once
+  // the wave's items come into registers first, all loads in flight at once:
+  // step r, lane l holds tile item w * kWaveItems + r * kWave + l
+  const uint32_t first = static_cast<uint32_t>(w) * kWaveItems + lane;
   uint32_t kr[kItemsPerThread], vr[kItemsPerThread];
 #pragma unroll
   for (int r = 0; r < kItemsPerThread; ++r) {
-    const uint32_t i = static_cast<uint32_t>(r) * kBlock + tid;
+    const uint32_t i = first + static_cast<uint32_t>(r) * kWave;
     kr[r] = i < count ? keys[base + i] : 0u;
     vr[r] = i < count ? vals[base + i] : 0u;
   }
   local_start[tid] = block_exclusive_u32(hist[static_cast<uint64_t>(tid) * tiles + blockIdx.x], wave_tot);
-  run[tid] = 0;
   digit_base[tid] = offsets[static_cast<uint64_t>(tid) * tiles + blockIdx.x];
+  for (int d = lane; d < kDigits; d += kWave) wave_cnt[w][d] = 0;
+  // (the wave reads only its own counts below, and a wave's LDS accesses
+  // complete in issue order)
+  __builtin_amdgcn_wave_barrier();
+  uint32_t rank[kItemsPerThread];
 #pragma unroll
   for (int r = 0; r < kItemsPerThread; ++r) {
-    for (int d = lane; d < kDigits; d += kWave) wave_cnt[w][d] = 0;
-    __syncthreads();
-    const uint32_t i = static_cast<uint32_t>(r) * kBlock + tid;
-    const bool valid = i < count;
-    const uint32_t k = kr[r];
-    const uint32_t v = vr[r];
-    const uint32_t d = (k >> shift) & (kDigits - 1);
+    const bool valid = first + static_cast<uint32_t>(r) * kWave < count;
+    const uint32_t d = (kr[r] >> shift) & (kDigits - 1);
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < kRadixBits; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lanes_below()));
-    if (valid && rank == 0) wave_cnt[w][d] = static_cast<uint32_t>(__popcll(peers));
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = local_start[d] + run[d] + rank;
-      for (int w2 = 0; w2 < w; ++w2) pos += wave_cnt[w2][d];
-      skey[pos] = k;
-      sval[pos] = v;
-    }
-    __syncthreads();
-    run[tid] += wave_cnt[0][tid] + wave_cnt[1][tid] + wave_cnt[2][tid] + wave_cnt[3][tid];
-    __syncthreads();  // the counts are read before the next round clears them
+    const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below()));
+    const uint32_t before = valid ? wave_cnt[w][d] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    rank[r] = before + below;
+    if (valid && below == 0) wave_cnt[w][d] = before + static_cast<uint32_t>(__popcll(peers));
+    __builtin_amdgcn_wave_barrier();
   }
+  __syncthreads();
+  {  // per digit, the counts of the waves before
+    uint32_t acc = 0;
+    for (int w2 = 0; w2 < kWaves; ++w2) {
+      const uint32_t c = wave_cnt[w2][tid];
+      wave_cnt[w2][tid] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItemsPerThread; ++r) {
+    if (first + static_cast<uint32_t>(r) * kWave < count) {
+      const uint32_t d = (kr[r] >> shift) & (kDigits - 1);
+      const uint32_t pos = local_start[d] + wave_cnt[w][d] + rank[r];
+      skey[pos] = kr[r];
+      sval[pos] = vr[r];
+    }
+  }
+  __syncthreads();
   for (uint32_t j = tid; j < count; j += kBlock) {
     const uint32_t k = skey[j];
     const uint32_t d = (k >> shift) & (kDigits - 1);
