@@ -785,13 +785,34 @@ __global__ __launch_bounds__(kBlock) void k_labels_radius(uint64_t n, const uint
                                                          int with_radius) {
   __shared__ uint32_t wave_max[kBlock / kWave];
   uint32_t r = 0;
-  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
-       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    const uint64_t k = key[v];
-    const uint32_t l = static_cast<uint32_t>(k);
-    lab[v] = l;
-    if (used) used[l] = 1;
-    if (with_radius) r = max(r, static_cast<uint32_t>(k >> 32) - static_cast<uint32_t>(key[pos ? pos[l] : l] >> 32));
+  // four vertices per lane per step, so that their chains of dependent loads
+  // (key, centre position, centre key) are in flight together
+  constexpr int kU = 4;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  for (uint64_t v0 = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v0 < n; v0 += kU * stride) {
+    uint64_t k[kU];
+    uint32_t c[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) k[u] = v0 + u * stride < n ? key[v0 + u * stride] : 0ull;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint64_t v = v0 + u * stride;
+      if (v >= n) continue;
+      const uint32_t l = static_cast<uint32_t>(k[u]);
+      lab[v] = l;
+      if (used) used[l] = 1;
+    }
+    if (with_radius) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const uint32_t l = static_cast<uint32_t>(k[u]);
+        c[u] = v0 + u * stride < n ? (pos ? pos[l] : l) : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (v0 + u * stride < n)
+          r = max(r, static_cast<uint32_t>(k[u] >> 32) - static_cast<uint32_t>(key[c[u]] >> 32));
+    }
   }
   if (with_radius) {
     for (int o = 32; o > 0; o >>= 1) r = max(r, static_cast<uint32_t>(__shfl_xor(r, o)));
@@ -1694,13 +1715,24 @@ __global__ __launch_bounds__(kBlock) void k_gather_listed(uint32_t n_list, Liste
 
 // ---------------------------------------------------------------------------
 // Undo and canonical labels.
-__global__ void k_undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_t *gid,
-                       const uint32_t *rep_up, uint32_t *rep_out) {
-  uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (v >= n) return;
-  const uint32_t a = lab[v];
-  const uint32_t a2 = newidx[a];
-  rep_out[v] = a2 != kNone ? rep_up[a2] : (gid ? gid[a] : a);
+// Grid-stride, four vertices per lane per step, so that their chains of
+// dependent loads (label, next-level index, representative) overlap.
+__global__ __launch_bounds__(kBlock) void k_undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx,
+                                                const uint32_t *gid, const uint32_t *rep_up, uint32_t *rep_out) {
+  constexpr int kU = 4;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  for (uint64_t v0 = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v0 < n; v0 += kU * stride) {
+    uint32_t a[kU], a2[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) a[u] = v0 + u * stride < n ? lab[v0 + u * stride] : 0u;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) a2[u] = v0 + u * stride < n ? newidx[a[u]] : kNone;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint64_t v = v0 + u * stride;
+      if (v < n) rep_out[v] = a2[u] != kNone ? rep_up[a2[u]] : (gid ? gid[a[u]] : a[u]);
+    }
+  }
 }
 
 __global__ void k_rep_from_keys(uint64_t n, const uint64_t *key, const uint32_t *gid, uint32_t *rep) {
@@ -1758,12 +1790,26 @@ constexpr uint32_t kCountBlocks = 2048;
 __global__ __launch_bounds__(kBlock) void k_canon_label(uint64_t n, const uint32_t *rep, const uint32_t *gid,
                                                        const uint32_t *minid, uint32_t *labels, uint32_t *partial) {
   uint32_t roots = 0;
-  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
-       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    const uint32_t id = gid ? gid[v] : static_cast<uint32_t>(v);
-    const uint32_t l = minid[rep[v]];
-    labels[id] = l;
-    roots += l == id;
+  // four vertices per lane per step: their representatives, then the
+  // representatives' minima, in flight together
+  constexpr int kU = 4;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  for (uint64_t v0 = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v0 < n; v0 += kU * stride) {
+    uint32_t r[kU], id[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint64_t v = v0 + u * stride;
+      r[u] = v < n ? rep[v] : 0u;
+      id[u] = v < n ? (gid ? gid[v] : static_cast<uint32_t>(v)) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) r[u] = v0 + u * stride < n ? minid[r[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (v0 + u * stride >= n) continue;
+      labels[id[u]] = r[u];
+      roots += r[u] == id[u];
+    }
   }
   block_sum(roots, partial);
 }
@@ -2508,7 +2554,8 @@ void gather_uniques(uint64_t n_next, const uint64_t *off_next, const uint64_t *s
 void undo(uint64_t n, const uint32_t *lab, const uint32_t *newidx, const uint32_t *gid, const uint32_t *rep_up,
           uint32_t *rep_out, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_undo, dim3(grid_for(n)), dim3(kBlock), 0, s, n, lab, newidx, gid, rep_up, rep_out);
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, 4 * kBlock), 1u << 16));
+  hipLaunchKernelGGL(k_undo, dim3(g), dim3(kBlock), 0, s, n, lab, newidx, gid, rep_up, rep_out);
   KC_LAUNCH_CHECK();
 }
 
