@@ -493,8 +493,11 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
       // the first sweep of a batch always runs: the previous batch ended with a change
       const int *prev = i == 0 ? nullptr : &g->flags[f - 1];
       slots[i] = g->timer.start(KC_K_SWEEP);
+      // hub chunks first, then wave, 16-lane and one-lane rows: the many
+      // short rows then read the keys the hubs lowered in the same sweep
+      // (R-MAT 24, sweeps schedule: 11.8 -> 11.0 sweeps, 24.7 -> 21.9 ms)
       sweep(block ? kSweepBlock : kSweepInPlace, count, off, col, key, key, hop, b, g->deg_inter, g->hub_inter,
-            &g->flags[f], prev, g->stream, block);
+            &g->flags[f], prev, g->stream, block, true);
       g->timer.stop(slots[i], bytes);
       if (with_jump) {
         TimedScope js(g->timer, KC_K_JUMP, 16 * n);

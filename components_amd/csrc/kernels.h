@@ -109,7 +109,7 @@ uint64_t bin_scratch_words(uint64_t n);
 // null): when it reads 0 the launch returns at once (converged earlier).
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin,
            uint64_t *kout, uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter,
-           int *flag, const int *prev_flag, hipStream_t s, uint64_t block = 0);
+           int *flag, const int *prev_flag, hipStream_t s, uint64_t block = 0, bool hubs_first = false);
 // Round t of the time-stepped schedule: vertices at key distance t push
 // key + hop; *pending = 1 while a vertex is still above distance t;
 // prev_pending (may be null) reading 0 makes the launch return at once.

@@ -205,6 +205,7 @@ struct SweepArgs {
   int *flag;
   const int *prev_flag;
   uint64_t block = 0;  // kSweepBlock: indices per block
+  int hubs_first = 0;  // k_sweep: sections dispatched from the hub chunks down (as k_round's)
 };
 
 // Edges whose gathers one lane keeps in flight together in the row loops.
@@ -274,7 +275,12 @@ __device__ __forceinline__ bool sweep_group(const SweepArgs &a, bool active, uin
 template <int MODE, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
   if (a.prev_flag && *a.prev_flag == 0) return;
-  const uint32_t blk = blockIdx.x;
+  uint32_t blk = blockIdx.x;
+  if (a.hubs_first) {
+    const uint32_t nh = a.b.blocks - a.b.bH, nw = a.b.bH - a.b.bW, ns = a.b.bW - a.b.bS;
+    blk = blk < nh ? a.b.bH + blk
+                   : blk < nh + nw ? a.b.bW + (blk - nh) : blk < nh + nw + ns ? a.b.bS + (blk - nh - nw) : blk - nh - nw - ns;
+  }
   const int tid = threadIdx.x;
   bool changed = false;
   if (blk < a.b.bS) {
@@ -2144,9 +2150,9 @@ void finish_bins(Bins &b) {
 
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin, uint64_t *kout,
            uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter, int *flag, const int *prev_flag,
-           hipStream_t s, uint64_t block) {
+           hipStream_t s, uint64_t block, bool hubs_first) {
   if (b.blocks == 0) return;
-  SweepArgs a{off, col, kin, kout, hop, b, deg_inter, hub_inter, flag, prev_flag, block};
+  SweepArgs a{off, col, kin, kout, hop, b, deg_inter, hub_inter, flag, prev_flag, block, hubs_first ? 1 : 0};
   dim3 g(b.blocks), t(kBlock);
   if (mode == kSweepInPlace && count)
     hipLaunchKernelGGL((k_sweep<kSweepInPlace, true>), g, t, 0, s, a);
