@@ -487,7 +487,9 @@ def test_owner_split_relabel_matches_one_pass():
                 assert np.array_equal(lab, ref) and np.array_equal(lab1, ref), (bins, split, bfs)
                 assert len(tr) == len(tr1) >= 2, (bins, split, bfs)
                 for lvl, ((info, keys, gid), (info1, keys1, gid1)) in enumerate(zip(tr, tr1)):
-                    assert tuple(info) == tuple(info1), (bins, split, bfs, lvl)
+                    # (sizes and radius: the count of in-place pull sweeps, the tail sweeps of the
+                    # rounds included, depends on the order in which the device runs the rows)
+                    assert (info[0], info[1], info[3]) == (info1[0], info1[1], info1[3]), (bins, split, bfs, lvl)
                     assert np.array_equal(keys, keys1) and np.array_equal(gid, gid1), (bins, split, bfs, lvl)
     finally:
         dg.close()
