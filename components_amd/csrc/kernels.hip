@@ -835,6 +835,7 @@ __global__ __launch_bounds__(kBlock) void k_labels_radius(uint64_t n, const uint
 // Generic three-phase exclusive scan over functor-defined values.
 constexpr int kScanPerThread = 8;
 constexpr int kScanTile = kBlock * kScanPerThread;
+constexpr uint64_t kScanFusedBlocks = 1024;  // two-launch scans up to this many tiles
 
 struct Pair64 {
   uint64_t a, b;
@@ -879,17 +880,44 @@ __device__ __forceinline__ uint64_t shfl_up_t(uint64_t x, int o) {
 }
 __device__ __forceinline__ Pair64 shfl_up_t(Pair64 x, int o) { return {shfl_up_t(x.a, o), shfl_up_t(x.b, o)}; }
 
+// Sum of one value per thread over the block (wave scans, then the wave sums
+// through LDS); every thread receives it.
+template <class T>
+__device__ T block_total(T v, T *wsum) {
+  const uint32_t ln = lane_id(), wv = threadIdx.x / kWave;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const T t = shfl_up_t(v, o);
+    if (ln >= static_cast<uint32_t>(o)) v = v + t;
+  }
+  if (ln == kWave - 1) wsum[wv] = v;
+  __syncthreads();
+  T tot{};
+  for (uint32_t w = 0; w < kBlock / kWave; ++w) tot = tot + wsum[w];
+  __syncthreads();
+  return tot;
+}
+
 // A tile is kScanPerThread rounds of kBlock consecutive items (lane i of a
 // round takes item i, so loads and stores are coalesced); each round is one
 // block-wide scan (wave scans, then the wave sums through LDS), carried into
-// the next.
+// the next.  block_prefix: the exclusive prefix of every block's tile; or
+// block_sums (block_prefix null): every block's tile sum, the block adding up
+// those before its own (for grids of up to kScanFusedBlocks blocks, in place
+// of a scan launch over the sums).
 template <class T, class Load, class Store>
 __global__ __launch_bounds__(kBlock) void k_scan_down(uint64_t n, Load load, Store store, const T *block_prefix,
-                                                     T *total_out) {
+                                                     T *total_out, const T *block_sums = nullptr) {
   __shared__ T wsum[kBlock / kWave];
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
   const uint32_t ln = lane_id(), wv = threadIdx.x / kWave;
-  T run = block_prefix ? block_prefix[blockIdx.x] : T{};
+  T run{};
+  if (block_prefix) {
+    run = block_prefix[blockIdx.x];
+  } else if (block_sums) {
+    T acc{};
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kBlock) acc = acc + block_sums[b];
+    run = block_total<T>(acc, wsum);
+  }
   for (int j = 0; j < kScanPerThread; ++j) {
     const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
     const T v = i < n ? load(i) : T{};
@@ -963,6 +991,13 @@ void scan_exclusive(uint64_t n, Load load, Store store, T *total, uint64_t *scra
   hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n, load,
                      sums);
   KC_LAUNCH_CHECK();
+  if (blocks <= kScanFusedBlocks) {
+    // each block adds up the tile sums before its own: two launches, not three
+    hipLaunchKernelGGL((k_scan_down<T, Load, Store>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n,
+                       load, store, static_cast<const T *>(nullptr), total, static_cast<const T *>(sums));
+    KC_LAUNCH_CHECK();
+    return;
+  }
   scan_exclusive<T>(blocks, ArrayLoad<T>{sums}, ArrayStore<T>{prefix}, total, rest, s);
   hipLaunchKernelGGL((k_scan_down<T, Load, Store>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, n,
                      load, store, static_cast<const T *>(prefix), static_cast<T *>(nullptr));
@@ -2314,10 +2349,16 @@ void segments(uint64_t N, const uint32_t *sk, const uint32_t *sv, const uint32_t
     hipLaunchKernelGGL((k_scan_reduce<Pair64, SegLoad>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, N,
                        SegLoad{sk, sv, deg_inter, cache}, sums);
     KC_LAUNCH_CHECK();
-    scan_exclusive<Pair64>(blocks, ArrayLoad<Pair64>{sums}, ArrayStore<Pair64>{prefix}, total, rest, s);
-    hipLaunchKernelGGL((k_scan_down<Pair64, SegLoadCached, SegStore>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
-                       0, s, N, SegLoadCached{sk, cache}, store, static_cast<const Pair64 *>(prefix),
-                       static_cast<Pair64 *>(nullptr));
+    if (blocks <= kScanFusedBlocks) {
+      hipLaunchKernelGGL((k_scan_down<Pair64, SegLoadCached, SegStore>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(kBlock), 0, s, N, SegLoadCached{sk, cache}, store, static_cast<const Pair64 *>(nullptr),
+                         total, static_cast<const Pair64 *>(sums));
+    } else {
+      scan_exclusive<Pair64>(blocks, ArrayLoad<Pair64>{sums}, ArrayStore<Pair64>{prefix}, total, rest, s);
+      hipLaunchKernelGGL((k_scan_down<Pair64, SegLoadCached, SegStore>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(kBlock), 0, s, N, SegLoadCached{sk, cache}, store, static_cast<const Pair64 *>(prefix),
+                         static_cast<Pair64 *>(nullptr));
+    }
     KC_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_seg_finish, dim3(1), dim3(1), 0, s, total, seg_off, sizes);
