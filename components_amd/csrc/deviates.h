@@ -79,14 +79,16 @@ KC_HD inline double canonical_draw(uint32_t engine_seed, bool fused) {
 // fall geometrically (entry k - 1 is about table[0]^k), so the count is about
 // log(w) / log(table[0]); the search starts there and steps to the exact
 // boundary, which takes a probe or two instead of a binary search's
-// log2(k_max) dependent loads.
+// log2(k_max) dependent loads.  The comparisons alone fix the result, so the
+// start is taken in single precision (a guess off by one costs a probe).
 KC_HD inline uint32_t deviate_from_table(double w, const double *table, uint32_t k_max) {
   if (k_max == 0) return 0;
   const double t0 = table[0];
   uint32_t k = 0;
   if (w <= t0 && t0 < 1.0) {
-    const double g = log(w) / log(t0);
-    k = g >= static_cast<double>(k_max) ? k_max : static_cast<uint32_t>(g);
+    // (t0 within a float ulp of 1 gives an infinite or undefined quotient: start at 0)
+    const float g = logf(static_cast<float>(w)) / logf(static_cast<float>(t0));
+    k = g > 0.f ? (g < static_cast<float>(k_max) ? static_cast<uint32_t>(g) : k_max) : 0u;
   }
   while (k > 0 && table[k - 1] < w) --k;
   while (k < k_max && table[k] >= w) ++k;

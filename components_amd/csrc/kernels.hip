@@ -2304,9 +2304,23 @@ __global__ void k_rank_fix(uint64_t n, const uint32_t *used, uint32_t *rank, con
   if (a == 0) *count = total->a + total->b;
 }
 
-__global__ void k_cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid) {
-  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (v < n) cid[v] = rank[lab[v]];
+// Grid-stride, four vertices per lane per step (their label and rank loads
+// in flight together).
+__global__ __launch_bounds__(kBlock) void k_cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank,
+                                                       uint32_t *cid) {
+  constexpr int kU = 4;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  for (uint64_t v0 = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v0 < n; v0 += kU * stride) {
+    uint32_t l[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) l[u] = v0 + u * stride < n ? lab[v0 + u * stride] : 0u;
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (v0 + u * stride < n) l[u] = rank[l[u]];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (v0 + u * stride < n) cid[v0 + u * stride] = l[u];
+  }
 }
 
 void cluster_ranks_hot(uint64_t n, uint64_t k, const uint32_t *lab, uint32_t *used, uint32_t *rank, uint64_t *count,
@@ -2325,7 +2339,8 @@ void cluster_ranks_hot(uint64_t n, uint64_t k, const uint32_t *lab, uint32_t *us
 
 void cluster_ids(uint64_t n, const uint32_t *lab, const uint32_t *rank, uint32_t *cid, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_cluster_ids, dim3(grid_for(n)), dim3(kBlock), 0, s, n, lab, rank, cid);
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, 4 * kBlock), 1u << 16));
+  hipLaunchKernelGGL(k_cluster_ids, dim3(g), dim3(kBlock), 0, s, n, lab, rank, cid);
   KC_LAUNCH_CHECK();
 }
 
