@@ -382,7 +382,9 @@ constexpr uint64_t kSplitRowsAbove = 4096;
 // times, the hyperbolic graph of 2^24 vertices: 23; G(n,m) 2^20, the 3D grid
 // 256^3 and the 2D geometric graph of 2^24 vertices: 1.8 to 2.6)
 constexpr double kHotShareFactor = 4.0;
-constexpr int kTailShift = 5;  // -bfs: pull sweeps finish a level once at most n/32 rows are pending
+// -bfs: pull sweeps finish a level once at most n/8 rows are pending (3D grid 256^3: 7.01 -> 6.90 ms
+// against n/32; n/4 took 7.22, R-MAT 24 and the others alike)
+constexpr int kTailShift = 3;
 constexpr uint64_t kPullRoundsMin = 1u << 16;  // -bfs: levels at least this large choose round directions
 constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulled
 constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average degree or more (sparse,
