@@ -382,6 +382,7 @@ constexpr uint64_t kSplitRowsAbove = 4096;
 // times, the hyperbolic graph of 2^24 vertices: 23; G(n,m) 2^20, the 3D grid
 // 256^3 and the 2D geometric graph of 2^24 vertices: 1.8 to 2.6)
 constexpr double kHotShareFactor = 4.0;
+constexpr uint64_t kLevelHotRows = 28672;  // labels the one-pass relabel serves from LDS (32-bit)
 // -bfs: pull sweeps finish a level once at most n/8 rows are pending (3D grid 256^3: 7.01 -> 6.90 ms
 // against n/32; n/4 took 7.22, R-MAT 24 and the others alike)
 constexpr int kTailShift = 3;
@@ -958,6 +959,11 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
   uint32_t *top_rep = nullptr;
   Bins b;
   bool bins_valid = false;  // bins of the current level graph (kept across stall levels)
+  // a contracted level whose first kLevelHotRows vertices hold at least
+  // kHotShareFactor times their proportional share of its edge slots: its
+  // relabel keeps their labels in LDS (R-MAT 24 level 1, whose low indices
+  // are the hub clusters; not the grids)
+  bool level_hot = false;
 
   while (n_l > 0) {
     // a pre-contraction level comes before the cut-off, as the local phase
@@ -1163,7 +1169,7 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         relabel_split(off_l, col_l, g->cid, n_l, b, g->l0_pieces, g->l0_part_list, g->l0_part_at, relab,
                       g->deg_inter, r_inter, n_l, g->stream);
       else
-        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l ? n_l : 0, g->stream, hot_ids);
+        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l || level_hot ? n_l : 0, g->stream, hot_ids);
     }
     if (rb.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * rb.nH + 4ull * rb.nH);
@@ -1231,8 +1237,12 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
       csr_offsets(n_next, g->uniq, off_next, g->sizes, g->scan_scratch, g->stream);
     }
     d2h(g, g->h_sizes + 2, g->sizes + 2, sizeof(uint64_t));
+    const uint64_t hot_rows = std::min<uint64_t>(n_next, kLevelHotRows);
+    d2h(g, g->h_sizes + 15, off_next + hot_rows, sizeof(uint64_t));
     sync(g);
     const uint64_t m2_next = g->h_sizes[2];
+    level_hot = m2_next > 0 && static_cast<double>(g->h_sizes[15]) * n_next >=
+                                   kHotShareFactor * static_cast<double>(hot_rows) * m2_next;
     // dedup: every bucket slot read once, the unique ones written, per
     // cluster its bucket bounds and its unique count
     g->timer.retag(dedup_slot, KC_K_DEDUP, 4 * P + 4 * m2_next + 8 * (n_next + 1) + 4 * n_next);
