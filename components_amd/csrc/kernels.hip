@@ -1328,14 +1328,14 @@ __global__ __launch_bounds__(B) void k_dedup_medium(uint32_t n_medium, const uin
     __syncthreads();
     for (uint32_t j = tid; j < s; j += B) lds_insert(tab, mask, bucket[base + j]);
     __syncthreads();
-    // each thread owns a contiguous range of slots
-    const uint32_t per = tsz / B > 0 ? tsz / B : 1;
-    const uint32_t j0 = tid * per;
+    // each thread owns the slots tid, tid + B, ... (consecutive lanes read
+    // consecutive words: no LDS bank conflicts) and writes its kept values as
+    // one run, the runs in thread order
     uint32_t c = 0;
-    for (uint32_t j = j0; j < j0 + per && j < tsz; ++j) c += tab[j] != kNone;
+    for (uint32_t j = tid; j < tsz; j += B) c += tab[j] != kNone;
     uint32_t tot;
     uint32_t p = block_scan_waves<B>(c, wsum, tot);
-    for (uint32_t j = j0; j < j0 + per && j < tsz; ++j)
+    for (uint32_t j = tid; j < tsz; j += B)
       if (tab[j] != kNone) out[base + p++] = tab[j];
     if (tid == 0) uniq[a2] = tot;
     __syncthreads();
