@@ -264,7 +264,7 @@ struct kc_graph {
   uint64_t *d_huge_info = nullptr;
   uint64_t huge_info_cap = 0;
   int *flags = nullptr;
-  uint32_t *small = nullptr;  // [0..3] bin counters, [4] radius, [5] roots
+  uint32_t *small = nullptr;  // [0..3] bin counters, [4] radius, [5] roots, [6] hubs found, [7] packed keys do not fit
   uint64_t *sizes = nullptr;  // [0] n_next [1] P [2] m2_next [3..5] dedup counters [8] N [9] clusters
   // pinned host mirrors
   uint64_t *h_sizes = nullptr;
@@ -386,6 +386,7 @@ constexpr uint64_t kLevelHotRows = 28672;  // labels the one-pass relabel serves
 // -bfs: pull sweeps finish a level once at most n/8 rows are pending (3D grid 256^3: 7.01 -> 6.90 ms
 // against n/32; n/4 took 7.22, R-MAT 24 and the others alike)
 constexpr int kTailShift = 3;
+constexpr uint64_t kPackedSweepsMin = 1u << 16;  // levels at least this large sweep packed keys
 constexpr uint64_t kPullRoundsMin = 1u << 16;  // -bfs: levels at least this large choose round directions
 constexpr uint32_t kDirectionRounds = 6;        // rounds 1 .. this may be pulled
 constexpr uint64_t kDirectionMinDegree = 8;     // ... on levels of this average degree or more (sparse,
@@ -479,12 +480,16 @@ uint64_t sweep_bytes(uint64_t n, uint64_t m2, bool count) {
 // Sweeps are launched in speculative batches; a sweep whose predecessor
 // changed nothing returns at once.  Returns the number of sweeps that did work
 // (the last of them found no change).
+// packed (non-null: plain in-place sweeps without counts or jumps): the
+// sweeps run over these packed keys (pack_keys) instead of key.
 uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                            uint64_t *key, uint64_t hop, const Bins &b, bool count, bool with_jump,
-                           uint32_t &flag_base, bool book_bytes = true, uint64_t block = 0) {
+                           uint32_t &flag_base, bool book_bytes = true, uint64_t block = 0,
+                           uint32_t *packed = nullptr) {
   uint64_t done = 0;
   const uint32_t batch = n < (1u << 16) ? 8u : 3u;
-  const uint64_t bytes = book_bytes ? sweep_bytes(n, m2, count) : 0;
+  // (packed: 4-byte keys for the neighbours and the row's own)
+  const uint64_t bytes = !book_bytes ? 0 : packed ? 8 * (n + 1) + 8 * m2 + 8 * n : sweep_bytes(n, m2, count);
   while (true) {
     if (flag_base + batch + 2 >= kMaxFlags) {
       fill_u32(reinterpret_cast<uint32_t *>(g->flags), kMaxFlags, 0, g->stream);
@@ -499,8 +504,11 @@ uint64_t sweep_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t 
       // hub chunks first, then wave, 16-lane and one-lane rows: the many
       // short rows then read the keys the hubs lowered in the same sweep
       // (R-MAT 24, sweeps schedule: 11.8 -> 11.0 sweeps, 24.7 -> 21.9 ms)
-      sweep(block ? kSweepBlock : kSweepInPlace, count, off, col, key, key, hop, b, g->deg_inter, g->hub_inter,
-            &g->flags[f], prev, g->stream, block, true);
+      if (packed)
+        sweep_packed(off, col, packed, b, &g->flags[f], prev, g->stream, true);
+      else
+        sweep(block ? kSweepBlock : kSweepInPlace, count, off, col, key, key, hop, b, g->deg_inter, g->hub_inter,
+              &g->flags[f], prev, g->stream, block, true);
       g->timer.stop(slots[i], bytes);
       if (with_jump) {
         TimedScope js(g->timer, KC_K_JUMP, 16 * n);
@@ -1067,7 +1075,20 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
                                     o->tail_sweeps < 0 ? -1 : o->tail_sweeps == 0 ? kTailShift : o->tail_sweeps,
                                     choose, pos_l != nullptr && off_l == g->off0p && g->l0_hot_skewed);
       } else {
-        sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base);
+        // Levels below 2^24 vertices whose key distances stay below 255 sweep
+        // packed 32-bit keys (pack_keys; the buffer of the cluster ranks is
+        // free until they are built).  A distance only falls, so a key plus
+        // one hop stays within 32 bits.
+        uint32_t *pk = n_l >= kPackedSweepsMin && n_l <= (1ull << kPackedLabelBits) ? g->crank : nullptr;
+        if (pk) {
+          pack_keys(n_l, g->key, pk, g->small + 7, g->stream);
+          d2h(g, g->h_small + 7, g->small + 7, sizeof(uint32_t));
+          sync(g);
+          if (g->h_small[7]) pk = nullptr;
+        }
+        sweeps = sweep_to_fixpoint(g, n_l, m2_l, off_l, col_l, g->key, kOneHop, b, false, false, flag_base, true, 0,
+                                   pk);
+        if (pk) unpack_keys(n_l, pk, g->key, g->stream);
       }
     }
     if (!pre && !local && !o->use_bfs && sweeps == 1) {

@@ -110,6 +110,17 @@ uint64_t bin_scratch_words(uint64_t n);
 void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const uint64_t *kin,
            uint64_t *kout, uint64_t hop, const Bins &b, uint32_t *deg_inter, uint32_t *hub_inter,
            int *flag, const int *prev_flag, hipStream_t s, uint64_t block = 0, bool hubs_first = false);
+// Packed keys (distance << 24 | label) for the pull sweeps of levels below
+// 2^24 vertices: pack_keys writes them and sets *bad (zeroed first) when a
+// distance exceeds kPackedMaxDist - 1 or a label 24 bits; sweep_packed is
+// sweep(kSweepInPlace, no counts) over them; unpack_keys restores the keys.
+constexpr uint32_t kPackedLabelBits = 24;
+constexpr uint32_t kPackedHop = 1u << kPackedLabelBits;
+constexpr uint32_t kPackedMaxDist = 255;
+void pack_keys(uint64_t n, const uint64_t *key, uint32_t *pk, uint32_t *bad, hipStream_t s);
+void unpack_keys(uint64_t n, const uint32_t *pk, uint64_t *key, hipStream_t s);
+void sweep_packed(const uint64_t *off, const uint32_t *col, uint32_t *pk, const Bins &b, int *flag,
+                  const int *prev_flag, hipStream_t s, bool hubs_first);
 // Round t of the time-stepped schedule: vertices at key distance t push
 // key + hop; *pending = 1 while a vertex is still above distance t;
 // prev_pending (may be null) reading 0 makes the launch return at once.

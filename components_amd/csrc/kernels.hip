@@ -312,6 +312,110 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
   }
 }
 
+// Packed keys for the pull sweeps of levels below 2^24 vertices whose key
+// distances stay below 255: key (distance, label) as distance << 24 | label,
+// the same order in half the bytes, so that a 128-byte line brings 32 keys to
+// the gathers instead of 16.  *bad receives a nonzero value when a key does
+// not fit (the caller then sweeps the 64-bit keys).
+__global__ __launch_bounds__(kBlock) void k_pack_keys(uint64_t n, const uint64_t *key, uint32_t *pk, uint32_t *bad) {
+  bool over = false;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint64_t k = key[v];
+    const uint32_t d = static_cast<uint32_t>(k >> 32), l = static_cast<uint32_t>(k);
+    // a neighbour's key plus one hop must fit as well: distances up to 254
+    const bool fits = d < kPackedMaxDist && l < (1u << kPackedLabelBits);
+    over |= !fits;
+    pk[v] = fits ? (d << kPackedLabelBits) | l : ~0u;
+  }
+  if (__ballot(over) != 0 && lane_id() == 0) *bad = 1;
+}
+
+__global__ void k_unpack_keys(uint64_t n, const uint32_t *pk, uint64_t *key) {
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t k = pk[v];
+  key[v] = (static_cast<uint64_t>(k >> kPackedLabelBits) << 32) | (k & ((1u << kPackedLabelBits) - 1u));
+}
+
+// The in-place pull sweep over packed keys (k_sweep's kSweepInPlace without
+// counts): the same sections, section order and kGather loads in flight.
+struct PackedSweepArgs {
+  const uint64_t *off;
+  const uint32_t *col;
+  uint32_t *key;
+  Bins b;
+  int *flag;
+  const int *prev_flag;
+  int hubs_first;
+};
+
+template <int W>
+__device__ __forceinline__ bool packed_group(const PackedSweepArgs &a, bool active, uint32_t v, uint64_t e0,
+                                             uint64_t e1, int gl, bool hub) {
+  const uint32_t kv = active ? a.key[v] : ~0u;
+  uint32_t best = kv;
+  if (active) {
+    uint64_t e = e0 + gl;
+    for (; e + (kGather - 1) * W < e1; e += kGather * W) {
+      uint32_t c[kGather], k[kGather];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) c[j] = a.col[e + j * W];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) k[j] = a.key[c[j]];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) best = min(best, k[j] + kPackedHop);
+    }
+    for (; e < e1; e += W) best = min(best, a.key[a.col[e]] + kPackedHop);
+  }
+  if (W > 1)
+    for (int o = W >> 1; o > 0; o >>= 1) best = min(best, static_cast<uint32_t>(__shfl_xor(best, o, W)));
+  bool changed = false;
+  if (active && gl == 0 && best < kv) {
+    if (hub) {
+      changed = atomicMin(&a.key[v], best) > best;
+    } else {
+      a.key[v] = best;
+      changed = true;
+    }
+  }
+  return changed;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sweep_packed(PackedSweepArgs a) {
+  if (a.prev_flag && *a.prev_flag == 0) return;
+  uint32_t blk = blockIdx.x;
+  if (a.hubs_first) {
+    const uint32_t nh = a.b.blocks - a.b.bH, nw = a.b.bH - a.b.bW, ns = a.b.bW - a.b.bS;
+    blk = blk < nh ? a.b.bH + blk
+                   : blk < nh + nw ? a.b.bW + (blk - nh) : blk < nh + nw + ns ? a.b.bS + (blk - nh - nw) : blk - nh - nw - ns;
+  }
+  const int tid = threadIdx.x;
+  bool changed = false;
+  if (blk < a.b.bS) {
+    const uint32_t i = blk * kBlock + tid;
+    const bool act = i < a.b.nT;
+    const uint32_t v = act ? a.b.listT[i] : 0;
+    changed = packed_group<1>(a, act, v, act ? a.off[v] : 0, act ? a.off[v + 1] : 0, 0, false);
+  } else if (blk < a.b.bW) {
+    const uint32_t i = (blk - a.b.bS) * (kBlock / 16) + tid / 16;
+    const bool act = i < a.b.nS;
+    const uint32_t v = act ? a.b.listS[i] : 0;
+    changed = packed_group<16>(a, act, v, act ? a.off[v] : 0, act ? a.off[v + 1] : 0, tid & 15, false);
+  } else if (blk < a.b.bH) {
+    const uint32_t i = (blk - a.b.bW) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nW;
+    const uint32_t v = act ? a.b.listW[i] : 0;
+    changed = packed_group<kWave>(a, act, v, act ? a.off[v] : 0, act ? a.off[v + 1] : 0, tid & 63, false);
+  } else {
+    const uint32_t i = (blk - a.b.bH) * (kBlock / kWave) + tid / kWave;
+    const bool act = i < a.b.nH;
+    const HubChunk h = act ? a.b.hubs[i] : HubChunk{0, 0, 0, 0};
+    changed = packed_group<kWave>(a, act, h.v, h.e0, h.e1, tid & 63, true);
+  }
+  if (__ballot(changed) != 0 && lane_id() == 0) *a.flag = 1;
+}
+
 // Sums the chunk counts of every hub into deg_inter and records each chunk's
 // offset inside its hub's run of cluster edges.
 // ---------------------------------------------------------------------------
@@ -2199,6 +2303,28 @@ void sweep(int mode, bool count, const uint64_t *off, const uint32_t *col, const
     hipLaunchKernelGGL((k_sweep<kSweepBlock, false>), g, t, 0, s, a);
   else
     hipLaunchKernelGGL((k_sweep<kSweepCountOnly, true>), g, t, 0, s, a);
+  KC_LAUNCH_CHECK();
+}
+
+void pack_keys(uint64_t n, const uint64_t *key, uint32_t *pk, uint32_t *bad, hipStream_t s) {
+  fill_u32(bad, 1, 0u, s);
+  if (!n) return;
+  const unsigned g = static_cast<unsigned>(std::min<uint64_t>(ceil_div(n, kBlock), 4096));
+  hipLaunchKernelGGL(k_pack_keys, dim3(g), dim3(kBlock), 0, s, n, key, pk, bad);
+  KC_LAUNCH_CHECK();
+}
+
+void unpack_keys(uint64_t n, const uint32_t *pk, uint64_t *key, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_unpack_keys, dim3(static_cast<unsigned>(grid_for(n))), dim3(kBlock), 0, s, n, pk, key);
+  KC_LAUNCH_CHECK();
+}
+
+void sweep_packed(const uint64_t *off, const uint32_t *col, uint32_t *pk, const Bins &b, int *flag,
+                  const int *prev_flag, hipStream_t s, bool hubs_first) {
+  if (b.blocks == 0) return;
+  PackedSweepArgs a{off, col, pk, b, flag, prev_flag, hubs_first ? 1 : 0};
+  hipLaunchKernelGGL(k_sweep_packed, dim3(b.blocks), dim3(kBlock), 0, s, a);
   KC_LAUNCH_CHECK();
 }
 

@@ -495,6 +495,39 @@ def test_owner_split_relabel_matches_one_pass():
         dg.close()
 
 
+@pytest.mark.parametrize("beta", [0.1, 0.02])
+@pytest.mark.parametrize("case", ["rmat", "grid2"])
+def test_packed_sweeps_trace_matches_oracle(case, beta):
+    """Pull sweeps (the default schedule) on levels of at least 2^16 vertices run over packed 32-bit
+    keys (distance << 24 | label) when every distance stays below 255; at beta 0.02 the deviates
+    exceed that and the sweeps keep the 64-bit keys.  Both give the oracle's keys, ids, sizes and
+    radii at every level, and its labels."""
+    if case == "rmat":
+        g = ca.generate("rmat", n=1 << 17, m=1 << 20, seed=5)
+    else:
+        g = ca.generate("grid_2d", nx=400, ny=400, p=0.6, seed=3)
+    assert int((np.diff(g.offsets.astype(np.int64)) > 0).sum()) >= 1 << 16
+    seq = 1024
+    dg = ca.DeviceGraph(g)
+    try:
+        lab, st = dg.find_components(seed=21, beta=beta, stream=ca.STREAM_HASHED, use_bfs=False, seq_limit=seq,
+                                     record_trace=True)
+        tr = dg.trace()
+    finally:
+        dg.close()
+    run = oracle.Run(g.offsets, g.targets, variant="exp", seed=21, beta=beta, seq_limit=seq, stream=ca.STREAM_HASHED,
+                     keep_keys=True)
+    want = run.levels()
+    print(f"{case} beta {beta}: levels {len(want)}, sweeps {st['sweeps']}", flush=True)
+    assert len(tr) == len(want) >= 1
+    for lvl, ((info, keys, gid), winfo) in enumerate(zip(tr, want)):
+        assert info[0] == winfo[0] and info[1] == winfo[1] and info[3] == winfo[3], (case, beta, lvl)
+        wkeys, wgid = run.level_keys(lvl)
+        assert np.array_equal(gid, wgid), (case, beta, lvl)
+        assert np.array_equal(keys, wkeys), (case, beta, lvl)
+    assert np.array_equal(lab, oracle.canonical_labels(g.offsets, g.targets))
+
+
 def test_option_checks_and_timed_class():
     g = ca.generate("rmat", n=1 << 16, m=1 << 20, seed=4)
     ref = oracle.canonical_labels(g.offsets, g.targets)
