@@ -662,7 +662,6 @@ Bins round0_bins(kc_graph *g, const Bins &b);
 // Time-stepped schedule until no vertex lies above the current round, then
 // one counting pass for the cluster-edge counts.  Returns the rounds that did
 // work (the reported "sweeps" of this schedule).
-
 uint64_t rounds_to_fixpoint(kc_graph *g, uint64_t n, uint64_t m2, const uint64_t *off, const uint32_t *col,
                             uint64_t *key, const Bins &level_bins, const BinCfg &cfg, uint32_t &flag_base,
                             int tail_shift, bool choose_direction, bool index_round0) {
@@ -1190,7 +1189,8 @@ void run_contraction(kc_graph *g, const kc_options *o, RunOut &out) {
         relabel_split(off_l, col_l, g->cid, n_l, b, g->l0_pieces, g->l0_part_list, g->l0_part_at, relab,
                       g->deg_inter, r_inter, n_l, g->stream);
       else
-        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l || level_hot ? n_l : 0, g->stream, hot_ids);
+        relabel(off_l, col_l, g->cid, b, relab, g->deg_inter, g->hub_inter, pos_l || level_hot ? n_l : 0,
+                g->stream, hot_ids);
     }
     if (rb.nH) {
       TimedScope ts(g->timer, KC_K_COUNT, 12ull * rb.nH + 4ull * rb.nH);
